@@ -1,0 +1,119 @@
+"""ctypes binding of libimpeller_hip.so (the C ABI in include/impeller_hip.h).
+
+The library is required: importing a function from here on a machine where
+the shared object is missing raises, there is no CPU fallback.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libimpeller_hip.so")
+
+
+class HipError(RuntimeError):
+    def __init__(self, fn, code):
+        super().__init__(f"{fn} returned hipError {code}")
+        self.code = code
+
+
+class PolyPtr(ctypes.Structure):
+    """core/src/poly_ptr.rs:4-11 / core/src/common/iter/src/iter.cuh:12-18"""
+    _fields_ = [("ptr", ctypes.c_void_p), ("len", ctypes.c_size_t),
+                ("rotate", ctypes.c_size_t), ("offset", ctypes.c_size_t),
+                ("whole_len", ctypes.c_size_t)]
+
+
+ConstPolyPtr = PolyPtr  # identical layout (iter.cuh:20-26)
+
+
+class PolySlice(ctypes.Structure):
+    """common/src/typ.rs:13-40 / core/src/common/slice/poly_slice.h:3-13"""
+    _fields_ = [("begin", ctypes.c_ulonglong), ("slice_len", ctypes.c_ulonglong),
+                ("len", ctypes.c_ulonglong)]
+
+
+_P = ctypes.c_void_p
+_U32 = ctypes.c_uint
+_U64 = ctypes.c_ulonglong
+_INT = ctypes.c_int
+_ERR = ctypes.c_int
+
+# name -> (restype, argtypes); every symbol include/impeller_hip.h declares
+SIGNATURES = {
+    "ssip_ntt": (_ERR, [PolyPtr, _P, _U32, _P, _U32, _U32]),
+    "ssip_precompute": (_ERR, [_P, _U32, _P]),
+    "recompute_ntt": (_ERR, [PolyPtr, _P, _U32, _P, _U32, _P, _U32, _U32]),
+    "gen_pq_omegas": (None, [_P, _P, _U32, _U32, _P]),
+    "distribute_powers": (_ERR, [PolyPtr, _P, _U64, _P, ctypes.POINTER(PolySlice)]),
+    "msm": (_ERR, [_P, _P, _U64, _U32, _U32, _U32, _U32, _U32, _P, _P, _U32, _P, _P]),
+    "msm_precompute": (_ERR, [_U64, _P, _U32]),
+    "msm_cfg": (_ERR, [_U32, _U32, _P, _P, _U64, _U32, _U32, _U32, _U32, _U32, _P, _P, _U32,
+                       _P, _P]),
+    "msm_precompute_cfg": (_ERR, [_U32, _U32, _U64, _P, _U32]),
+    "fused_gate_abc_d_qe": (_ERR, [_P, _P, _U64, ctypes.POINTER(PolySlice), _P, _P]),
+    "impeller_ntt": (_ERR, [_P, _P, _U32, _INT, _P, _P, _U32, _P]),
+    "impeller_twiddles": (_ERR, [_P, _U64, _P, _P]),
+    "impeller_msm_workspace_size": (ctypes.c_size_t, [_U64, _U32]),
+    "impeller_msm": (_ERR, [_P, _P, _U64, _U32, _P, _P, ctypes.c_size_t, _P]),
+    "impeller_gen_scalars": (_ERR, [_P, _U64, _INT, _U64, _U64, _P]),
+    "impeller_gen_bases": (_ERR, [_P, _U64, _U64, _U64, _P]),
+    "impeller_field_op": (_ERR, [_INT, _INT, _P, _P, _P, _U64, _P]),
+    "impeller_g1_op": (_ERR, [_INT, _P, _P, _P, _U64, _P]),
+    "impeller_timing_enable": (None, [_INT]),
+    "impeller_timing_reset": (None, []),
+    "impeller_timing_read": (_INT, [_INT, ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_ulonglong)]),
+    "impeller_version": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m impeller_amd.build` "
+                "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(name, code):
+    if code != 0:
+        raise HipError(name, code)
+
+
+def call(name, *args):
+    fn = getattr(lib(), name)
+    r = fn(*args)
+    if SIGNATURES[name][0] is _ERR:
+        check(name, r)
+    return r
+
+
+KERNEL_IDS = {"ntt_bitrev": 0, "ntt_dit": 1, "msm_digits": 2, "msm_scan": 3, "msm_scatter": 4,
+              "msm_buckets": 5, "msm_reduce": 6, "msm_sum": 7, "msm_final": 8, "gate": 9}
+
+
+def timing_enable(on=True):
+    lib().impeller_timing_enable(1 if on else 0)
+
+
+def timing_reset():
+    lib().impeller_timing_reset()
+
+
+def timing_read(name):
+    """(total_ms, launches) for a kernel id since the last reset."""
+    t = ctypes.c_double()
+    c = ctypes.c_ulonglong()
+    check("impeller_timing_read", lib().impeller_timing_read(KERNEL_IDS[name], ctypes.byref(t),
+                                                             ctypes.byref(c)))
+    return t.value, c.value

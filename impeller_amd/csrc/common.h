@@ -1,0 +1,71 @@
+// common.h -- shared host/device helpers for the HIP library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "../../include/impeller_hip.h"
+#include "field.h"
+
+namespace imp {
+
+#define IMP_CHECK(expr)                                                                  \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess) {                                                              \
+      fprintf(stderr, "[impeller_hip] %s:%d: %s -> %s\n", __FILE__, __LINE__, #expr,     \
+              hipGetErrorString(_e));                                                    \
+      return _e;                                                                         \
+    }                                                                                    \
+  } while (0)
+
+#define IMP_REQUIRE(cond, msg)                                                           \
+  do {                                                                                   \
+    if (!(cond)) {                                                                       \
+      fprintf(stderr, "[impeller_hip] invalid argument: %s (%s)\n", msg, #cond);         \
+      return hipErrorInvalidValue;                                                       \
+    }                                                                                    \
+  } while (0)
+
+// A PolyPtr-style view: logical index i -> physical (off + i) mod wl, with
+// off < wl and i < wl so one conditional subtraction suffices.
+struct View {
+  u32* base;
+  u64 off;
+  u64 wl;
+};
+
+__host__ __device__ __forceinline__ u64 view_phys(const View& v, u64 i) {
+  u64 p = v.off + i;
+  return p >= v.wl ? p - v.wl : p;
+}
+
+// iter.cuh:309-317: rotate != 0 requires an unsliced view and means
+// offset = len - rotate.
+inline bool make_view(const PolyPtr& p, View* v) {
+  u64 off = p.offset, wl = p.whole_len ? p.whole_len : p.len;
+  if (p.rotate != 0) {
+    if (p.len != wl || p.offset != 0) return false;
+    off = p.len - p.rotate;
+  }
+  if (p.len > wl || (wl && off >= wl)) return false;
+  v->base = p.ptr;
+  v->off = off;
+  v->wl = wl;
+  return true;
+}
+
+__host__ __device__ __forceinline__ u64 bit_reverse(u64 x, u32 bits) {
+  u64 r = 0;
+  for (u32 i = 0; i < bits; i++) {
+    r = (r << 1) | (x & 1);
+    x >>= 1;
+  }
+  return r;
+}
+
+__device__ __forceinline__ u32 brev_bits(u32 x, u32 bits) {
+  return bits == 0 ? 0u : (__builtin_bitreverse32(x) >> (32 - bits));
+}
+
+}  // namespace imp

@@ -1,0 +1,493 @@
+// field.h -- BN254 Fr / Fq Montgomery arithmetic and G1 XYZZ group law for
+// gfx950 (wave64 VALU, 8 x u32 limbs).
+//
+// Representation matches the reference (core/src/common/mont/src/field.cuh:
+// 638-743): 8 little-endian u32 limbs, Montgomery form with R = 2^256,
+// canonical (< modulus) at every API boundary.  Multiplication is the CIOS
+// Montgomery product (Koc, Acar, Kaliski 1996) in its "no final carry word"
+// form, which is valid because the top limb of both BN254 moduli is below
+// 2^31 - 1; the result of one product is < 2m and one conditional subtraction
+// makes it canonical.  No MFMA: this is exact big-integer work.
+//
+// Curve formulas are the EFD short-Weierstrass XYZZ set the reference uses
+// (core/src/common/curve/src/curve_xyzz.cuh:233-391): madd-2008-s,
+// add-2008-s, dbl-2008-s-1, mdbl-2008-s-1, affine conversion by z-scaling.
+// Identity: XYZZ with zz == 0; affine (0, 0).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace imp {
+
+typedef uint32_t u32;
+typedef uint64_t u64;
+
+// Constants: core/src/common/mont/src/bn254_fr.cuh:21-25,67 and
+// bn254_fq.cuh:13-17,59 (little-endian limbs).
+struct FrParams {
+  static constexpr u32 m[8] = {0xf0000001u, 0x43e1f593u, 0x79b97091u, 0x2833e848u,
+                               0x8181585du, 0xb85045b6u, 0xe131a029u, 0x30644e72u};
+  static constexpr u32 np = 4026531839u;  // -m^-1 mod 2^32
+  static constexpr u32 r1[8] = {0x4ffffffbu, 0xac96341cu, 0x9f60cd29u, 0x36fc7695u,
+                                0x7879462eu, 0x666ea36fu, 0x9a07df2fu, 0x0e0a77c1u};
+  static constexpr u32 r2[8] = {0xae216da7u, 0x1bb8e645u, 0xe35c59e3u, 0x53fe3ab1u,
+                                0x53bb8085u, 0x8c49833du, 0x7f4e44a5u, 0x0216d0b1u};
+};
+struct FqParams {
+  static constexpr u32 m[8] = {0xd87cfd47u, 0x3c208c16u, 0x6871ca8du, 0x97816a91u,
+                               0x8181585du, 0xb85045b6u, 0xe131a029u, 0x30644e72u};
+  static constexpr u32 np = 3834012553u;
+  static constexpr u32 r1[8] = {0xc58f0d9du, 0xd35d438du, 0xf5c70b3du, 0x0a78eb28u,
+                                0x7879462cu, 0x666ea36fu, 0x9a07df2fu, 0x0e0a77c1u};
+  static constexpr u32 r2[8] = {0x538afa89u, 0xf32cfc5bu, 0xd44501fbu, 0xb5e71911u,
+                                0x0a417ff6u, 0x47ab1effu, 0xcab8351fu, 0x06d89f71u};
+};
+
+template <class P>
+struct Fp {
+  u32 v[8];
+
+  __host__ __device__ __forceinline__ static Fp zero() {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = 0;
+    return r;
+  }
+  __host__ __device__ __forceinline__ static Fp one() {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = P::r1[i];
+    return r;
+  }
+  __host__ __device__ __forceinline__ static Fp r2() {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = P::r2[i];
+    return r;
+  }
+  __device__ __forceinline__ bool is_zero() const {
+    u32 acc = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) acc |= v[i];
+    return acc == 0;
+  }
+  __device__ __forceinline__ bool operator==(const Fp& o) const {
+    u32 acc = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) acc |= v[i] ^ o.v[i];
+    return acc == 0;
+  }
+};
+
+typedef Fp<FrParams> Fr;
+typedef Fp<FqParams> Fq;
+
+// ---------------------------------------------------------------- loads/stores
+// 32-byte element as two 16-byte vector accesses.
+template <class F>
+__device__ __forceinline__ F load_fe(const u32* p) {
+  const uint4* q = reinterpret_cast<const uint4*>(p);
+  uint4 a = q[0], b = q[1];
+  F r;
+  r.v[0] = a.x; r.v[1] = a.y; r.v[2] = a.z; r.v[3] = a.w;
+  r.v[4] = b.x; r.v[5] = b.y; r.v[6] = b.z; r.v[7] = b.w;
+  return r;
+}
+template <class F>
+__device__ __forceinline__ void store_fe(u32* p, const F& x) {
+  uint4* q = reinterpret_cast<uint4*>(p);
+  q[0] = make_uint4(x.v[0], x.v[1], x.v[2], x.v[3]);
+  q[1] = make_uint4(x.v[4], x.v[5], x.v[6], x.v[7]);
+}
+// Non-temporal variants for data touched once per pass.
+template <class F>
+__device__ __forceinline__ F load_fe_nt(const u32* p) {
+  F r;
+#pragma unroll
+  for (int i = 0; i < 8; i++) r.v[i] = __builtin_nontemporal_load(p + i);
+  return r;
+}
+
+// --------------------------------------------------------------- arithmetic
+// r = a - m if a >= m else a   (a < 2m)
+template <class P>
+__device__ __forceinline__ Fp<P> reduce_once(const Fp<P>& a) {
+  Fp<P> d;
+  u32 borrow = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    u64 t = (u64)a.v[i] - P::m[i] - borrow;
+    d.v[i] = (u32)t;
+    borrow = (u32)(t >> 63);
+  }
+  Fp<P> r;
+#pragma unroll
+  for (int i = 0; i < 8; i++) r.v[i] = borrow ? a.v[i] : d.v[i];
+  return r;
+}
+
+template <class P>
+__device__ __forceinline__ Fp<P> add(const Fp<P>& a, const Fp<P>& b) {
+  // a, b < m < 2^254 so a + b < 2^255: no carry out of the top limb.
+  Fp<P> s;
+  u32 c = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    u64 t = (u64)a.v[i] + b.v[i] + c;
+    s.v[i] = (u32)t;
+    c = (u32)(t >> 32);
+  }
+  return reduce_once<P>(s);
+}
+
+template <class P>
+__device__ __forceinline__ Fp<P> sub(const Fp<P>& a, const Fp<P>& b) {
+  Fp<P> d;
+  u32 borrow = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    u64 t = (u64)a.v[i] - b.v[i] - borrow;
+    d.v[i] = (u32)t;
+    borrow = (u32)(t >> 63);
+  }
+  // add back m when the subtraction wrapped
+  Fp<P> r;
+  u32 c = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    u64 t = (u64)d.v[i] + (borrow ? P::m[i] : 0u) + c;
+    r.v[i] = (u32)t;
+    c = (u32)(t >> 32);
+  }
+  return r;
+}
+
+template <class P>
+__device__ __forceinline__ Fp<P> neg(const Fp<P>& a) {
+  if (a.is_zero()) return a;
+  Fp<P> r;
+  u32 borrow = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    u64 t = (u64)P::m[i] - a.v[i] - borrow;
+    r.v[i] = (u32)t;
+    borrow = (u32)(t >> 63);
+  }
+  return r;
+}
+
+template <class P>
+__device__ __forceinline__ Fp<P> dbl(const Fp<P>& a) {
+  return add<P>(a, a);
+}
+
+// Montgomery product a*b*2^-256 mod m, result in [0, 2m) (no final subtract).
+template <class P>
+__device__ __forceinline__ Fp<P> mul_lazy(const Fp<P>& a, const Fp<P>& b) {
+  u32 t[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) t[j] = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    u64 A = (u64)a.v[0] * b.v[i] + t[0];
+    t[0] = (u32)A;
+    A >>= 32;
+    u32 q = t[0] * P::np;
+    u64 C = ((u64)q * P::m[0] + t[0]) >> 32;
+#pragma unroll
+    for (int j = 1; j < 8; j++) {
+      A = (u64)a.v[j] * b.v[i] + t[j] + A;
+      t[j] = (u32)A;
+      A >>= 32;
+      C = (u64)q * P::m[j] + t[j] + C;
+      t[j - 1] = (u32)C;
+      C >>= 32;
+    }
+    t[7] = (u32)(C + A);
+  }
+  Fp<P> r;
+#pragma unroll
+  for (int j = 0; j < 8; j++) r.v[j] = t[j];
+  return r;
+}
+
+template <class P>
+__device__ __forceinline__ Fp<P> mul(const Fp<P>& a, const Fp<P>& b) {
+  return reduce_once<P>(mul_lazy<P>(a, b));
+}
+template <class P>
+__device__ __forceinline__ Fp<P> sqr(const Fp<P>& a) {
+  return mul<P>(a, a);
+}
+
+// Montgomery -> plain integer (mul by 1), plain integer -> Montgomery (mul by R^2)
+template <class P>
+__device__ __forceinline__ Fp<P> to_number(const Fp<P>& a) {
+  Fp<P> one_int = Fp<P>::zero();
+  one_int.v[0] = 1;
+  return mul<P>(a, one_int);
+}
+template <class P>
+__device__ __forceinline__ Fp<P> from_number(const Fp<P>& a) {
+  return mul<P>(a, Fp<P>::r2());
+}
+
+// a^e for a 32-bit exponent, square-and-multiply from the top bit
+template <class P>
+__device__ __forceinline__ Fp<P> pow_u32(const Fp<P>& a, u32 e) {
+  Fp<P> acc = Fp<P>::one();
+  for (int i = 31; i >= 0; i--) {
+    acc = sqr<P>(acc);
+    if ((e >> i) & 1) acc = mul<P>(acc, a);
+  }
+  return acc;
+}
+
+// a^e for a 256-bit exponent given as limbs
+template <class P>
+__device__ __forceinline__ Fp<P> pow_big(const Fp<P>& a, const u32 e[8]) {
+  Fp<P> acc = Fp<P>::one();
+  for (int w = 7; w >= 0; w--) {
+    for (int i = 31; i >= 0; i--) {
+      acc = sqr<P>(acc);
+      if ((e[w] >> i) & 1) acc = mul<P>(acc, a);
+    }
+  }
+  return acc;
+}
+
+// inversion by Fermat: a^(m-2) (field.cuh:1172-1175); inv(0) = 0
+template <class P>
+__device__ __forceinline__ Fp<P> inv(const Fp<P>& a) {
+  u32 e[8];
+  u32 borrow = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    u64 t = (u64)P::m[i] - (i == 0 ? 2u : 0u) - borrow;
+    e[i] = (u32)t;
+    borrow = (u32)(t >> 63);
+  }
+  return pow_big<P>(a, e);
+}
+
+// ------------------------------------------------------------------ G1 curve
+struct G1Aff {
+  Fq x, y;
+  __device__ __forceinline__ bool is_identity() const { return y.is_zero(); }
+};
+struct G1Xyzz {
+  Fq x, y, zz, zzz;
+  __device__ __forceinline__ static G1Xyzz identity() {
+    G1Xyzz r;
+    r.x = Fq::zero();
+    r.y = Fq::zero();
+    r.zz = Fq::zero();
+    r.zzz = Fq::one();
+    return r;
+  }
+  __device__ __forceinline__ bool is_identity() const { return zz.is_zero(); }
+};
+
+__device__ __forceinline__ G1Aff load_aff(const u32* p) {
+  G1Aff r;
+  r.x = load_fe<Fq>(p);
+  r.y = load_fe<Fq>(p + 8);
+  return r;
+}
+__device__ __forceinline__ void store_aff(u32* p, const G1Aff& a) {
+  store_fe<Fq>(p, a.x);
+  store_fe<Fq>(p + 8, a.y);
+}
+__device__ __forceinline__ G1Xyzz load_xyzz(const u32* p) {
+  G1Xyzz r;
+  r.x = load_fe<Fq>(p);
+  r.y = load_fe<Fq>(p + 8);
+  r.zz = load_fe<Fq>(p + 16);
+  r.zzz = load_fe<Fq>(p + 24);
+  return r;
+}
+__device__ __forceinline__ void store_xyzz(u32* p, const G1Xyzz& a) {
+  store_fe<Fq>(p, a.x);
+  store_fe<Fq>(p + 8, a.y);
+  store_fe<Fq>(p + 16, a.zz);
+  store_fe<Fq>(p + 24, a.zzz);
+}
+
+__device__ __forceinline__ G1Xyzz aff_to_xyzz(const G1Aff& a) {
+  if (a.is_identity()) return G1Xyzz::identity();
+  G1Xyzz r;
+  r.x = a.x;
+  r.y = a.y;
+  r.zz = Fq::one();
+  r.zzz = Fq::one();
+  return r;
+}
+
+// mdbl-2008-s-1: 2 * affine point
+__device__ __forceinline__ G1Xyzz aff_dbl(const G1Aff& p) {
+  typedef FqParams Q;
+  Fq u = dbl<Q>(p.y);
+  Fq v = sqr<Q>(u);
+  Fq w = mul<Q>(u, v);
+  Fq s = mul<Q>(p.x, v);
+  Fq x2 = sqr<Q>(p.x);
+  Fq m = add<Q>(dbl<Q>(x2), x2);
+  Fq x3 = sub<Q>(sub<Q>(sqr<Q>(m), s), s);
+  Fq y3 = sub<Q>(mul<Q>(m, sub<Q>(s, x3)), mul<Q>(w, p.y));
+  G1Xyzz r;
+  r.x = x3;
+  r.y = y3;
+  r.zz = v;
+  r.zzz = w;
+  return r;
+}
+
+// dbl-2008-s-1
+__device__ __forceinline__ G1Xyzz xyzz_dbl(const G1Xyzz& p) {
+  typedef FqParams Q;
+  if (p.is_identity()) return p;
+  Fq u = dbl<Q>(p.y);
+  Fq v = sqr<Q>(u);
+  Fq w = mul<Q>(u, v);
+  Fq s = mul<Q>(p.x, v);
+  Fq x2 = sqr<Q>(p.x);
+  Fq m = add<Q>(dbl<Q>(x2), x2);
+  Fq x3 = sub<Q>(sub<Q>(sqr<Q>(m), s), s);
+  Fq y3 = sub<Q>(mul<Q>(m, sub<Q>(s, x3)), mul<Q>(w, p.y));
+  G1Xyzz r;
+  r.x = x3;
+  r.y = y3;
+  r.zz = mul<Q>(v, p.zz);
+  r.zzz = mul<Q>(w, p.zzz);
+  return r;
+}
+
+// madd-2008-s: XYZZ + affine
+__device__ __forceinline__ G1Xyzz xyzz_madd(const G1Xyzz& p, const G1Aff& q) {
+  typedef FqParams Q;
+  if (q.is_identity()) return p;
+  if (p.is_identity()) return aff_to_xyzz(q);
+  Fq u2 = mul<Q>(q.x, p.zz);
+  Fq s2 = mul<Q>(q.y, p.zzz);
+  Fq P_ = sub<Q>(u2, p.x);
+  Fq R_ = sub<Q>(s2, p.y);
+  if (P_.is_zero() && R_.is_zero()) return aff_dbl(q);
+  Fq pp = sqr<Q>(P_);
+  Fq ppp = mul<Q>(P_, pp);
+  Fq qq = mul<Q>(p.x, pp);
+  Fq x3 = sub<Q>(sub<Q>(sub<Q>(sqr<Q>(R_), ppp), qq), qq);
+  Fq y3 = sub<Q>(mul<Q>(R_, sub<Q>(qq, x3)), mul<Q>(p.y, ppp));
+  G1Xyzz r;
+  r.x = x3;
+  r.y = y3;
+  r.zz = mul<Q>(p.zz, pp);
+  r.zzz = mul<Q>(p.zzz, ppp);
+  return r;
+}
+
+// add-2008-s: XYZZ + XYZZ
+__device__ __forceinline__ G1Xyzz xyzz_add(const G1Xyzz& p, const G1Xyzz& q) {
+  typedef FqParams Q;
+  if (p.is_identity()) return q;
+  if (q.is_identity()) return p;
+  Fq u1 = mul<Q>(p.x, q.zz);
+  Fq u2 = mul<Q>(q.x, p.zz);
+  Fq s1 = mul<Q>(p.y, q.zzz);
+  Fq s2 = mul<Q>(q.y, p.zzz);
+  Fq P_ = sub<Q>(u2, u1);
+  Fq R_ = sub<Q>(s2, s1);
+  if (P_.is_zero() && R_.is_zero()) return xyzz_dbl(p);
+  Fq pp = sqr<Q>(P_);
+  Fq ppp = mul<Q>(P_, pp);
+  Fq qq = mul<Q>(u1, pp);
+  Fq x3 = sub<Q>(sub<Q>(sub<Q>(sqr<Q>(R_), ppp), qq), qq);
+  Fq y3 = sub<Q>(mul<Q>(R_, sub<Q>(qq, x3)), mul<Q>(s1, ppp));
+  G1Xyzz r;
+  r.x = x3;
+  r.y = y3;
+  r.zz = mul<Q>(mul<Q>(p.zz, q.zz), pp);
+  r.zzz = mul<Q>(mul<Q>(p.zzz, q.zzz), ppp);
+  return r;
+}
+
+__device__ __forceinline__ G1Aff aff_neg(const G1Aff& a) {
+  G1Aff r;
+  r.x = a.x;
+  r.y = neg<FqParams>(a.y);
+  return r;
+}
+
+// z-scaling to affine; identity -> (0, 0)
+__device__ __forceinline__ G1Aff xyzz_to_aff(const G1Xyzz& p) {
+  typedef FqParams Q;
+  G1Aff r;
+  if (p.is_identity()) {
+    r.x = Fq::zero();
+    r.y = Fq::zero();
+    return r;
+  }
+  Fq a = inv<Q>(p.zzz);
+  Fq b = sqr<Q>(mul<Q>(p.zz, a));
+  r.x = mul<Q>(p.x, b);
+  r.y = mul<Q>(p.y, a);
+  return r;
+}
+
+// k * P, k a 256-bit plain integer (double-and-add from the top bit)
+__device__ __forceinline__ G1Xyzz scalar_mul(const G1Aff& p, const u32 k[8]) {
+  G1Xyzz acc = G1Xyzz::identity();
+  for (int w = 7; w >= 0; w--) {
+    for (int i = 31; i >= 0; i--) {
+      acc = xyzz_dbl(acc);
+      if ((k[w] >> i) & 1) acc = xyzz_madd(acc, p);
+    }
+  }
+  return acc;
+}
+
+// k * P for a small k (< 2^32), XYZZ input
+__device__ __forceinline__ G1Xyzz xyzz_mul_u32(const G1Xyzz& p, u32 k) {
+  G1Xyzz acc = G1Xyzz::identity();
+  for (int i = 31; i >= 0; i--) {
+    acc = xyzz_dbl(acc);
+    if ((k >> i) & 1) acc = xyzz_add(acc, p);
+  }
+  return acc;
+}
+
+// ------------------------------------------------------------- seeded inputs
+// Counter-based SplitMix64 generator.  The same definition lives in the CPU
+// checker so both sides see identical bytes for a (seed, index).
+__host__ __device__ __forceinline__ u64 smix(u64 z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// uniform Fr (its Montgomery representation is uniform in [0, r))
+__device__ __forceinline__ Fr gen_uniform_fr(u64 seed, u64 i) {
+  u64 key = smix(seed) ^ (i * 0xD1B54A32D192ED03ull);
+  for (u64 a = 0;; a++) {
+    u64 w[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) w[j] = smix(key + 4 * a + (u64)j);
+    w[3] &= 0x3FFFFFFFFFFFFFFFull;
+    Fr r;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      r.v[2 * j] = (u32)w[j];
+      r.v[2 * j + 1] = (u32)(w[j] >> 32);
+    }
+    // accept if r < m
+    u32 borrow = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      u64 t = (u64)r.v[j] - FrParams::m[j] - borrow;
+      borrow = (u32)(t >> 63);
+    }
+    if (borrow) return r;
+  }
+}
+
+}  // namespace imp

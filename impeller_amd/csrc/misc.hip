@@ -1,0 +1,195 @@
+// misc.hip -- fused gate kernel, seeded input generators and the small
+// device-arithmetic entry points used to test the field and curve code.
+#include "common.h"
+#include "timing.h"
+
+namespace imp {
+
+// ---------------------------------------------------------------- fused gate
+// out = a*b*c - d + q*e*(1 - q) over Fr, one element per index, grid-stride
+// (the generated kernels' loop, core/src/fused_kernels/kernel_gen.rs:281-304).
+// Operands are PolyPtr views (make_slice_iter semantics, iter.cuh:309-327).
+struct GateViews {
+  View in[6];
+  View out;
+};
+
+__global__ __launch_bounds__(256) void k_gate(GateViews g, u64 len) {
+  const u64 stride = (u64)gridDim.x * blockDim.x;
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
+    Fr a = load_fe<Fr>(g.in[0].base + 8 * view_phys(g.in[0], i));
+    Fr b = load_fe<Fr>(g.in[1].base + 8 * view_phys(g.in[1], i));
+    Fr c = load_fe<Fr>(g.in[2].base + 8 * view_phys(g.in[2], i));
+    Fr d = load_fe<Fr>(g.in[3].base + 8 * view_phys(g.in[3], i));
+    Fr q = load_fe<Fr>(g.in[4].base + 8 * view_phys(g.in[4], i));
+    Fr e = load_fe<Fr>(g.in[5].base + 8 * view_phys(g.in[5], i));
+    Fr t1 = sub<FrParams>(mul<FrParams>(mul<FrParams>(a, b), c), d);
+    Fr t2 = mul<FrParams>(mul<FrParams>(q, e), sub<FrParams>(Fr::one(), q));
+    store_fe<Fr>(g.out.base + 8 * view_phys(g.out, i), add<FrParams>(t1, t2));
+  }
+}
+
+// ---------------------------------------------------------------- generators
+__global__ void k_gen_scalars(u32* out, u64 n, int kind, u64 seed, u64 start) {
+  u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  u64 i = start + t;
+  Fr v;
+  if (kind == 0) {
+    v = gen_uniform_fr(seed, i);
+  } else {
+    u64 h = smix(smix(seed) ^ (i * 0xD1B54A32D192ED03ull) ^ 0x5DEECE66Dull);
+    if (kind == 1) {
+      v = (h % 10) == 0 ? Fr::one() : Fr::zero();
+    } else {
+      v = Fr::zero();
+      v.v[0] = (u32)(h & 0xFFFFu);
+      v = from_number<FrParams>(v);
+    }
+  }
+  store_fe<Fr>(out + 8 * t, v);
+}
+
+// P_i = to_number(k_i) * G, k_i = uniform Fr; G = (1, 2)
+__global__ void k_gen_bases(u32* out, u64 n, u64 seed, u64 start) {
+  u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  Fr k = to_number<FrParams>(gen_uniform_fr(seed, start + t));
+  G1Aff g;
+  g.x = Fq::zero();
+  g.x.v[0] = 1;
+  g.x = from_number<FqParams>(g.x);
+  g.y = Fq::zero();
+  g.y.v[0] = 2;
+  g.y = from_number<FqParams>(g.y);
+  store_aff(out + 16 * t, xyzz_to_aff(scalar_mul(g, k.v)));
+}
+
+// ------------------------------------------------------- arithmetic probes
+template <class P>
+__global__ void k_field_op(int op, u32* r, const u32* a, const u32* b, u64 n) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  typedef Fp<P> F;
+  F x = load_fe<F>(a + 8 * i);
+  F y = b ? load_fe<F>(b + 8 * i) : F::zero();
+  F z;
+  switch (op) {
+    case 0: z = mul<P>(x, y); break;
+    case 1: z = add<P>(x, y); break;
+    case 2: z = sub<P>(x, y); break;
+    case 3: z = neg<P>(x); break;
+    case 4: z = from_number<P>(x); break;
+    case 5: z = to_number<P>(x); break;
+    case 6: z = pow_big<P>(x, y.v); break;
+    default: z = inv<P>(x); break;
+  }
+  store_fe<F>(r + 8 * i, z);
+}
+
+__global__ void k_g1_op(int op, u32* r, const u32* a, const u32* b, u64 n) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  switch (op) {
+    case 0: store_xyzz(r + 32 * i, xyzz_madd(load_xyzz(a + 32 * i), load_aff(b + 16 * i))); break;
+    case 1: store_xyzz(r + 32 * i, xyzz_add(load_xyzz(a + 32 * i), load_xyzz(b + 32 * i))); break;
+    case 2: store_xyzz(r + 32 * i, xyzz_dbl(load_xyzz(a + 32 * i))); break;
+    case 3: store_aff(r + 16 * i, xyzz_to_aff(load_xyzz(a + 32 * i))); break;
+    default: {
+      Fr k = load_fe<Fr>(b + 8 * i);
+      store_aff(r + 16 * i, xyzz_to_aff(scalar_mul(load_aff(a + 16 * i), k.v)));
+    }
+  }
+}
+
+}  // namespace imp
+
+using namespace imp;
+
+extern "C" hipError_t fused_gate_abc_d_qe(const ConstPolyPtr* vars, const PolyPtr* mut_vars,
+                                          unsigned long long len, const PolySlice* slice_info,
+                                          void* local_buffer, hipStream_t stream) {
+  (void)slice_info;
+  (void)local_buffer;
+  IMP_REQUIRE(vars && mut_vars, "null pointer");
+  // The generated kernels receive device arrays of PolyPtr (fused_kernels.rs:
+  // 172-180); the views are read back to the host once to build kernel args.
+  ConstPolyPtr in[6];
+  PolyPtr out;
+  IMP_CHECK(hipMemcpyAsync(in, vars, sizeof in, hipMemcpyDeviceToHost, stream));
+  IMP_CHECK(hipMemcpyAsync(&out, mut_vars, sizeof out, hipMemcpyDeviceToHost, stream));
+  IMP_CHECK(hipStreamSynchronize(stream));
+  GateViews g;
+  for (int k = 0; k < 6; k++) {
+    PolyPtr p = {const_cast<u32*>(in[k].ptr), in[k].len, in[k].rotate, in[k].offset,
+                 in[k].whole_len};
+    IMP_REQUIRE(p.ptr && p.len >= len, "gate input view");
+    IMP_REQUIRE(make_view(p, &g.in[k]), "gate input view");
+  }
+  IMP_REQUIRE(out.ptr && out.len >= len, "gate output view");
+  IMP_REQUIRE(make_view(out, &g.out), "gate output view");
+  if (len == 0) return hipSuccess;
+  int dev = 0, cus = 256;
+  IMP_CHECK(hipGetDevice(&dev));
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  u64 blocks = (len + 255) / 256;
+  u64 cap = (u64)cus * 8;
+  if (blocks > cap) blocks = cap;
+  {
+    TimedScope ts(K_GATE, stream);
+    hipLaunchKernelGGL(k_gate, dim3((u32)blocks), dim3(256), 0, stream, g, (u64)len);
+  }
+  return hipGetLastError();
+}
+
+extern "C" hipError_t impeller_gen_scalars(uint32_t* out_d, unsigned long long n, int kind,
+                                           unsigned long long seed, unsigned long long start,
+                                           hipStream_t stream) {
+  IMP_REQUIRE(out_d || n == 0, "null pointer");
+  IMP_REQUIRE(kind >= 0 && kind <= 2, "kind");
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gen_scalars, dim3((u32)((n + 255) / 256)), dim3(256), 0, stream, out_d,
+                     (u64)n, kind, (u64)seed, (u64)start);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t impeller_gen_bases(uint32_t* out_d, unsigned long long n,
+                                         unsigned long long seed, unsigned long long start,
+                                         hipStream_t stream) {
+  IMP_REQUIRE(out_d || n == 0, "null pointer");
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gen_bases, dim3((u32)((n + 127) / 128)), dim3(128), 0, stream, out_d,
+                     (u64)n, (u64)seed, (u64)start);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t impeller_field_op(int field, int op, uint32_t* r_d, const uint32_t* a_d,
+                                        const uint32_t* b_d, unsigned long long n,
+                                        hipStream_t stream) {
+  IMP_REQUIRE(r_d && a_d, "null pointer");
+  IMP_REQUIRE(op >= 0 && op <= 7 && (field == 0 || field == 1), "op/field");
+  IMP_REQUIRE(b_d || op == 3 || op == 4 || op == 5 || op == 7, "b operand");
+  if (n == 0) return hipSuccess;
+  dim3 grid((u32)((n + 127) / 128));
+  if (field == 0)
+    hipLaunchKernelGGL(k_field_op<FrParams>, grid, dim3(128), 0, stream, op, r_d, a_d, b_d,
+                       (u64)n);
+  else
+    hipLaunchKernelGGL(k_field_op<FqParams>, grid, dim3(128), 0, stream, op, r_d, a_d, b_d,
+                       (u64)n);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t impeller_g1_op(int op, uint32_t* r_d, const uint32_t* a_d,
+                                     const uint32_t* b_d, unsigned long long n,
+                                     hipStream_t stream) {
+  IMP_REQUIRE(r_d && a_d, "null pointer");
+  IMP_REQUIRE(op >= 0 && op <= 4, "op");
+  IMP_REQUIRE(b_d || op == 2 || op == 3, "b operand");
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_g1_op, dim3((u32)((n + 63) / 64)), dim3(64), 0, stream, op, r_d, a_d,
+                     b_d, (u64)n);
+  return hipGetLastError();
+}
+
+extern "C" const char* impeller_version(void) { return "impeller_hip 0.1 (gfx950)"; }
