@@ -182,8 +182,10 @@ __device__ __forceinline__ Fp<P> dbl(const Fp<P>& a) {
 }
 
 // Montgomery product a*b*2^-256 mod m, result in [0, 2m) (no final subtract).
+//
+// Operand-scanning CIOS in plain C (kept as the portable reference form).
 template <class P>
-__device__ __forceinline__ Fp<P> mul_lazy(const Fp<P>& a, const Fp<P>& b) {
+__device__ __forceinline__ Fp<P> mul_lazy_cios(const Fp<P>& a, const Fp<P>& b) {
   u32 t[8];
 #pragma unroll
   for (int j = 0; j < 8; j++) t[j] = 0;
@@ -209,6 +211,71 @@ __device__ __forceinline__ Fp<P> mul_lazy(const Fp<P>& a, const Fp<P>& b) {
 #pragma unroll
   for (int j = 0; j < 8; j++) r.v[j] = t[j];
   return r;
+}
+
+// (acc, hi) += x * y with a 96-bit accumulator: one v_mad_u64_u32 whose
+// carry-out feeds one v_addc_co_u32.  The 64-bit accumulator is the mad's own
+// addend register pair, so no zero-extension moves are needed.
+__device__ __forceinline__ void mac96(u64& acc, u32& hi, u32 x, u32 y) {
+  u64 c;
+  asm("v_mad_u64_u32 %0, %1, %3, %4, %0\n\t"
+      "v_addc_co_u32 %2, %1, %2, 0, %1"
+      : "+v"(acc), "=&s"(c), "+v"(hi)
+      : "v"(x), "v"(y));
+}
+// same with a wave-uniform (SGPR) second factor, e.g. a modulus limb
+__device__ __forceinline__ void mac96s(u64& acc, u32& hi, u32 x, u32 y) {
+  u64 c;
+  asm("v_mad_u64_u32 %0, %1, %3, %4, %0\n\t"
+      "v_addc_co_u32 %2, %1, %2, 0, %1"
+      : "+v"(acc), "=&s"(c), "+v"(hi)
+      : "v"(x), "s"(y));
+}
+
+// Product-scanning Montgomery multiplication (FIPS, Koc-Acar-Kaliski 1996):
+// column k accumulates a_j*b_(k-j) and m_j*q_(k-j) into a 96-bit accumulator;
+// m_k = acc * q' mod 2^32 for k < 8.  For a, b < 2m and 4m < 2^256 the result
+// is < 2m and fits 8 limbs.
+template <class P>
+__device__ __forceinline__ Fp<P> mul_lazy_fips(const Fp<P>& a, const Fp<P>& b) {
+  u32 m[8];
+  Fp<P> r;
+  u64 acc = 0;
+  u32 hi = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+#pragma unroll
+    for (int j = 0; j < i; j++) {
+      mac96(acc, hi, a.v[j], b.v[i - j]);
+      mac96s(acc, hi, m[j], P::m[i - j]);
+    }
+    mac96(acc, hi, a.v[i], b.v[0]);
+    m[i] = (u32)acc * P::np;
+    mac96s(acc, hi, m[i], P::m[0]);
+    acc = (acc >> 32) | ((u64)hi << 32);
+    hi = 0;
+  }
+#pragma unroll
+  for (int i = 8; i < 16; i++) {
+#pragma unroll
+    for (int j = i - 7; j < 8; j++) {
+      mac96(acc, hi, a.v[j], b.v[i - j]);
+      mac96s(acc, hi, m[j], P::m[i - j]);
+    }
+    r.v[i - 8] = (u32)acc;
+    acc = (acc >> 32) | ((u64)hi << 32);
+    hi = 0;
+  }
+  return r;
+}
+
+template <class P>
+__device__ __forceinline__ Fp<P> mul_lazy(const Fp<P>& a, const Fp<P>& b) {
+#ifdef IMP_MONT_CIOS
+  return mul_lazy_cios<P>(a, b);
+#else
+  return mul_lazy_fips<P>(a, b);
+#endif
 }
 
 template <class P>

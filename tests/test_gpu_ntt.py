@@ -93,7 +93,8 @@ def test_recompute_ntt(stream, k):
     assert (om.to_numpy()[3] == O.pow_(w, L(8))).all()
     pq_deg = g.get_pq_len(k).bit_length()
     step = (1 << k) >> pq_deg
-    assert (pq.to_numpy()[1] == O.pow_(w, L(step))).all()
+    if pq.len > 1:
+        assert (pq.to_numpy()[1] == O.pow_(w, L(step))).all()
     buf = ScalarArray.from_numpy(np.zeros((1 << k, 8), np.uint32))
     buf.rotate(-2)
     phys = ((1 << k) - buf.get_rotation() + np.arange(1 << k)) % (1 << k)
