@@ -73,25 +73,54 @@ def main():
     stream.sync()
     x_orig = x.clone()
 
-    def run_msm(c, ws):
-        out = device_u32(16)
-        _lib.call("impeller_msm", bases.data_ptr(), scal.data_ptr(), nm, c, out.data_ptr(),
-                  ws.data_ptr(), M.workspace_size(nm, c), stream.raw())
-        return out
-
-    # window sweep (c = 12..18) on one warm call each
+    # MSM configurations: precomputed window tables (one bucket set, the
+    # reference's TARGET_WINDOWS = 1 layout, tables built once as setup) and
+    # the plain per-window layout without tables.
     windows = [args.window] if args.window else list(range(12, 19))
-    ws = device_u32((max(M.workspace_size(nm, c) for c in windows) + 3) // 4)
-    sweep = {}
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def time_msm(tables, c, target, task, reps=2):
+        ws = device_u32((M.cfg_workspace_size(nm, c, target, task) + 3) // 4)
+        out = device_u32(16)
+        M.msm_device_cfg(tables, scal, nm, c, target, stream, task=task, workspace=ws, out=out,
+                         sync=False)
+        best = None
+        for _ in range(reps):
+            ev0.record()
+            M.msm_device_cfg(tables, scal, nm, c, target, stream, task=task, workspace=ws,
+                             out=out, sync=False)
+            ev1.record()
+            ev1.synchronize()
+            t = ev0.elapsed_time(ev1)
+            best = t if best is None else min(best, t)
+        del ws
+        return best
+
+    sweep, sweep_plain, setup_ms = {}, {}, {}
     for c in windows:
-        run_msm(c, ws)
+        nt = M.n_tables(c, 1)
+        tables = device_u32(nt * nm * 16)
+        tables[: nm * 16].copy_(bases)
         ev0.record()
-        run_msm(c, ws)
+        M.precompute_device(tables, nm, c, 1, stream)
         ev1.record()
         ev1.synchronize()
-        sweep[c] = ev0.elapsed_time(ev1)
+        setup_ms[c] = ev0.elapsed_time(ev1)
+        sweep[c] = time_msm(tables, c, 1, 64)
+        del tables
+        torch.cuda.empty_cache()
+    for c in (windows if args.window else (15, 16, 17)):
+        sweep_plain[c] = time_msm(bases, c, 0, 64)
     best_c = min(sweep, key=sweep.get)
+    tables = device_u32(M.n_tables(best_c, 1) * nm * 16)
+    tables[: nm * 16].copy_(bases)
+    M.precompute_device(tables, nm, best_c, 1, stream)
+    ws = device_u32((M.cfg_workspace_size(nm, best_c, 1, 64) + 3) // 4)
+    res_buf = device_u32(16)
+
+    def run_msm(c, _ws=None):
+        return M.msm_device_cfg(tables, scal, nm, c, 1, stream, task=64, workspace=ws,
+                                out=res_buf, sync=False)
 
     def step():
         N.ntt(x, tw, k, stream)
@@ -193,12 +222,15 @@ def main():
             "workload": f"NTT 2^{k} + coset NTT 2^{k} + G1 MSM 2^{args.log_msm} "
                         "(plain single-call path)",
             "ntt_log": k, "msm_log": args.log_msm, "msm_window": best_c,
+            "msm_layout": "precomputed window tables (one bucket set), task 64",
             "parallelism": "single GPU",
         },
         "ntt_2^24_ms": round(ntt_ms, 3),
         "coset_ntt_2^24_ms": round(coset_ms, 3),
         "msm_2^22_ms": round(msm_ms, 3),
         "msm_window_sweep_ms": {str(c): round(v, 3) for c, v in sweep.items()},
+        "msm_table_setup_ms": {str(c): round(v, 2) for c, v in setup_ms.items()},
+        "msm_no_tables_ms": {str(c): round(v, 3) for c, v in sweep_plain.items()},
         "batched_vs_plain_speedup": None,
         "kernels_ms": {name: {"total": round(v[0], 3), "launches": v[1]}
                        for name, v in kt.items() if v[1]},

@@ -32,6 +32,20 @@ class PolySlice(ctypes.Structure):
                 ("len", ctypes.c_ulonglong)]
 
 
+class ImpNttItem(ctypes.Structure):
+    """include/impeller_hip.h ImpNttItem"""
+    _fields_ = [("data", ctypes.c_void_p), ("src", ctypes.c_void_p),
+                ("log_n", ctypes.c_uint), ("log_src", ctypes.c_uint),
+                ("twiddle", ctypes.c_void_p), ("inverse", ctypes.c_int),
+                ("coset_shift", ctypes.c_void_p), ("plan", ctypes.c_uint * 10),
+                ("plan_len", ctypes.c_uint)]
+
+
+class ImpNttBatch(ctypes.Structure):
+    _fields_ = [("dev", ctypes.c_void_p), ("count", ctypes.c_uint), ("nstages", ctypes.c_uint),
+                ("total", ctypes.c_uint * 9), ("lds", ctypes.c_uint * 9)]
+
+
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint
 _U64 = ctypes.c_ulonglong
@@ -53,12 +67,30 @@ SIGNATURES = {
     "fused_gate_abc_d_qe": (_ERR, [_P, _P, _U64, ctypes.POINTER(PolySlice), _P, _P]),
     "impeller_ntt": (_ERR, [_P, _P, _U32, _INT, _P, _P, _U32, _P]),
     "impeller_twiddles": (_ERR, [_P, _U64, _P, _P]),
+    "impeller_ntt_batch_bytes": (ctypes.c_size_t, [_U32]),
+    "impeller_ntt_batch_prepare": (_ERR, [ctypes.POINTER(ImpNttItem), _U32, _P, ctypes.c_size_t,
+                                          ctypes.POINTER(ImpNttBatch), _P]),
+    "impeller_ntt_batch_run": (_ERR, [ctypes.POINTER(ImpNttBatch), _U32, _P]),
+    "impeller_graph_begin": (_ERR, [_P]),
+    "impeller_graph_end": (_ERR, [_P, ctypes.POINTER(ctypes.c_void_p)]),
+    "impeller_graph_launch": (_ERR, [_P, _P]),
+    "impeller_graph_destroy": (_ERR, [_P]),
     "impeller_msm_workspace_size": (ctypes.c_size_t, [_U64, _U32]),
     "impeller_msm": (_ERR, [_P, _P, _U64, _U32, _P, _P, ctypes.c_size_t, _P]),
+    "impeller_msm_cfg_workspace_size": (ctypes.c_size_t, [_U64, _U32, _U32, _U32]),
+    "impeller_msm_tables": (_U32, [_U32, _U32]),
+    "impeller_msm_precompute": (_ERR, [_P, _U64, _U32, _U32, _P]),
+    "impeller_msm_cfg": (_ERR, [_P, _P, _U64, _U32, _U32, _U32, _P, _P, ctypes.c_size_t, _P]),
     "impeller_gen_scalars": (_ERR, [_P, _U64, _INT, _U64, _U64, _P]),
     "impeller_gen_bases": (_ERR, [_P, _U64, _U64, _U64, _P]),
     "impeller_field_op": (_ERR, [_INT, _INT, _P, _P, _P, _U64, _P]),
     "impeller_g1_op": (_ERR, [_INT, _P, _P, _P, _U64, _P]),
+    "impeller_mlp_param_count": (_U32, [_U32, _U32]),
+    "impeller_mlp_workspace_floats": (ctypes.c_size_t, [_U32, _U32, _U32]),
+    "impeller_mlp_forward": (_ERR, [_P, _P, _U32, _U32, _U32, _P, _P, _P]),
+    "impeller_mlp_grad": (_ERR, [_P, _P, _P, _U32, _U32, _U32, _P, _P, _P, _P]),
+    "impeller_mlp_train_step": (_ERR, [_P, _P, _P, _P, _P, _U32, _U32, _U32, ctypes.c_float,
+                                       _U32, _P, _P, _P]),
     "impeller_timing_enable": (None, [_INT]),
     "impeller_timing_reset": (None, []),
     "impeller_timing_read": (_INT, [_INT, ctypes.POINTER(ctypes.c_double),
@@ -98,8 +130,9 @@ def call(name, *args):
     return r
 
 
-KERNEL_IDS = {"ntt_bitrev": 0, "ntt_dit": 1, "msm_digits": 2, "msm_scan": 3, "msm_scatter": 4,
-              "msm_buckets": 5, "msm_reduce": 6, "msm_sum": 7, "msm_final": 8, "gate": 9}
+KERNEL_IDS = {"ntt_bitrev": 0, "ntt_dit": 1, "msm_digits": 2, "msm_bounds": 3,
+              "msm_buckets": 5, "msm_reduce": 6, "msm_sum": 7, "msm_final": 8, "gate": 9,
+              "msm_sort": 10, "msm_combine": 11}
 
 
 def timing_enable(on=True):

@@ -55,6 +55,8 @@ inline bool make_view(const PolyPtr& p, View* v) {
   return true;
 }
 
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
 __host__ __device__ __forceinline__ u64 bit_reverse(u64 x, u32 bits) {
   u64 r = 0;
   for (u32 i = 0; i < bits; i++) {

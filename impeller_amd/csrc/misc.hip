@@ -192,4 +192,30 @@ extern "C" hipError_t impeller_g1_op(int op, uint32_t* r_d, const uint32_t* a_d,
   return hipGetLastError();
 }
 
+extern "C" hipError_t impeller_graph_begin(hipStream_t stream) {
+  return hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+}
+
+extern "C" hipError_t impeller_graph_end(hipStream_t stream, void** graph_exec) {
+  IMP_REQUIRE(graph_exec, "null pointer");
+  hipGraph_t g = nullptr;
+  IMP_CHECK(hipStreamEndCapture(stream, &g));
+  hipGraphExec_t e = nullptr;
+  hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  IMP_CHECK(r);
+  *graph_exec = (void*)e;
+  return hipSuccess;
+}
+
+extern "C" hipError_t impeller_graph_launch(void* graph_exec, hipStream_t stream) {
+  IMP_REQUIRE(graph_exec, "null graph");
+  return hipGraphLaunch((hipGraphExec_t)graph_exec, stream);
+}
+
+extern "C" hipError_t impeller_graph_destroy(void* graph_exec) {
+  if (!graph_exec) return hipSuccess;
+  return hipGraphExecDestroy((hipGraphExec_t)graph_exec);
+}
+
 extern "C" const char* impeller_version(void) { return "impeller_hip 0.1 (gfx950)"; }

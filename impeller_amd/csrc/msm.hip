@@ -11,12 +11,16 @@
 // unique, so it is bit-identical to any correct CPU MSM on the same inputs.
 //
 // Pipeline (all on one stream, device resident):
-//   k_msm_digits   scalar -> signed digits, per-bucket histogram (atomics)
-//   k_msm_scan     per-window exclusive scan of the histogram
-//   k_msm_scatter  counting-sort scatter of (point index | sign) by bucket
-//   k_msm_buckets  mixed XYZZ+affine accumulation, one thread per bucket run
-//   k_msm_reduce   sum_b b * B_b via running sums over bucket segments
-//   k_msm_final    window partials -> Horner over windows -> affine
+//   k_msm_digits      scalar -> signed digits -> (bucket key, point | sign)
+//   k_sort_*          stable LSD radix sort of the keys per physical window
+//                     (4-bit digits, per-tile histograms, block-local ranks)
+//   k_bucket_bounds   run boundaries of each bucket in the sorted keys
+//   k_bucket_tasks    each bucket split into tasks of <= T entries, one
+//                     thread per task, mixed XYZZ + affine additions
+//   k_bucket_combine  per-bucket partial-sum tree (groups of 2^g) so that a
+//                     heavily loaded bucket never serialises on one thread
+//   k_msm_reduce      sum_b b * B_b via running sums over bucket segments
+//   k_msm_final       window partials -> Horner over windows -> affine
 #include <algorithm>
 #include <mutex>
 #include <vector>
@@ -26,32 +30,53 @@
 
 namespace imp {
 
+// ------------------------------------------------------------------- shape
 struct MsmShape {
-  u32 c;           // window bits
-  u32 W;           // logical windows
-  u32 n_windows;   // physical windows
-  u32 n_pre;       // point tables
-  u32 nb;          // buckets per physical window = 2^(c-1)
-  u32 seg;         // buckets per reduce segment
-  u32 nseg;        // segments per physical window
-  u64 N;           // points per table
+  u32 c;          // window bits
+  u32 W;          // logical windows = floor(254 / c) + 1
+  u32 n_windows;  // physical windows (bucket sets)
+  u32 n_pre;      // point tables
+  u32 nb;         // buckets per physical window = 2^(c-1)
+  u64 N;          // points per table
+  u64 seg_len;    // sort entries per physical window = n_pre * N
+  u32 tiles;      // radix-sort tiles per physical window
+  u32 passes;     // radix passes (4-bit digits over c-bit keys)
+  u32 T;          // entries per accumulation task
+  u32 g;          // log2 of the combine group size
+  u32 levels;     // combine levels
+  u64 task_cap;   // task slots per physical window
+  u32 seg;        // buckets per reduce segment
+  u32 nseg;       // reduce segments per physical window
 };
 
-static MsmShape msm_shape(u64 N, u32 c, u32 target_windows) {
+static const u32 kSortTile = 4096;  // 256 threads x 16 keys
+
+static MsmShape msm_shape(u64 N, u32 c, u32 target_windows, u32 task = 64) {
   MsmShape s;
   s.c = c;
   s.W = 254 / c + 1;
   s.n_windows = target_windows == 0 || target_windows > s.W ? s.W : target_windows;
   s.n_pre = (s.W + s.n_windows - 1) / s.n_windows;
   s.nb = 1u << (c - 1);
+  s.N = N;
+  s.seg_len = (u64)s.n_pre * N;
+  s.tiles = (u32)((s.seg_len + kSortTile - 1) / kSortTile);
+  if (s.tiles == 0) s.tiles = 1;
+  s.passes = (c + 3) / 4;
+  s.T = task;
+  s.g = 5;
+  s.task_cap = s.seg_len / s.T + s.nb + 1;
+  u64 max_tasks = s.seg_len / s.T + 1;
+  s.levels = 0;
+  while ((1ull << (s.g * s.levels)) < max_tasks && s.levels < 6) s.levels++;
   s.seg = s.nb >= 8 ? 8 : s.nb;
   s.nseg = s.nb / s.seg;
-  s.N = N;
   return s;
 }
 
 struct MsmLayout {
-  size_t keys, sorted, counts, offsets, cursor, buckets, partials, wsum, total;
+  size_t keysA, valsA, keysB, valsB, hist, bstart, bend, ntask, toff, partA, partB, rparts, wsum,
+      total;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -59,22 +84,20 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 static MsmLayout msm_layout(const MsmShape& s) {
   MsmLayout l;
   size_t o = 0;
-  l.keys = o;
-  o = align256(o + (size_t)s.W * s.N * 4);
-  l.sorted = o;
-  o = align256(o + (size_t)s.n_windows * s.n_pre * s.N * 4);
-  l.counts = o;
-  o = align256(o + (size_t)s.n_windows * (s.nb + 1) * 4);
-  l.offsets = o;
-  o = align256(o + (size_t)s.n_windows * (s.nb + 2) * 4);
-  l.cursor = o;
-  o = align256(o + (size_t)s.n_windows * (s.nb + 1) * 4);
-  l.buckets = o;
-  o = align256(o + (size_t)s.n_windows * s.nb * 128);
-  l.partials = o;
-  o = align256(o + (size_t)s.n_windows * s.nseg * 128);
-  l.wsum = o;
-  o = align256(o + (size_t)s.n_windows * 128);
+  const size_t ent = (size_t)s.n_windows * s.seg_len * 4;
+  l.keysA = o; o = align256(o + ent);
+  l.valsA = o; o = align256(o + ent);
+  l.keysB = o; o = align256(o + ent);
+  l.valsB = o; o = align256(o + ent);
+  l.hist = o; o = align256(o + (size_t)s.n_windows * 16 * s.tiles * 4 + 4 * s.n_windows);
+  l.bstart = o; o = align256(o + (size_t)s.n_windows * (s.nb + 1) * 4);
+  l.bend = o; o = align256(o + (size_t)s.n_windows * (s.nb + 1) * 4);
+  l.ntask = o; o = align256(o + (size_t)s.n_windows * (s.nb + 1) * 4);
+  l.toff = o; o = align256(o + (size_t)s.n_windows * (s.nb + 2) * 4);
+  l.partA = o; o = align256(o + (size_t)s.n_windows * s.task_cap * 128);
+  l.partB = o; o = align256(o + (size_t)s.n_windows * s.task_cap * 128);
+  l.rparts = o; o = align256(o + (size_t)s.n_windows * s.nseg * 128);
+  l.wsum = o; o = align256(o + (size_t)s.n_windows * 128);
   l.total = o;
   return l;
 }
@@ -85,50 +108,76 @@ __global__ void k_zero_u32(u32* p, u64 n) {
   if (i < n) p[i] = 0;
 }
 
-// signed digits (msm_impl.cuh:22-33 with digit == 2^(c-1) kept positive, so
-// the top window never carries out): key = |digit| | sign << 31
-__global__ void k_msm_digits(const u32* scalars, MsmShape s, u32* keys, u32* counts) {
+// Signed c-bit digits (msm_impl.cuh:22-33; a digit equal to 2^(c-1) stays
+// positive, so the top window never carries out).  Logical window w goes to
+// physical window w % n_windows at table offset (w / n_windows) * N
+// (msm_impl.cuh:56-67).  key = bucket |digit|, val = point index | sign << 31.
+__global__ void k_msm_digits(const u32* scalars, MsmShape s, u32* keys, u32* vals) {
   u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= s.N) return;
   Fr v = to_number<FrParams>(load_fe<Fr>(scalars + 8 * i));
   const u32 c = s.c, half = 1u << (c - 1), mask = (1u << c) - 1;
   u32 carry = 0;
-  for (u32 w = 0; w < s.W; w++) {
-    u32 bit = w * c, limb = bit >> 5, off = bit & 31;
-    u64 word = v.v[limb];
-    if (limb + 1 < 8) word |= (u64)v.v[limb + 1] << 32;
-    u32 d = (u32)((word >> off) & mask) + carry;
-    u32 key;
-    if (d > half) {
-      key = ((1u << c) - d) | 0x80000000u;
-      carry = 1;
-    } else {
-      key = d;
-      carry = 0;
+  const u32 slots = s.n_windows * s.n_pre;
+  for (u32 w = 0; w < slots; w++) {
+    u32 pw = w % s.n_windows, grp = w / s.n_windows;
+    u64 idx = (u64)pw * s.seg_len + (u64)grp * s.N + i;
+    u32 key = 0, sign = 0;
+    if (w < s.W) {
+      u32 bit = w * c, limb = bit >> 5, off = bit & 31;
+      u64 word = v.v[limb];
+      if (limb + 1 < 8) word |= (u64)v.v[limb + 1] << 32;
+      u32 d = (u32)((word >> off) & mask) + carry;
+      if (d > half) {
+        key = (1u << c) - d;
+        sign = 0x80000000u;
+        carry = 1;
+      } else {
+        key = d;
+        carry = 0;
+      }
     }
-    keys[(u64)w * s.N + i] = key;
-    u32 b = key & 0x7fffffffu;
-    if (b) atomicAdd(&counts[(w % s.n_windows) * (s.nb + 1) + b], 1u);
+    keys[idx] = key;
+    vals[idx] = (u32)((u64)grp * s.N + i) | sign;
   }
 }
 
-// one block per physical window: offsets[b] = sum_{b' < b} counts[b'],
-// offsets[nb + 1] = total; cursor = offsets
-__global__ __launch_bounds__(1024) void k_msm_scan(MsmShape s, const u32* counts, u32* offsets,
-                                                   u32* cursor) {
+// ---------------------------------------------------- LSD radix sort (4-bit)
+// Stable per pass: tiles of 16 consecutive keys per thread (blocked), block
+// ranks in (digit, thread) order, global bases from the scanned per-tile
+// histograms hist[(pw * 16 + digit) * tiles + tile].
+__global__ __launch_bounds__(256) void k_sort_hist(const u32* keys, MsmShape s, u32 shift,
+                                                   u32* hist) {
+  __shared__ u32 cnt[16];
+  const u32 tile = blockIdx.x, pw = blockIdx.y, t = threadIdx.x;
+  if (t < 16) cnt[t] = 0;
+  __syncthreads();
+  const u32* kk = keys + (u64)pw * s.seg_len;
+  u64 base = (u64)tile * kSortTile + (u64)t * 16;
+#pragma unroll
+  for (u32 j = 0; j < 16; j++) {
+    u64 idx = base + j;
+    if (idx < s.seg_len) atomicAdd(&cnt[(kk[idx] >> shift) & 15u], 1u);
+  }
+  __syncthreads();
+  if (t < 16) hist[((u64)pw * 16 + t) * s.tiles + tile] = cnt[t];
+}
+
+// exclusive scan of len entries per segment (in place allowed); with
+// write_total, out[len] = total (out_stride must then exceed len)
+__global__ __launch_bounds__(1024) void k_seg_scan(const u32* in, u32* out, u64 len,
+                                                   u64 in_stride, u64 out_stride,
+                                                   int write_total) {
   __shared__ u32 part[1024];
-  const u32 pw = blockIdx.x, n = s.nb + 1;
-  const u32* cnt = counts + pw * n;
-  u32* off = offsets + pw * (n + 1);
-  u32* cur = cursor + pw * n;
   const u32 T = blockDim.x, t = threadIdx.x;
-  const u32 per = (n + T - 1) / T;
-  u32 lo = t * per, hi = min(lo + per, n);
+  const u32* src = in + (u64)blockIdx.x * in_stride;
+  u32* dst = out + (u64)blockIdx.x * out_stride;
+  const u64 per = (len + T - 1) / T;
+  u64 lo = (u64)t * per, hi = lo + per < len ? lo + per : len;
   u32 sum = 0;
-  for (u32 b = lo; b < hi; b++) sum += cnt[b];
+  for (u64 b = lo; b < hi; b++) sum += src[b];
   part[t] = sum;
   __syncthreads();
-  // Hillis-Steele inclusive scan of the per-thread sums
   for (u32 d = 1; d < T; d <<= 1) {
     u32 v = t >= d ? part[t - d] : 0;
     __syncthreads();
@@ -136,70 +185,200 @@ __global__ __launch_bounds__(1024) void k_msm_scan(MsmShape s, const u32* counts
     __syncthreads();
   }
   u32 run = t ? part[t - 1] : 0;
-  for (u32 b = lo; b < hi; b++) {
-    off[b] = run;
-    cur[b] = run;
-    run += cnt[b];
+  for (u64 b = lo; b < hi; b++) {
+    u32 x = src[b];
+    dst[b] = run;
+    run += x;
   }
-  if (t == T - 1) off[n] = part[T - 1];
+  if (write_total && t == T - 1) dst[len] = part[T - 1];
 }
 
-__global__ void k_msm_scatter(MsmShape s, const u32* keys, u32* cursor, u32* sorted) {
+__global__ __launch_bounds__(256) void k_sort_scatter(const u32* ksrc, const u32* vsrc, u32* kdst,
+                                                      u32* vdst, MsmShape s, u32 shift,
+                                                      const u32* hist) {
+  __shared__ u32 cnt[16 * 256];
+  __shared__ u32 part[256];
+  __shared__ u32 gbase[16];
+  __shared__ u32 dbase[16];
+  const u32 tile = blockIdx.x, pw = blockIdx.y, t = threadIdx.x;
+  const u64 seg0 = (u64)pw * s.seg_len;
+  const u64 base = (u64)tile * kSortTile + (u64)t * 16;
+  u32 key[16], val[16];
+#pragma unroll
+  for (u32 d = 0; d < 16; d++) cnt[d * 256 + t] = 0;
+#pragma unroll
+  for (u32 j = 0; j < 16; j++) {
+    u64 idx = base + j;
+    if (idx < s.seg_len) {
+      key[j] = ksrc[seg0 + idx];
+      val[j] = vsrc[seg0 + idx];
+      cnt[((key[j] >> shift) & 15u) * 256 + t] += 1;
+    }
+  }
+  __syncthreads();
+  // exclusive scan of cnt in (digit, thread) order: thread t owns [16t, 16t+16)
+  u32 loc[16];
+  u32 sum = 0;
+#pragma unroll
+  for (u32 j = 0; j < 16; j++) {
+    loc[j] = cnt[t * 16 + j];
+    sum += loc[j];
+  }
+  part[t] = sum;
+  __syncthreads();
+  for (u32 d = 1; d < 256; d <<= 1) {
+    u32 v = t >= d ? part[t - d] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  u32 run = t ? part[t - 1] : 0;
+#pragma unroll
+  for (u32 j = 0; j < 16; j++) {
+    cnt[t * 16 + j] = run;
+    run += loc[j];
+  }
+  __syncthreads();
+  if (t < 16) {
+    gbase[t] = hist[((u64)pw * 16 + t) * s.tiles + tile];
+    dbase[t] = cnt[t * 256];
+  }
+  __syncthreads();
+  // rank within digit d = scan(d, t) - scan(d, 0); cnt column t is private
+#pragma unroll
+  for (u32 j = 0; j < 16; j++) {
+    u64 idx = base + j;
+    if (idx < s.seg_len) {
+      u32 d = (key[j] >> shift) & 15u;
+      u32 r = cnt[d * 256 + t];
+      cnt[d * 256 + t] = r + 1;
+      u64 pos = (u64)gbase[d] + r - dbase[d];
+      kdst[seg0 + pos] = key[j];
+      vdst[seg0 + pos] = val[j];
+    }
+  }
+}
+
+// bucket run bounds in the sorted keys (bucket 0 = zero digit, skipped)
+__global__ void k_bucket_bounds(const u32* keys, MsmShape s, u32* bstart, u32* bend) {
   u64 gi = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gi >= (u64)s.W * s.N) return;
-  u32 w = (u32)(gi / s.N);
-  u64 i = gi - (u64)w * s.N;
-  u32 key = keys[gi];
-  u32 b = key & 0x7fffffffu;
-  if (!b) return;
-  u32 pw = w % s.n_windows, grp = w / s.n_windows;
-  u32 pos = atomicAdd(&cursor[pw * (s.nb + 1) + b], 1u);
-  sorted[(u64)pw * s.n_pre * s.N + pos] = (u32)((u64)grp * s.N + i) | (key & 0x80000000u);
+  if (gi >= (u64)s.n_windows * s.seg_len) return;
+  u32 pw = (u32)(gi / s.seg_len);
+  u64 i = gi - (u64)pw * s.seg_len;
+  const u32* kk = keys + (u64)pw * s.seg_len;
+  u32 k = kk[i];
+  if (k == 0) return;
+  u32* st = bstart + (u64)pw * (s.nb + 1);
+  u32* en = bend + (u64)pw * (s.nb + 1);
+  if (i == 0 || kk[i - 1] != k) st[k] = (u32)i;
+  if (i + 1 == s.seg_len || kk[i + 1] != k) en[k] = (u32)(i + 1);
 }
 
-__global__ void k_msm_buckets(MsmShape s, const u32* bases, const u32* offsets,
-                              const u32* sorted, u32* buckets) {
-  u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (u64)s.n_windows * s.nb) return;
-  u32 pw = (u32)(t / s.nb), b = (u32)(t % s.nb) + 1;
-  const u32* off = offsets + pw * (s.nb + 2);
-  u32 lo = off[b], hi = off[b + 1];
-  const u32* list = sorted + (u64)pw * s.n_pre * s.N;
-  G1Xyzz acc = G1Xyzz::identity();
-  for (u32 k = lo; k < hi; k++) {
-    u32 e = list[k];
-    G1Aff p = load_aff(bases + 16 * (u64)(e & 0x7fffffffu));
-    if (e & 0x80000000u) p = aff_neg(p);
-    acc = xyzz_madd(acc, p);
+__global__ void k_task_counts(MsmShape s, const u32* bstart, const u32* bend, u32* ntask) {
+  u64 gi = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= (u64)s.n_windows * (s.nb + 1)) return;
+  u32 b = (u32)(gi % (s.nb + 1));
+  u32 cnt = bend[gi] - bstart[gi];
+  ntask[gi] = b == 0 ? 0u : (cnt + s.T - 1) / s.T;
+}
+
+// largest b in [1, nb] with toff[b] <= t
+__device__ __forceinline__ u32 find_bucket(const u32* toff, u32 nb, u32 t) {
+  u32 lo = 1, hi = nb;
+  while (lo < hi) {
+    u32 mid = (lo + hi + 1) >> 1;
+    if (toff[mid] <= t) lo = mid;
+    else hi = mid - 1;
   }
-  store_xyzz(buckets + 32 * t, acc);
+  return lo;
 }
 
-// segment [lo, lo + seg) of buckets (bucket index b = lo + k + 1):
+// one task = up to T consecutive entries of one bucket, mixed XYZZ + affine
+// additions; the next point is loaded while the current one is added.
+__global__ __launch_bounds__(256) void k_bucket_tasks(const u32* bases, const u32* vals,
+                                                      MsmShape s, const u32* bstart,
+                                                      const u32* bend, const u32* toff,
+                                                      u32* parts) {
+  u64 gi = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  u32 pw = (u32)(gi / s.task_cap);
+  if (pw >= s.n_windows) return;
+  u32 t = (u32)(gi - (u64)pw * s.task_cap);
+  const u32* to = toff + (u64)pw * (s.nb + 2);
+  if (t >= to[s.nb + 1]) return;
+  u32 b = find_bucket(to, s.nb, t);
+  u32 k = t - to[b];
+  u32 lo = bstart[(u64)pw * (s.nb + 1) + b] + k * s.T;
+  u32 be = bend[(u64)pw * (s.nb + 1) + b];
+  u32 hi = lo + s.T < be ? lo + s.T : be;
+  const u32* vv = vals + (u64)pw * s.seg_len;
+  G1Xyzz acc = G1Xyzz::identity();
+  u32 e = vv[lo];
+  G1Aff p = load_aff(bases + 16 * (u64)(e & 0x7fffffffu));
+  for (u32 j = lo; j < hi; j++) {
+    G1Aff cur = p;
+    u32 ce = e;
+    if (j + 1 < hi) {
+      e = vv[j + 1];
+      p = load_aff(bases + 16 * (u64)(e & 0x7fffffffu));
+    }
+    if (ce & 0x80000000u) cur = aff_neg(cur);
+    acc = xyzz_madd(acc, cur);
+  }
+  store_xyzz(parts + 32 * ((u64)pw * s.task_cap + t), acc);
+}
+
+// level l of the per-bucket partial-sum tree: group k of bucket b sums
+// src[toff_b + kG .. toff_b + kG + G) (live entries only) into dst[toff_b + k]
+__global__ __launch_bounds__(256) void k_bucket_combine(const u32* src, u32* dst, MsmShape s,
+                                                        const u32* toff, u32 level) {
+  u64 gi = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  u32 pw = (u32)(gi / s.task_cap);
+  if (pw >= s.n_windows) return;
+  u32 t = (u32)(gi - (u64)pw * s.task_cap);
+  const u32* to = toff + (u64)pw * (s.nb + 2);
+  if (t >= to[s.nb + 1]) return;
+  u32 b = find_bucket(to, s.nb, t);
+  u32 k = t - to[b];
+  u32 nt = to[b + 1] - to[b];
+  u32 sh_prev = s.g * (level - 1), sh = s.g * level;
+  u32 n_prev = (u32)(((u64)nt + (1ull << sh_prev) - 1) >> sh_prev);
+  u32 n_cur = (u32)(((u64)nt + (1ull << sh) - 1) >> sh);
+  if (k >= n_cur) return;
+  const u64 base = (u64)pw * s.task_cap + to[b];
+  const u32 G = 1u << s.g;
+  G1Xyzz acc = load_xyzz(src + 32 * (base + (u64)k * G));
+  for (u32 j = 1; j < G && k * G + j < n_prev; j++)
+    acc = xyzz_add(acc, load_xyzz(src + 32 * (base + (u64)k * G + j)));
+  store_xyzz(dst + 32 * (base + k), acc);
+}
+
+// segment [lo, lo + seg) of buckets (bucket number b = lo + k + 1):
 // partial = sum_k (lo + k + 1) * B = T + lo * R with running sums
-__global__ void k_msm_reduce(MsmShape s, const u32* buckets, u32* partials) {
+__global__ void k_msm_reduce(MsmShape s, const u32* parts, const u32* toff, u32* rparts) {
   u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (u64)s.n_windows * s.nseg) return;
   u32 pw = (u32)(t / s.nseg), sg = (u32)(t % s.nseg);
   u32 lo = sg * s.seg;
-  const u32* bk = buckets + 32 * ((u64)pw * s.nb + lo);
+  const u32* to = toff + (u64)pw * (s.nb + 2);
+  const u32* pp = parts + 32 * (u64)pw * s.task_cap;
   G1Xyzz run = G1Xyzz::identity(), tot = G1Xyzz::identity();
   for (int k = (int)s.seg - 1; k >= 0; k--) {
-    run = xyzz_add(run, load_xyzz(bk + 32 * k));
+    u32 b = lo + k + 1;
+    if (to[b + 1] > to[b]) run = xyzz_add(run, load_xyzz(pp + 32 * (u64)to[b]));
     tot = xyzz_add(tot, run);
   }
   if (lo) tot = xyzz_add(tot, xyzz_mul_u32(run, lo));
-  store_xyzz(partials + 32 * t, tot);
+  store_xyzz(rparts + 32 * t, tot);
 }
 
 // one block per physical window: sum of its segment partials
-__global__ __launch_bounds__(256) void k_msm_sum_partials(MsmShape s, const u32* partials,
+__global__ __launch_bounds__(256) void k_msm_sum_partials(MsmShape s, const u32* rparts,
                                                           u32* wsum) {
   __shared__ u32 red[256 * 32];
   const u32 pw = blockIdx.x, t = threadIdx.x;
   G1Xyzz acc = G1Xyzz::identity();
   for (u32 k = t; k < s.nseg; k += blockDim.x)
-    acc = xyzz_add(acc, load_xyzz(partials + 32 * ((u64)pw * s.nseg + k)));
+    acc = xyzz_add(acc, load_xyzz(rparts + 32 * ((u64)pw * s.nseg + k)));
   store_xyzz(red + 32 * t, acc);
   __syncthreads();
   for (u32 d = blockDim.x / 2; d > 0; d >>= 1) {
@@ -209,11 +388,10 @@ __global__ __launch_bounds__(256) void k_msm_sum_partials(MsmShape s, const u32*
     }
     __syncthreads();
   }
-  if (t == 0) {
-    for (int i = 0; i < 32; i++) wsum[32 * pw + i] = red[i];
-  }
+  if (t < 32) wsum[32 * pw + t] = red[t];
 }
 
+// Horner over physical windows, then affine
 __global__ void k_msm_final(MsmShape s, const u32* wsum, u32* result) {
   G1Xyzz acc = load_xyzz(wsum + 32 * (s.n_windows - 1));
   for (int pw = (int)s.n_windows - 2; pw >= 0; pw--) {
@@ -221,6 +399,10 @@ __global__ void k_msm_final(MsmShape s, const u32* wsum, u32* result) {
     acc = xyzz_add(acc, load_xyzz(wsum + 32 * pw));
   }
   store_aff(result, xyzz_to_aff(acc));
+}
+
+__global__ void k_store_identity_aff(u32* result) {
+  if (threadIdx.x < 16) result[threadIdx.x] = 0;
 }
 
 // table t = 2^(t * step) * table 0 (affine), t = 1 .. n_pre-1
@@ -236,45 +418,104 @@ __global__ void k_msm_precompute(u32* tables, u64 N, u32 n_pre, u32 step) {
 }
 
 // ------------------------------------------------------------------ driver
+static inline dim3 grid1(u64 n, u32 block) { return dim3((u32)((n + block - 1) / block)); }
+
+// bases: n_pre tables of N affine points (table t = 2^(t * n_windows * c) * P)
 static hipError_t msm_run(const u32* bases, const u32* scalars, const MsmShape& s,
                           u32* result, char* ws, hipStream_t stream) {
+  if (s.N == 0) {
+    hipLaunchKernelGGL(k_store_identity_aff, dim3(1), dim3(64), 0, stream, result);
+    return hipGetLastError();
+  }
   MsmLayout l = msm_layout(s);
-  u32* keys = (u32*)(ws + l.keys);
-  u32* sorted = (u32*)(ws + l.sorted);
-  u32* counts = (u32*)(ws + l.counts);
-  u32* offsets = (u32*)(ws + l.offsets);
-  u32* cursor = (u32*)(ws + l.cursor);
-  u32* buckets = (u32*)(ws + l.buckets);
-  u32* partials = (u32*)(ws + l.partials);
+  u32* kA = (u32*)(ws + l.keysA);
+  u32* vA = (u32*)(ws + l.valsA);
+  u32* kB = (u32*)(ws + l.keysB);
+  u32* vB = (u32*)(ws + l.valsB);
+  u32* hist = (u32*)(ws + l.hist);
+  u32* bstart = (u32*)(ws + l.bstart);
+  u32* bend = (u32*)(ws + l.bend);
+  u32* ntask = (u32*)(ws + l.ntask);
+  u32* toff = (u32*)(ws + l.toff);
+  u32* partA = (u32*)(ws + l.partA);
+  u32* partB = (u32*)(ws + l.partB);
+  u32* rparts = (u32*)(ws + l.rparts);
   u32* wsum = (u32*)(ws + l.wsum);
-  const u32 T = 256;
-  u64 ncount = (u64)s.n_windows * (s.nb + 1);
-  hipLaunchKernelGGL(k_zero_u32, dim3((u32)((ncount + T - 1) / T)), dim3(T), 0, stream, counts,
-                     ncount);
-  if (s.N) {
-    { TimedScope ts_(K_MSM_DIGITS, stream); hipLaunchKernelGGL(k_msm_digits, dim3((u32)((s.N + T - 1) / T)), dim3(T), 0, stream,
-                       scalars, s, keys, counts); }
+  const u32 TB = 256;
+  {
+    TimedScope ts_(K_MSM_DIGITS, stream);
+    hipLaunchKernelGGL(k_msm_digits, grid1(s.N, TB), dim3(TB), 0, stream, scalars, s, kA, vA);
   }
-  { TimedScope ts_(K_MSM_SCAN, stream); hipLaunchKernelGGL(k_msm_scan, dim3(s.n_windows), dim3(1024), 0, stream, s, counts, offsets,
-                     cursor); }
-  u64 nk = (u64)s.W * s.N;
-  if (nk) {
-    { TimedScope ts_(K_MSM_SCATTER, stream); hipLaunchKernelGGL(k_msm_scatter, dim3((u32)((nk + T - 1) / T)), dim3(T), 0, stream, s,
-                       keys, cursor, sorted); }
+  {
+    TimedScope ts_(K_MSM_SORT, stream);
+    u32 *ks = kA, *vs = vA, *kd = kB, *vd = vB;
+    dim3 tg(s.tiles, s.n_windows);
+    for (u32 p = 0; p < s.passes; p++) {
+      hipLaunchKernelGGL(k_sort_hist, tg, dim3(256), 0, stream, ks, s, 4 * p, hist);
+      hipLaunchKernelGGL(k_seg_scan, dim3(s.n_windows), dim3(1024), 0, stream, hist, hist,
+                         (u64)16 * s.tiles, (u64)16 * s.tiles, (u64)16 * s.tiles, 0);
+      hipLaunchKernelGGL(k_sort_scatter, tg, dim3(256), 0, stream, ks, vs, kd, vd, s, 4 * p,
+                         hist);
+      u32* tk = ks; ks = kd; kd = tk;
+      u32* tv = vs; vs = vd; vd = tv;
+    }
+    kA = ks;  // sorted keys / values
+    vA = vs;
   }
-  u64 nbk = (u64)s.n_windows * s.nb;
-  { TimedScope ts_(K_MSM_BUCKETS, stream); hipLaunchKernelGGL(k_msm_buckets, dim3((u32)((nbk + T - 1) / T)), dim3(T), 0, stream, s, bases,
-                     offsets, sorted, buckets); }
-  u64 nsg = (u64)s.n_windows * s.nseg;
-  { TimedScope ts_(K_MSM_REDUCE, stream); hipLaunchKernelGGL(k_msm_reduce, dim3((u32)((nsg + T - 1) / T)), dim3(T), 0, stream, s,
-                     buckets, partials); }
-  { TimedScope ts_(K_MSM_SUM, stream); hipLaunchKernelGGL(k_msm_sum_partials, dim3(s.n_windows), dim3(256), 0, stream, s, partials,
-                     wsum); }
-  { TimedScope ts_(K_MSM_FINAL, stream); hipLaunchKernelGGL(k_msm_final, dim3(1), dim3(1), 0, stream, s, wsum, result); }
+  {
+    TimedScope ts_(K_MSM_SCAN, stream);
+    u64 nbw = (u64)s.n_windows * (s.nb + 1);
+    hipLaunchKernelGGL(k_zero_u32, grid1(nbw, TB), dim3(TB), 0, stream, bstart, nbw);
+    hipLaunchKernelGGL(k_zero_u32, grid1(nbw, TB), dim3(TB), 0, stream, bend, nbw);
+    hipLaunchKernelGGL(k_bucket_bounds, grid1((u64)s.n_windows * s.seg_len, TB), dim3(TB), 0,
+                       stream, kA, s, bstart, bend);
+    hipLaunchKernelGGL(k_task_counts, grid1(nbw, TB), dim3(TB), 0, stream, s, bstart, bend,
+                       ntask);
+    hipLaunchKernelGGL(k_seg_scan, dim3(s.n_windows), dim3(1024), 0, stream, ntask, toff,
+                       (u64)(s.nb + 1), (u64)(s.nb + 1), (u64)(s.nb + 2), 1);
+  }
+  const u64 ntasks = (u64)s.n_windows * s.task_cap;
+  {
+    TimedScope ts_(K_MSM_BUCKETS, stream);
+    hipLaunchKernelGGL(k_bucket_tasks, grid1(ntasks, TB), dim3(TB), 0, stream, bases, vA, s,
+                       bstart, bend, toff, partA);
+  }
+  u32* fin = partA;
+  {
+    TimedScope ts_(K_MSM_COMBINE, stream);
+    u32 *src = partA, *dst = partB;
+    for (u32 lv = 1; lv <= s.levels; lv++) {
+      hipLaunchKernelGGL(k_bucket_combine, grid1(ntasks, TB), dim3(TB), 0, stream, src, dst, s,
+                         toff, lv);
+      u32* tmp = src; src = dst; dst = tmp;
+    }
+    fin = src;
+  }
+  {
+    TimedScope ts_(K_MSM_REDUCE, stream);
+    hipLaunchKernelGGL(k_msm_reduce, grid1((u64)s.n_windows * s.nseg, TB), dim3(TB), 0, stream,
+                       s, fin, toff, rparts);
+  }
+  {
+    TimedScope ts_(K_MSM_SUM, stream);
+    hipLaunchKernelGGL(k_msm_sum_partials, dim3(s.n_windows), dim3(256), 0, stream, s, rparts,
+                       wsum);
+  }
+  {
+    TimedScope ts_(K_MSM_FINAL, stream);
+    hipLaunchKernelGGL(k_msm_final, dim3(1), dim3(1), 0, stream, s, wsum, result);
+  }
   return hipGetLastError();
 }
 
 static bool valid_window(u32 c) { return c >= 4 && c <= 20; }
+
+static hipError_t precompute_device(u32* tables, u64 N, const MsmShape& s, hipStream_t stream) {
+  if (s.n_pre <= 1 || N == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_msm_precompute, grid1(N, 128), dim3(128), 0, stream, tables, N, s.n_pre,
+                     s.n_windows * s.c);
+  return hipGetLastError();
+}
 
 // Host-memory contract of the reference (msm.cu:9-41): copy, run, copy back.
 static hipError_t msm_host(u32 window, u32 target, void* const* buffers,
@@ -333,9 +574,8 @@ static hipError_t msm_precompute_host(u32 window, u32 target, unsigned long long
   u32* d = nullptr;
   IMP_CHECK(hipMalloc((void**)&d, (size_t)s.n_pre * len * 64));
   IMP_CHECK(hipMemcpy(d, h_points[0], (size_t)len * 64, hipMemcpyHostToDevice));
-  hipLaunchKernelGGL(k_msm_precompute, dim3((u32)((len + 255) / 256)), dim3(256), 0, 0, d,
-                     (u64)len, s.n_pre, s.n_windows * s.c);
-  IMP_CHECK(hipGetLastError());
+  IMP_CHECK(precompute_device(d, len, s, 0));
+  IMP_CHECK(hipDeviceSynchronize());
   for (u32 t = 1; t < s.n_pre; t++)
     IMP_CHECK(hipMemcpy(h_points[t], d + (size_t)16 * t * len, (size_t)len * 64,
                         hipMemcpyDeviceToHost));
@@ -399,4 +639,36 @@ extern "C" hipError_t impeller_msm(const uint32_t* bases_d, const uint32_t* scal
   MsmShape s = msm_shape(len, window, 0);
   IMP_REQUIRE(workspace_bytes >= msm_layout(s).total, "workspace too small");
   return msm_run(bases_d, scalars_d, s, result_d, (char*)workspace, stream);
+}
+
+extern "C" size_t impeller_msm_cfg_workspace_size(unsigned long long len, unsigned int window,
+                                                  unsigned int target_windows,
+                                                  unsigned int task) {
+  if (!valid_window(window) || task == 0) return 0;
+  return msm_layout(msm_shape(len, window, target_windows, task)).total;
+}
+
+extern "C" unsigned int impeller_msm_tables(unsigned int window, unsigned int target_windows) {
+  if (!valid_window(window)) return 0;
+  return msm_shape(1, window, target_windows).n_pre;
+}
+
+extern "C" hipError_t impeller_msm_precompute(uint32_t* tables_d, unsigned long long len,
+                                              unsigned int window, unsigned int target_windows,
+                                              hipStream_t stream) {
+  IMP_REQUIRE(valid_window(window), "window");
+  IMP_REQUIRE(tables_d || len == 0, "null pointer");
+  return precompute_device(tables_d, len, msm_shape(len, window, target_windows), stream);
+}
+
+extern "C" hipError_t impeller_msm_cfg(const uint32_t* tables_d, const uint32_t* scalars_d,
+                                       unsigned long long len, unsigned int window,
+                                       unsigned int target_windows, unsigned int task,
+                                       uint32_t* result_d, void* workspace,
+                                       size_t workspace_bytes, hipStream_t stream) {
+  IMP_REQUIRE(valid_window(window) && task > 0 && task <= 4096, "window / task");
+  IMP_REQUIRE(result_d && workspace && (len == 0 || (tables_d && scalars_d)), "null pointer");
+  MsmShape s = msm_shape(len, window, target_windows, task);
+  IMP_REQUIRE(workspace_bytes >= msm_layout(s).total, "workspace too small");
+  return msm_run(tables_d, scalars_d, s, result_d, (char*)workspace, stream);
 }

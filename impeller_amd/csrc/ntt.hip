@@ -72,80 +72,157 @@ __device__ __forceinline__ Fr twiddle_at(const Twiddle& tw, u64 e) {
   return mul<FrParams>(a, load_fe<Fr>(tw.hi + 8 * eh));
 }
 
-// --------------------------------------------------------- bit reversal
-// q = (a: 5 bits | b: L-10 bits | c: 5 bits), rev(q) = (rev c | rev b | rev a)
-template <bool PRO>
-__global__ __launch_bounds__(256) void k_bitrev_tiles(View x, u32 L, Factor pro) {
-  extern __shared__ uint4 lds4[];
-  u32* t0 = reinterpret_cast<u32*>(lds4);
-  u32* t1 = t0 + 1024 * 8;
-  const u32 mb = L - 10;
-  const u64 b = blockIdx.x;
-  const u64 rb = bit_reverse(b, mb);
+// ------------------------------------------------------ butterfly helpers
+// radix-2: (a, b) <- (a + w b, a - w b); e == 0 means w = 1
+__device__ __forceinline__ Fr tw_load(const Twiddle& tw, bool two, u64 e) {
+  return two ? twiddle_at<true>(tw, e) : twiddle_at<false>(tw, e);
+}
+
+__device__ __forceinline__ void bfly2(u32* pa, u32* pb, u64 e, const Twiddle& tw, bool two) {
+  Fr a = load_fe<Fr>(pa), b = load_fe<Fr>(pb);
+  if (e != 0) b = mul<FrParams>(b, tw_load(tw, two, e));
+  store_fe<Fr>(pa, add<FrParams>(a, b));
+  store_fe<Fr>(pb, sub<FrParams>(a, b));
+}
+
+// Two consecutive DIT levels on positions g0 < g1 = g0 + 2^j < g2 = g0 +
+// 2^(j+1) < g3: level j pairs (g0,g1),(g2,g3) with exponent e1; level j+1
+// pairs (g0,g2) with e2 and (g1,g3) with e3.  The same butterflies, in the
+// same order per element, as two radix-2 levels.
+__device__ __forceinline__ void bfly4(u32* p0, u32* p1, u32* p2, u32* p3, u64 e1, u64 e2, u64 e3,
+                                      const Twiddle& tw, bool two) {
+  Fr a0 = load_fe<Fr>(p0), a1 = load_fe<Fr>(p1), a2 = load_fe<Fr>(p2), a3 = load_fe<Fr>(p3);
+  if (e1 != 0) {
+    Fr w1 = tw_load(tw, two, e1);
+    a1 = mul<FrParams>(a1, w1);
+    a3 = mul<FrParams>(a3, w1);
+  }
+  Fr b0 = add<FrParams>(a0, a1), b1 = sub<FrParams>(a0, a1);
+  Fr b2 = add<FrParams>(a2, a3), b3 = sub<FrParams>(a2, a3);
+  if (e2 != 0) b2 = mul<FrParams>(b2, tw_load(tw, two, e2));
+  b3 = mul<FrParams>(b3, tw_load(tw, two, e3));
+  store_fe<Fr>(p0, add<FrParams>(b0, b2));
+  store_fe<Fr>(p2, sub<FrParams>(b0, b2));
+  store_fe<Fr>(p1, add<FrParams>(b1, b3));
+  store_fe<Fr>(p3, sub<FrParams>(b1, b3));
+}
+
+// ------------------------------------------------------------ job descriptor
+// One transform = a bit-reversal stage (optionally with the first 5 DIT
+// levels) and nst LDS passes.  src is read by the bit-reversal stage (the
+// same view as x for an in-place transform); source positions >= src_len
+// read as zero (zero-extension of a 2^k polynomial to 2^(k+e)).
+struct NttStage {
+  u32 s, D, logC;
+  u32 blocks;
+};
+
+struct NttJob {
+  View x;
+  View src;
+  u64 src_len;
+  u32 L;
+  u32 fuse;  // 0 or 5 (L >= 10)
+  u32 two;   // two-level twiddles
+  Twiddle tw;
+  Factor pro, epi;  // tab == nullptr: none
+  u32 nst;
+  u32 br_blocks;
+  NttStage st[8];
+};
+
+__device__ __forceinline__ Fr src_load(const NttJob& J, u64 q) {
+  if (q >= J.src_len) return Fr::zero();
+  Fr v = load_fe<Fr>(J.src.base + 8 * view_phys(J.src, q));
+  if (J.pro.tab) v = mul<FrParams>(v, factor_at(J.pro, q));
+  return v;
+}
+
+// Bit reversal on tile pairs: q = (a | b | c) (5 | L-10 | 5 bits) moves to
+// p = rev(q) = (rev c | rev b | rev a), i.e. from tile b to tile rev(b); a
+// work-group owns both tiles, so the permutation can run in place.  In LDS
+// the destination tiles are laid out by position [a * 32 + c], whose low 5
+// bits c are the bits DIT levels 0..4 pair up (exponent (c mod 2^j) 2^(L-j-1)).
+__device__ void br_tile_body(const NttJob& J, u64 b, u32* lds) {
+  const u32 L = J.L;
+  u32* d0 = lds;              // positions of tile b
+  u32* d1 = lds + 1024 * 8;   // positions of tile rev(b)
+  const u64 rb = bit_reverse(b, L - 10);
   if (b > rb) return;
   const bool self = (b == rb);
-  for (u32 idx = threadIdx.x; idx < 1024; idx += blockDim.x) {
-    u64 a = idx >> 5, c = idx & 31;
-    u64 q0 = (a << (L - 5)) | (b << 5) | c;
-    Fr v0 = load_fe<Fr>(x.base + 8 * view_phys(x, q0));
-    if (PRO) v0 = mul<FrParams>(v0, factor_at(pro, q0));
-    store_fe<Fr>(t0 + 8 * idx, v0);
-    if (!self) {
-      u64 q1 = (a << (L - 5)) | (rb << 5) | c;
-      Fr v1 = load_fe<Fr>(x.base + 8 * view_phys(x, q1));
-      if (PRO) v1 = mul<FrParams>(v1, factor_at(pro, q1));
-      store_fe<Fr>(t1 + 8 * idx, v1);
-    }
-  }
   __syncthreads();
   for (u32 idx = threadIdx.x; idx < 1024; idx += blockDim.x) {
     u32 a = idx >> 5, c = idx & 31;
-    u32 src = (brev_bits(c, 5) << 5) | brev_bits(a, 5);
-    u64 q0 = ((u64)a << (L - 5)) | (b << 5) | c;
-    const u32* from0 = self ? t0 : t1;
-    store_fe<Fr>(x.base + 8 * view_phys(x, q0), load_fe<Fr>(from0 + 8 * src));
+    u32 slot = (brev_bits(c, 5) << 5) | brev_bits(a, 5);
+    store_fe<Fr>((self ? d0 : d1) + 8 * slot, src_load(J, ((u64)a << (L - 5)) | (b << 5) | c));
+    if (!self) store_fe<Fr>(d0 + 8 * slot, src_load(J, ((u64)a << (L - 5)) | (rb << 5) | c));
+  }
+  __syncthreads();
+  if (J.fuse == 5) {
+    const bool two = J.two != 0;
+    const u32 nbuf = self ? 1 : 2;
+    for (u32 j = 0; j < 4; j += 2) {
+      for (u32 qi = threadIdx.x; qi < nbuf * 256; qi += blockDim.x) {
+        u32* row = ((qi >> 8) ? d1 : d0) + 8 * (((qi & 255) >> 3) * 32);
+        u32 k = qi & 7;
+        u32 glo = k & ((1u << j) - 1);
+        u32 c0 = ((k >> j) << (j + 2)) | glo;
+        u64 e1 = (u64)glo << (L - j - 1);
+        u64 e2 = (u64)glo << (L - j - 2);
+        u64 e3 = ((u64)glo + (1u << j)) << (L - j - 2);
+        bfly4(row + 8 * c0, row + 8 * (c0 | (1u << j)), row + 8 * (c0 | (2u << j)),
+              row + 8 * (c0 | (3u << j)), e1, e2, e3, J.tw, two);
+      }
+      __syncthreads();
+    }
+    for (u32 bi = threadIdx.x; bi < nbuf * 512; bi += blockDim.x) {
+      u32* row = ((bi >> 9) ? d1 : d0) + 8 * (((bi & 511) >> 4) * 32);
+      u32 k = bi & 15;
+      bfly2(row + 8 * k, row + 8 * (k | 16u), (u64)k << (L - 5), J.tw, two);
+    }
+    __syncthreads();
+  }
+  for (u32 idx = threadIdx.x; idx < 1024; idx += blockDim.x) {
+    u32 a = idx >> 5, c = idx & 31;
+    u64 p0 = ((u64)a << (L - 5)) | (b << 5) | c;
+    store_fe<Fr>(J.x.base + 8 * view_phys(J.x, p0), load_fe<Fr>(d0 + 8 * idx));
     if (!self) {
-      u64 q1 = ((u64)a << (L - 5)) | (rb << 5) | c;
-      store_fe<Fr>(x.base + 8 * view_phys(x, q1), load_fe<Fr>(t0 + 8 * src));
+      u64 p1 = ((u64)a << (L - 5)) | (rb << 5) | c;
+      store_fe<Fr>(J.x.base + 8 * view_phys(J.x, p1), load_fe<Fr>(d1 + 8 * idx));
     }
   }
 }
 
-// whole array (n <= 512) in one work-group
-template <bool PRO>
-__global__ __launch_bounds__(256) void k_bitrev_small(View x, u32 L, Factor pro) {
-  extern __shared__ uint4 lds4[];
-  u32* t = reinterpret_cast<u32*>(lds4);
-  const u32 n = 1u << L;
-  for (u32 i = threadIdx.x; i < n; i += blockDim.x) {
-    Fr v = load_fe<Fr>(x.base + 8 * view_phys(x, i));
-    if (PRO) v = mul<FrParams>(v, factor_at(pro, i));
-    store_fe<Fr>(t + 8 * i, v);
-  }
+// whole array (L < 10) in one work-group
+__device__ void br_small_body(const NttJob& J, u32* lds) {
+  const u32 n = 1u << J.L;
   __syncthreads();
   for (u32 i = threadIdx.x; i < n; i += blockDim.x)
-    store_fe<Fr>(x.base + 8 * view_phys(x, i), load_fe<Fr>(t + 8 * brev_bits(i, L)));
+    store_fe<Fr>(lds + 8 * brev_bits(i, J.L), src_load(J, i));
+  __syncthreads();
+  for (u32 i = threadIdx.x; i < n; i += blockDim.x)
+    store_fe<Fr>(J.x.base + 8 * view_phys(J.x, i), load_fe<Fr>(lds + 8 * i));
 }
 
-// ------------------------------------------------------------- DIT pass
-// Levels s .. s+D-1 on positions p = hi*2^(s+D) + g*2^s + lo.
+__device__ __forceinline__ void br_body(const NttJob& J, u64 b, u32* lds) {
+  if (J.L >= 10) br_tile_body(J, b, lds);
+  else br_small_body(J, lds);
+}
+
+// LDS pass over levels s .. s+D-1 on positions p = hi*2^(s+D) + g*2^s + lo.
 // contiguous (s == 0): work-group = C consecutive groups of G positions.
-// strided (2^s >= C): work-group = fixed hi, C consecutive lo.
-template <bool TWO, bool EPI>
-__global__ __launch_bounds__(256) void k_dit_pass(View x, u32 L, u32 s, u32 D, u32 logC,
-                                                  Twiddle tw, Factor epi) {
-  extern __shared__ uint4 lds4[];
-  u32* t = reinterpret_cast<u32*>(lds4);
+// strided (2^s >= C): work-group = fixed hi, C consecutive lo.  The last pass
+// multiplies each output by the epilogue factor (n^-1, coset un-shift).
+__device__ void dit_body(const NttJob& J, const NttStage& S, bool last, u64 wg, u32* t) {
+  const u32 L = J.L, s = S.s, D = S.D, logC = S.logC;
   const u32 C = 1u << logC, G = 1u << D, GC = G * C;
-  const u64 wg = blockIdx.x;
-  const bool contig = (s == 0);
+  const bool contig = (s == 0), two = J.two != 0;
   u64 hi = 0, lo0 = 0;
   if (!contig) {
     hi = wg >> (s - logC);
     lo0 = (wg & ((1ull << (s - logC)) - 1)) << logC;
   }
-
-  // load: consecutive threads read consecutive addresses
+  __syncthreads();
   for (u32 idx = threadIdx.x; idx < GC; idx += blockDim.x) {
     u32 g, c;
     u64 p;
@@ -158,30 +235,40 @@ __global__ __launch_bounds__(256) void k_dit_pass(View x, u32 L, u32 s, u32 D, u
       c = idx & (C - 1);
       p = (hi << (s + D)) | ((u64)g << s) | (lo0 + c);
     }
-    store_fe<Fr>(t + 8 * (g * C + c), load_fe<Fr>(x.base + 8 * view_phys(x, p)));
+    store_fe<Fr>(t + 8 * (g * C + c), load_fe<Fr>(J.x.base + 8 * view_phys(J.x, p)));
   }
   __syncthreads();
-
-  const u32 nb = GC >> 1;
-  for (u32 j = 0; j < D; j++) {
+  u32 j = 0;
+  for (; j + 1 < D; j += 2) {
+    const u32 l = s + j;
+    const u32 sh1 = L - l - 2;
+    for (u32 qi = threadIdx.x; qi < (GC >> 2); qi += blockDim.x) {
+      u32 c = qi & (C - 1), k = qi >> logC;
+      u32 glo = k & ((1u << j) - 1);
+      u32 g0 = ((k >> j) << (j + 2)) | glo;
+      u64 lo = contig ? 0 : lo0 + c;
+      u64 base = (((u64)glo) << s) + lo;
+      u64 e1 = base << (sh1 + 1), e2 = base << sh1, e3 = (base + (1ull << l)) << sh1;
+      bfly4(t + 8 * (g0 * C + c), t + 8 * ((g0 | (1u << j)) * C + c),
+            t + 8 * ((g0 | (2u << j)) * C + c), t + 8 * ((g0 | (3u << j)) * C + c), e1, e2, e3,
+            J.tw, two);
+    }
+    __syncthreads();
+  }
+  if (j < D) {
     const u32 l = s + j;
     const u32 sh = L - l - 1;
-    for (u32 bi = threadIdx.x; bi < nb; bi += blockDim.x) {
+    for (u32 bi = threadIdx.x; bi < (GC >> 1); bi += blockDim.x) {
       u32 c = bi & (C - 1), k = bi >> logC;
       u32 glo = k & ((1u << j) - 1);
       u32 g0 = ((k >> j) << (j + 1)) | glo, g1 = g0 | (1u << j);
       u64 lo = contig ? 0 : lo0 + c;
       u64 e = ((((u64)glo) << s) + lo) << sh;
-      u32* pa = t + 8 * (g0 * C + c);
-      u32* pb = t + 8 * (g1 * C + c);
-      Fr a = load_fe<Fr>(pa), bv = load_fe<Fr>(pb);
-      if (e != 0) bv = mul<FrParams>(bv, twiddle_at<TWO>(tw, e));
-      store_fe<Fr>(pa, add<FrParams>(a, bv));
-      store_fe<Fr>(pb, sub<FrParams>(a, bv));
+      bfly2(t + 8 * (g0 * C + c), t + 8 * (g1 * C + c), e, J.tw, two);
     }
     __syncthreads();
   }
-
+  const bool epi = last && J.epi.tab != nullptr;
   for (u32 idx = threadIdx.x; idx < GC; idx += blockDim.x) {
     u32 g, c;
     u64 p;
@@ -195,8 +282,53 @@ __global__ __launch_bounds__(256) void k_dit_pass(View x, u32 L, u32 s, u32 D, u
       p = (hi << (s + D)) | ((u64)g << s) | (lo0 + c);
     }
     Fr v = load_fe<Fr>(t + 8 * (g * C + c));
-    if (EPI) v = mul<FrParams>(v, factor_at(epi, p));
-    store_fe<Fr>(x.base + 8 * view_phys(x, p), v);
+    if (epi) v = mul<FrParams>(v, factor_at(J.epi, p));
+    store_fe<Fr>(J.x.base + 8 * view_phys(J.x, p), v);
+  }
+}
+
+// --------------------------------------------------------------- kernels
+__global__ __launch_bounds__(256) void k_ntt_br(NttJob J) {
+  extern __shared__ uint4 lds4[];
+  br_body(J, blockIdx.x, reinterpret_cast<u32*>(lds4));
+}
+
+__global__ __launch_bounds__(256) void k_ntt_dit(NttJob J, u32 stage) {
+  extern __shared__ uint4 lds4[];
+  dit_body(J, J.st[stage], stage + 1 == J.nst, blockIdx.x, reinterpret_cast<u32*>(lds4));
+}
+
+__device__ __forceinline__ u32 find_job(const u32* prefix, u32 njobs, u32 t) {
+  u32 lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    u32 mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Batched stages: work-groups stride over the concatenated block space of all
+// jobs (prefix[j] = first block of job j in this stage); jobs without work in
+// the stage contribute zero blocks.
+__global__ __launch_bounds__(256) void k_ntt_br_batch(const NttJob* jobs, u32 njobs,
+                                                      const u32* prefix) {
+  extern __shared__ uint4 lds4[];
+  const u32 total = prefix[njobs];
+  for (u32 t = blockIdx.x; t < total; t += gridDim.x) {
+    u32 j = find_job(prefix, njobs, t);
+    br_body(jobs[j], t - prefix[j], reinterpret_cast<u32*>(lds4));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ntt_dit_batch(const NttJob* jobs, u32 njobs,
+                                                       const u32* prefix, u32 stage) {
+  extern __shared__ uint4 lds4[];
+  const u32 total = prefix[njobs];
+  for (u32 t = blockIdx.x; t < total; t += gridDim.x) {
+    u32 j = find_job(prefix, njobs, t);
+    const NttJob& J = jobs[j];
+    dit_body(J, J.st[stage], stage + 1 == J.nst, t - prefix[j], reinterpret_cast<u32*>(lds4));
   }
 }
 
@@ -260,113 +392,96 @@ __global__ void k_ninv(u32* out, u32 L) {
 }
 
 // ----------------------------------------------------------------- planner
+// fuse: DIT levels done inside the bit-reversal kernel (0 or 5, needs L >= 10);
+// lds_log: log2 of the elements one work-group stages (10: 32 KiB, 11: 64 KiB);
+// d[]: level counts of the following LDS passes.
 struct PassPlan {
+  u32 fuse;
+  u32 lds_log;
   u32 n;
-  u32 d[32];
+  u32 d[8];
 };
 
 static void default_plan(u32 L, PassPlan* p) {
-  // passes of at most 8 levels, as equal as possible
-  u32 np = (L + 7) / 8;
+  p->fuse = L >= 10 ? 5 : 0;
+  p->lds_log = 10;
+  u32 rest = L - p->fuse;
+  u32 np = (rest + 7) / 8;
   if (np == 0) np = 1;
   p->n = np;
-  for (u32 i = 0; i < np; i++) p->d[i] = L / np + (i < L % np ? 1 : 0);
+  for (u32 i = 0; i < np; i++) p->d[i] = rest / np + (i < rest % np ? 1 : 0);
 }
 
-static u32 ilog2(u64 x) {
-  u32 r = 0;
-  while ((1ull << (r + 1)) <= x) r++;
-  return r;
-}
-
-struct NttArgs {
-  View x;
-  u32 L;
-  Twiddle tw;
-  bool two_level;
-  Factor pro;  // tab == nullptr: none
-  Factor epi;
-  const PassPlan* plan;
-};
-
-static hipError_t run_ntt(const NttArgs& a, hipStream_t stream) {
-  const u32 L = a.L;
-  // 1. bit reversal (+ prologue)
-  {
-  TimedScope ts(K_NTT_BITREV, stream);
-  if (L >= 10) {
-    u64 grid = 1ull << (L - 10);
-    size_t lds = 2 * 1024 * 32;
-    if (a.pro.tab)
-      hipLaunchKernelGGL((k_bitrev_tiles<true>), dim3((u32)grid), dim3(256), lds, stream, a.x, L,
-                         a.pro);
-    else
-      hipLaunchKernelGGL((k_bitrev_tiles<false>), dim3((u32)grid), dim3(256), lds, stream, a.x,
-                         L, a.pro);
-  } else {
-    size_t lds = (size_t)32 << L;
-    if (a.pro.tab)
-      hipLaunchKernelGGL((k_bitrev_small<true>), dim3(1), dim3(256), lds, stream, a.x, L, a.pro);
-    else
-      hipLaunchKernelGGL((k_bitrev_small<false>), dim3(1), dim3(256), lds, stream, a.x, L,
-                         a.pro);
-  }
-  }
-  IMP_CHECK(hipGetLastError());
-  if (L == 0) return hipSuccess;
-
-  // 2. DIT passes
-  u32 s = 0;
-  for (u32 pi = 0; pi < a.plan->n; pi++) {
-    u32 D = a.plan->d[pi];
-    if (D == 0) continue;
-    bool last = (pi + 1 == a.plan->n);
-    u32 logC;
-    u64 grid;
-    // at most 1024 elements (32 KiB) staged per work-group
-    u32 maxLogC = D >= 10 ? 0 : 10 - D;
-    if (s == 0) {
-      u32 groups_log = L - D;
-      logC = maxLogC < groups_log ? maxLogC : groups_log;
-      grid = 1ull << (L - D - logC);
-    } else {
-      logC = maxLogC < s ? maxLogC : s;
-      grid = 1ull << (L - D - logC);
-    }
-    size_t lds = ((size_t)32 << D) << logC;
-    bool epi = last && a.epi.tab;
-    TimedScope ts(K_NTT_DIT, stream);
-#define IMP_LAUNCH_DIT(TWO, EPI)                                                              \
-  hipLaunchKernelGGL((k_dit_pass<TWO, EPI>), dim3((u32)grid), dim3(256), lds, stream, a.x, L, \
-                     s, D, logC, a.tw, a.epi)
-    if (a.two_level) {
-      if (epi) IMP_LAUNCH_DIT(true, true);
-      else IMP_LAUNCH_DIT(true, false);
-    } else {
-      if (epi) IMP_LAUNCH_DIT(false, true);
-      else IMP_LAUNCH_DIT(false, false);
-    }
-#undef IMP_LAUNCH_DIT
-    IMP_CHECK(hipGetLastError());
-    s += D;
-  }
-  return hipSuccess;
-}
-
-static bool plan_from(const unsigned int* passes, unsigned int n, u32 L, PassPlan* p) {
-  if (!passes || n == 0) {
+// plan array: {fuse, lds_log, d0, d1, ...}; fuse + sum(d) == L
+static bool plan_from(const unsigned int* plan, unsigned int n, u32 L, PassPlan* p) {
+  if (!plan || n == 0) {
     default_plan(L, p);
     return true;
   }
-  if (n > 32) return false;
-  u32 sum = 0;
-  for (u32 i = 0; i < n; i++) {
-    if (passes[i] == 0 || passes[i] > 10) return false;
-    p->d[i] = passes[i];
-    sum += passes[i];
+  if (n < 3 || n > 10) return false;
+  p->fuse = plan[0];
+  p->lds_log = plan[1];
+  if (!(p->fuse == 0 || (p->fuse == 5 && L >= 10))) return false;
+  if (p->lds_log != 10 && p->lds_log != 11) return false;
+  u32 sum = p->fuse;
+  p->n = n - 2;
+  for (u32 i = 0; i < p->n; i++) {
+    u32 d = plan[2 + i];
+    if (d == 0 || d > p->lds_log) return false;
+    p->d[i] = d;
+    sum += d;
   }
-  p->n = n;
   return sum == L;
+}
+
+static size_t stage_lds(const NttJob& J, u32 stage) {
+  return ((size_t)32 << J.st[stage].D) << J.st[stage].logC;
+}
+static size_t br_lds(const NttJob& J) { return J.L >= 10 ? 2 * 1024 * 32 : ((size_t)32 << J.L); }
+
+static NttJob make_job(View x, View src, u64 src_len, u32 L, const Twiddle& tw, bool two,
+                       Factor pro, Factor epi, const PassPlan& plan) {
+  NttJob J;
+  J.x = x;
+  J.src = src;
+  J.src_len = src_len;
+  J.L = L;
+  J.fuse = (L >= 10 && plan.fuse == 5) ? 5 : 0;
+  J.two = two ? 1 : 0;
+  J.tw = tw;
+  J.pro = pro;
+  J.epi = epi;
+  J.br_blocks = L >= 10 ? (u32)(1ull << (L - 10)) : 1;
+  J.nst = 0;
+  u32 s = J.fuse;
+  for (u32 pi = 0; pi < plan.n && L > 0; pi++) {
+    u32 D = plan.d[pi];
+    if (D == 0) continue;
+    u32 maxLogC = D >= plan.lds_log ? 0 : plan.lds_log - D;
+    u32 logC = s == 0 ? (maxLogC < L - D ? maxLogC : L - D) : (maxLogC < s ? maxLogC : s);
+    NttStage& st = J.st[J.nst++];
+    st.s = s;
+    st.D = D;
+    st.logC = logC;
+    st.blocks = (u32)(1ull << (L - D - logC));
+    s += D;
+  }
+  return J;
+}
+
+static hipError_t run_job(const NttJob& J, hipStream_t stream) {
+  {
+    TimedScope ts(K_NTT_BITREV, stream);
+    hipLaunchKernelGGL(k_ntt_br, dim3(J.br_blocks), dim3(256), br_lds(J), stream, J);
+  }
+  IMP_CHECK(hipGetLastError());
+  for (u32 st = 0; st < J.nst; st++) {
+    TimedScope ts(K_NTT_DIT, stream);
+    hipLaunchKernelGGL(k_ntt_dit, dim3(J.st[st].blocks), dim3(256), stage_lds(J, st), stream, J,
+                       st);
+    IMP_CHECK(hipGetLastError());
+  }
+  return hipSuccess;
 }
 
 // factor tables in stream-ordered temporary memory
@@ -391,18 +506,16 @@ using namespace imp;
 // =============================================================== C ABI
 extern "C" hipError_t ssip_ntt(PolyPtr x, const unsigned int* twiddle, unsigned int log_len,
                                hipStream_t stream, const unsigned int, const unsigned int) {
-  IMP_REQUIRE(log_len <= 32, "log_len");
+  IMP_REQUIRE(log_len <= 31, "log_len");
   IMP_REQUIRE(x.len == (1ull << log_len), "x.len must be 2^log_len");
   IMP_REQUIRE(x.ptr != nullptr && (log_len == 0 || twiddle != nullptr), "null pointer");
-  NttArgs a = {};
-  IMP_REQUIRE(make_view(x, &a.x), "PolyPtr view");
-  a.L = log_len;
-  a.tw.tab = twiddle;
-  a.two_level = false;
+  View v;
+  IMP_REQUIRE(make_view(x, &v), "PolyPtr view");
+  Twiddle tw = {twiddle, nullptr, nullptr, 0};
   PassPlan plan;
   default_plan(log_len, &plan);
-  a.plan = &plan;
-  return run_ntt(a, stream);
+  Factor none = {nullptr, 0};
+  return run_job(make_job(v, v, x.len, log_len, tw, false, none, none, plan), stream);
 }
 
 extern "C" hipError_t recompute_ntt(PolyPtr x, const unsigned int* pq_d, unsigned int pq_deg,
@@ -413,16 +526,18 @@ extern "C" hipError_t recompute_ntt(PolyPtr x, const unsigned int* pq_d, unsigne
   IMP_REQUIRE(log_len <= 31, "log_len");
   IMP_REQUIRE(x.len == (1ull << log_len), "x.len must be 2^log_len");
   IMP_REQUIRE(x.ptr != nullptr && omegas_d != nullptr, "null pointer");
-  NttArgs a = {};
-  IMP_REQUIRE(make_view(x, &a.x), "PolyPtr view");
-  a.L = log_len;
+  View v;
+  IMP_REQUIRE(make_view(x, &v), "PolyPtr view");
   PassPlan plan;
   default_plan(log_len, &plan);
-  a.plan = &plan;
+  Factor none = {nullptr, 0};
+  Twiddle tw = {omegas_d, nullptr, nullptr, 0};  // n <= 2: only omega^0 = 1 is used
+  bool two = false;
   u32* tabs = nullptr;
   if (log_len >= 2) {
-    // two-level twiddles: lo[i] = w^i (i < 2^h), hi[j] = w^(j * 2^h)
-    u32 h = (log_len - 1 + 1) / 2;
+    // two-level twiddles from omegas[k] = omega^(2^k): lo[i] = w^i (i < 2^h),
+    // hi[j] = w^(j * 2^h); omega^e = lo[e mod 2^h] * hi[e >> h] exactly
+    u32 h = log_len / 2;
     u64 nlo = 1ull << h, nhi = 1ull << (log_len - 1 - h);
     IMP_CHECK(hipMallocAsync((void**)&tabs, (size_t)(nlo + nhi) * 32, stream));
     hipLaunchKernelGGL(k_pow_from_omegas, dim3((u32)((nlo + 255) / 256)), dim3(256), 0, stream,
@@ -430,14 +545,10 @@ extern "C" hipError_t recompute_ntt(PolyPtr x, const unsigned int* pq_d, unsigne
     hipLaunchKernelGGL(k_pow_from_omegas, dim3((u32)((nhi + 255) / 256)), dim3(256), 0, stream,
                        tabs + 8 * nlo, nhi, (u64)h, omegas_d);
     IMP_CHECK(hipGetLastError());
-    a.two_level = true;
-    a.tw.lo = tabs;
-    a.tw.hi = tabs + 8 * nlo;
-    a.tw.h = h;
-  } else {
-    a.tw.tab = omegas_d;  // n <= 2: only omega^0 = 1 is ever used
+    two = true;
+    tw = Twiddle{nullptr, tabs, tabs + 8 * nlo, h};
   }
-  hipError_t e = run_ntt(a, stream);
+  hipError_t e = run_job(make_job(v, v, x.len, log_len, tw, two, none, none, plan), stream);
   if (tabs) IMP_CHECK(hipFreeAsync(tabs, stream));
   return e;
 }
@@ -524,31 +635,142 @@ extern "C" hipError_t impeller_ntt(uint32_t* x, const uint32_t* twiddle, unsigne
                                    hipStream_t stream) {
   IMP_REQUIRE(log_len <= 31, "log_len");
   IMP_REQUIRE(x && (log_len == 0 || twiddle), "null pointer");
-  NttArgs a = {};
-  a.x = View{x, 0, 1ull << log_len};
-  a.L = log_len;
-  a.tw.tab = twiddle;
+  View v = {x, 0, 1ull << log_len};
+  Twiddle tw = {twiddle, nullptr, nullptr, 0};
   PassPlan plan;
   IMP_REQUIRE(plan_from(plan_passes, n_passes, log_len, &plan), "pass plan");
-  a.plan = &plan;
+  Factor pro = {nullptr, 0}, epi = {nullptr, 0};
   u32* pro_tab = nullptr;
   u32* epi_tab = nullptr;
   u32* ninv = nullptr;
   if (!inverse) {
-    if (coset_shift) IMP_CHECK(make_factor(coset_shift, nullptr, log_len, stream, &a.pro, &pro_tab));
+    if (coset_shift) IMP_CHECK(make_factor(coset_shift, nullptr, log_len, stream, &pro, &pro_tab));
   } else {
     IMP_CHECK(hipMallocAsync((void**)&ninv, 32, stream));
     hipLaunchKernelGGL(k_ninv, dim3(1), dim3(1), 0, stream, ninv, (u32)log_len);
     IMP_CHECK(hipGetLastError());
     // factor(i) = n^-1 * shift^i (shift given as its inverse by the caller)
     // or the constant n^-1 (base 1, one digit)
-    IMP_CHECK(make_factor(coset_shift, ninv, coset_shift ? log_len : 10, stream, &a.epi,
-                          &epi_tab));
+    IMP_CHECK(make_factor(coset_shift, ninv, coset_shift ? log_len : 10, stream, &epi, &epi_tab));
   }
-  hipError_t e = run_ntt(a, stream);
+  hipError_t e = run_job(make_job(v, v, v.wl, log_len, tw, false, pro, epi, plan), stream);
   if (pro_tab) IMP_CHECK(hipFreeAsync(pro_tab, stream));
   if (epi_tab) IMP_CHECK(hipFreeAsync(epi_tab, stream));
   if (ninv) IMP_CHECK(hipFreeAsync(ninv, stream));
   return e;
 }
 
+// ------------------------------------------------------------ batched NTT
+namespace {
+const size_t kFactorBytes = (size_t)3 * 1024 * 32;
+const size_t kJobArea = 2 * kFactorBytes + 256;  // pro table, epi table, n^-1
+size_t batch_jobs_off() { return 0; }
+size_t batch_prefix_off(u32 count) { return align_up((size_t)count * sizeof(NttJob), 256); }
+size_t batch_factor_off(u32 count) {
+  return batch_prefix_off(count) + align_up((size_t)9 * (count + 1) * 4, 256);
+}
+}  // namespace
+
+extern "C" size_t impeller_ntt_batch_bytes(unsigned int count) {
+  return batch_factor_off(count) + (size_t)count * kJobArea;
+}
+
+extern "C" hipError_t impeller_ntt_batch_prepare(const ImpNttItem* items, unsigned int count,
+                                                 void* dev, size_t bytes, ImpNttBatch* out,
+                                                 hipStream_t stream) {
+  IMP_REQUIRE(items && out && count > 0 && dev, "null pointer / empty batch");
+  IMP_REQUIRE(bytes >= impeller_ntt_batch_bytes(count), "batch buffer too small");
+  char* base = (char*)dev;
+  std::vector<NttJob> jobs(count);
+  std::vector<u32> prefix((size_t)9 * (count + 1), 0);
+  u32 nst_max = 0;
+  for (u32 i = 0; i < count; i++) {
+    const ImpNttItem& it = items[i];
+    IMP_REQUIRE(it.data && it.log_n <= 31 && (it.log_n == 0 || it.twiddle), "batch item");
+    IMP_REQUIRE(!it.src || it.log_src <= it.log_n, "batch item source size");
+    PassPlan plan;
+    IMP_REQUIRE(plan_from(it.plan_len ? it.plan : nullptr, it.plan_len, it.log_n, &plan),
+                "batch item plan");
+    View x = {it.data, 0, 1ull << it.log_n};
+    View src = it.src ? View{const_cast<u32*>(it.src), 0, 1ull << it.log_src} : x;
+    u64 src_len = it.src ? (1ull << it.log_src) : x.wl;
+    Twiddle tw = {it.twiddle, nullptr, nullptr, 0};
+    char* area = base + batch_factor_off(count) + (size_t)i * kJobArea;
+    u32* pro_tab = (u32*)area;
+    u32* epi_tab = (u32*)(area + kFactorBytes);
+    u32* ninv = (u32*)(area + 2 * kFactorBytes);
+    Factor pro = {nullptr, 0}, epi = {nullptr, 0};
+    u32 ndig = it.log_n <= 10 ? 1 : (it.log_n <= 20 ? 2 : 3);
+    if (!it.inverse) {
+      if (it.coset_shift) {
+        hipLaunchKernelGGL(k_factor_tables, dim3((1024 * ndig + 255) / 256), dim3(256), 0,
+                           stream, pro_tab, it.coset_shift, (const u32*)nullptr, ndig);
+        pro = Factor{pro_tab, ndig};
+      }
+    } else {
+      hipLaunchKernelGGL(k_ninv, dim3(1), dim3(1), 0, stream, ninv, it.log_n);
+      u32 nd = it.coset_shift ? ndig : 1;
+      hipLaunchKernelGGL(k_factor_tables, dim3((1024 * nd + 255) / 256), dim3(256), 0, stream,
+                         epi_tab, it.coset_shift, (const u32*)ninv, nd);
+      epi = Factor{epi_tab, nd};
+    }
+    IMP_CHECK(hipGetLastError());
+    jobs[i] = make_job(x, src, src_len, it.log_n, tw, false, pro, epi, plan);
+    if (jobs[i].nst > nst_max) nst_max = jobs[i].nst;
+  }
+  out->dev = dev;
+  out->count = count;
+  out->nstages = 1 + nst_max;
+  for (u32 st = 0; st < 9; st++) {
+    u32 run = 0;
+    size_t lds = 0;
+    for (u32 i = 0; i < count; i++) {
+      prefix[(size_t)st * (count + 1) + i] = run;
+      u32 blocks = 0;
+      if (st == 0) {
+        blocks = jobs[i].br_blocks;
+        size_t l = br_lds(jobs[i]);
+        if (l > lds) lds = l;
+      } else if (st - 1 < jobs[i].nst) {
+        blocks = jobs[i].st[st - 1].blocks;
+        size_t l = stage_lds(jobs[i], st - 1);
+        if (l > lds) lds = l;
+      }
+      run += blocks;
+    }
+    prefix[(size_t)st * (count + 1) + count] = run;
+    out->total[st] = run;
+    out->lds[st] = (unsigned int)lds;
+  }
+  IMP_CHECK(hipMemcpyAsync(base + batch_jobs_off(), jobs.data(), sizeof(NttJob) * count,
+                           hipMemcpyHostToDevice, stream));
+  IMP_CHECK(hipMemcpyAsync(base + batch_prefix_off(count), prefix.data(), prefix.size() * 4,
+                           hipMemcpyHostToDevice, stream));
+  // the host staging vectors die with this call
+  return hipStreamSynchronize(stream);
+}
+
+extern "C" hipError_t impeller_ntt_batch_run(const ImpNttBatch* b, unsigned int max_blocks,
+                                             hipStream_t stream) {
+  IMP_REQUIRE(b && b->dev && b->count > 0, "batch");
+  const char* base = (const char*)b->dev;
+  const NttJob* jobs = (const NttJob*)(base + batch_jobs_off());
+  const u32* prefix = (const u32*)(base + batch_prefix_off(b->count));
+  for (u32 st = 0; st < b->nstages; st++) {
+    u32 total = b->total[st];
+    if (total == 0) continue;
+    u32 grid = max_blocks && total > max_blocks ? max_blocks : total;
+    const u32* pf = prefix + (size_t)st * (b->count + 1);
+    if (st == 0) {
+      TimedScope ts(K_NTT_BITREV, stream);
+      hipLaunchKernelGGL(k_ntt_br_batch, dim3(grid), dim3(256), b->lds[st], stream, jobs,
+                         b->count, pf);
+    } else {
+      TimedScope ts(K_NTT_DIT, stream);
+      hipLaunchKernelGGL(k_ntt_dit_batch, dim3(grid), dim3(256), b->lds[st], stream, jobs,
+                         b->count, pf, st - 1);
+    }
+    IMP_CHECK(hipGetLastError());
+  }
+  return hipSuccess;
+}

@@ -11,13 +11,14 @@ enum KernelId {
   K_NTT_BITREV = 0,
   K_NTT_DIT = 1,
   K_MSM_DIGITS = 2,
-  K_MSM_SCAN = 3,
-  K_MSM_SCATTER = 4,
+  K_MSM_SCAN = 3,  // bucket bounds + task scan
   K_MSM_BUCKETS = 5,
   K_MSM_REDUCE = 6,
   K_MSM_SUM = 7,
   K_MSM_FINAL = 8,
   K_GATE = 9,
+  K_MSM_SORT = 10,
+  K_MSM_COMBINE = 11,
   K_COUNT = 16
 };
 

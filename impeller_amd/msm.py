@@ -107,6 +107,36 @@ def workspace_size(n: int, window: int) -> int:
     return int(_lib.call("impeller_msm_workspace_size", n, window))
 
 
+def cfg_workspace_size(n: int, window: int, target: int = 0, task: int = 64) -> int:
+    return int(_lib.call("impeller_msm_cfg_workspace_size", n, window, target, task))
+
+
+def n_tables(window: int, target: int) -> int:
+    return int(_lib.call("impeller_msm_tables", window, target))
+
+
+def precompute_device(tables_t, n, window, target, stream):
+    """Fill tables 1.. of a device tensor holding n_tables * n affine points."""
+    _lib.call("impeller_msm_precompute", tables_t.data_ptr(), n, window, target, stream.raw())
+
+
+def msm_device_cfg(tables_t, scalars_t, n, window, target, stream, task=64, workspace=None,
+                   out=None, sync=True):
+    """Device MSM over precomputed tables; returns the (16,) affine result."""
+    from .runtime import device_u32, to_host
+    ws_bytes = cfg_workspace_size(n, window, target, task)
+    if workspace is None:
+        workspace = device_u32((ws_bytes + 3) // 4)
+    if out is None:
+        out = device_u32(16)
+    _lib.call("impeller_msm_cfg", tables_t.data_ptr(), scalars_t.data_ptr(), n, window, target,
+              task, out.data_ptr(), workspace.data_ptr(), ws_bytes, stream.raw())
+    if not sync:
+        return out
+    stream.sync()
+    return to_host(out, 16)[0]
+
+
 def msm_device(bases_t, scalars_t, n, window, stream, workspace=None):
     """Device MSM; returns the affine result as a (16,) uint32 numpy array."""
     from .runtime import device_u32, to_host

@@ -110,7 +110,12 @@ def twiddles(omega: np.ndarray, n: int, stream: Stream, device_id: int = 0):
 
 def ntt(x_tensor, twiddle_tensor, log_len, stream: Stream, inverse=False, coset_shift=None,
         passes=None):
-    """In-place NTT of a contiguous device tensor (impeller_ntt)."""
+    """In-place NTT of a contiguous device tensor (impeller_ntt).
+
+    passes: optional plan [fuse, lds_log, d0, d1, ...] -- levels fused into the
+    bit-reversal kernel (0 or 5), log2 elements staged per work-group (10 or
+    11) and the level count of each following pass (fuse + sum(d) == log_len).
+    """
     plan = None
     n_passes = 0
     if passes:
@@ -120,3 +125,65 @@ def ntt(x_tensor, twiddle_tensor, log_len, stream: Stream, inverse=False, coset_
     shift = coset_shift.data_ptr() if coset_shift is not None else None
     _lib.call("impeller_ntt", x_tensor.data_ptr(), twiddle_tensor.data_ptr(), log_len,
               1 if inverse else 0, shift, plan, n_passes, stream.raw())
+
+
+# ------------------------------------------------------------- batched NTT
+class NttBatch:
+    """A prepared batch of independent NTTs (impeller_ntt_batch_*).
+
+    items: dicts with keys data (device tensor, 2^log_n elements, result in
+    place), log_n, twiddle (device table of the omega used), and optionally
+    src (device tensor, 2^log_src coefficients, zero-extended), log_src,
+    inverse, coset_shift (device element), plan (list).
+    """
+
+    def __init__(self, items, stream: Stream):
+        from .runtime import device_u32
+        n = len(items)
+        arr = (_lib.ImpNttItem * n)()
+        self._keep = []
+        for i, it in enumerate(items):
+            a = arr[i]
+            a.data = it["data"].data_ptr()
+            a.src = it["src"].data_ptr() if it.get("src") is not None else None
+            a.log_n = it["log_n"]
+            a.log_src = it.get("log_src", it["log_n"])
+            a.twiddle = it["twiddle"].data_ptr()
+            a.inverse = 1 if it.get("inverse") else 0
+            a.coset_shift = it["coset_shift"].data_ptr() if it.get("coset_shift") is not None else None
+            plan = it.get("plan") or []
+            a.plan_len = len(plan)
+            for k, v in enumerate(plan):
+                a.plan[k] = v
+            self._keep.append(it)
+        nbytes = int(_lib.call("impeller_ntt_batch_bytes", n))
+        self.dev = device_u32((nbytes + 3) // 4)
+        self.info = _lib.ImpNttBatch()
+        _lib.call("impeller_ntt_batch_prepare", arr, n, self.dev.data_ptr(), nbytes,
+                  ctypes.byref(self.info), stream.raw())
+
+    def run(self, stream: Stream, max_blocks=0):
+        _lib.call("impeller_ntt_batch_run", ctypes.byref(self.info), max_blocks, stream.raw())
+
+
+class Graph:
+    """A HIP graph captured from the work a callable launches on stream."""
+
+    def __init__(self, stream: Stream, fn):
+        _lib.call("impeller_graph_begin", stream.raw())
+        try:
+            fn()
+        finally:
+            h = ctypes.c_void_p()
+            _lib.call("impeller_graph_end", stream.raw(), ctypes.byref(h))
+        self.handle = h
+
+    def launch(self, stream: Stream):
+        _lib.call("impeller_graph_launch", self.handle, stream.raw())
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.lib().impeller_graph_destroy(self.handle)
+        except Exception:
+            pass

@@ -139,12 +139,57 @@ hipError_t fused_gate_abc_d_qe(const ConstPolyPtr *vars, const PolyPtr *mut_vars
  * omega actually used (omega^-1 for the inverse).  coset_shift (nullable,
  * device, 1 element) multiplies x[i] by shift^i before a forward transform
  * or by shift^-i after an inverse one (shift passed as its inverse then).
- * plan_passes: pass split (level counts, summing to log_len) or NULL for
- * the library default. */
+ * plan_passes (n_passes entries, or NULL for the library default):
+ * {fuse, lds_log, d0, d1, ...}: fuse = DIT levels done inside the
+ * bit-reversal kernel (0, or 5 when log_len >= 10), lds_log = log2 of the
+ * elements one work-group stages in LDS (10 or 11), d_i = levels of each
+ * following pass; fuse + sum(d_i) == log_len. */
 hipError_t impeller_ntt(uint32_t *x, const uint32_t *twiddle, unsigned int log_len,
                         int inverse, const uint32_t *coset_shift,
                         const unsigned int *plan_passes, unsigned int n_passes,
                         hipStream_t stream);
+
+/* Batched NTT (no reference analogue): count independent transforms run as
+ * one launch per stage (bit reversal, then each LDS pass), work-groups
+ * striding over the concatenated tiles of all transforms.  Per item: data
+ * (2^log_n elements, in place) receives the result; src, when non-NULL, is
+ * the 2^log_src-element input, zero-extended to 2^log_n (coset extension);
+ * twiddle/inverse/coset_shift/plan as for impeller_ntt.  prepare fills the
+ * device buffer dev (impeller_ntt_batch_bytes(count) bytes) and the host
+ * descriptor *batch, synchronously; run only launches kernels (it can be
+ * captured into a HIP graph); max_blocks (0 = no cap) bounds the grid of
+ * each stage launch. */
+typedef struct ImpNttItem {
+    uint32_t *data;
+    const uint32_t *src;
+    unsigned int log_n, log_src;
+    const uint32_t *twiddle;
+    int inverse;
+    const uint32_t *coset_shift;
+    unsigned int plan[10];
+    unsigned int plan_len;
+} ImpNttItem;
+
+typedef struct ImpNttBatch {
+    void *dev;
+    unsigned int count, nstages;
+    unsigned int total[9];
+    unsigned int lds[9];
+} ImpNttBatch;
+
+size_t impeller_ntt_batch_bytes(unsigned int count);
+hipError_t impeller_ntt_batch_prepare(const ImpNttItem *items, unsigned int count, void *dev,
+                                      size_t bytes, ImpNttBatch *batch, hipStream_t stream);
+hipError_t impeller_ntt_batch_run(const ImpNttBatch *batch, unsigned int max_blocks,
+                                  hipStream_t stream);
+
+/* Stream capture into a HIP graph (no reference analogue): begin/end wrap
+ * hipStreamBeginCapture / hipStreamEndCapture + hipGraphInstantiate; the
+ * executable graph is returned as an opaque handle. */
+hipError_t impeller_graph_begin(hipStream_t stream);
+hipError_t impeller_graph_end(hipStream_t stream, void **graph_exec);
+hipError_t impeller_graph_launch(void *graph_exec, hipStream_t stream);
+hipError_t impeller_graph_destroy(void *graph_exec);
 
 /* Fills twiddle[i] = omega^i (i < len) on the device.  omega is host memory. */
 hipError_t impeller_twiddles(uint32_t *twiddle_d, unsigned long long len,
@@ -157,6 +202,25 @@ size_t impeller_msm_workspace_size(unsigned long long len, unsigned int window);
 hipError_t impeller_msm(const uint32_t *bases_d, const uint32_t *scalars_d,
                         unsigned long long len, unsigned int window, uint32_t *result_d,
                         void *workspace, size_t workspace_bytes, hipStream_t stream);
+
+/* Configurable device MSM.  tables_d holds impeller_msm_tables(window,
+ * target_windows) tables of len affine points: table t = 2^(t * n_windows *
+ * window) * P with n_windows = min(target_windows, floor(254/window) + 1)
+ * (target_windows = 0: one bucket set per window, a single table); the
+ * reference's precompute layout (core/src/msm/src/msm_impl.cuh:56-67).
+ * impeller_msm_precompute fills tables 1.. from table 0 in place.  task is
+ * the number of bucket entries one thread accumulates before the per-bucket
+ * partial-sum tree takes over. */
+size_t impeller_msm_cfg_workspace_size(unsigned long long len, unsigned int window,
+                                       unsigned int target_windows, unsigned int task);
+unsigned int impeller_msm_tables(unsigned int window, unsigned int target_windows);
+hipError_t impeller_msm_precompute(uint32_t *tables_d, unsigned long long len,
+                                   unsigned int window, unsigned int target_windows,
+                                   hipStream_t stream);
+hipError_t impeller_msm_cfg(const uint32_t *tables_d, const uint32_t *scalars_d,
+                            unsigned long long len, unsigned int window,
+                            unsigned int target_windows, unsigned int task, uint32_t *result_d,
+                            void *workspace, size_t workspace_bytes, hipStream_t stream);
 
 /* Seeded synthetic inputs on the device (the CPU checker has the same
  * definition): kind 0 uniform Fr, 1 selector-like (90% zero, else one),
@@ -177,10 +241,30 @@ hipError_t impeller_field_op(int field, int op, uint32_t *r_d, const uint32_t *a
 hipError_t impeller_g1_op(int op, uint32_t *r_d, const uint32_t *a_d, const uint32_t *b_d,
                           unsigned long long n, hipStream_t stream);
 
+/* FP32 cost model of the autotuner (no reference analogue): a 3-layer MLP
+ * F -> H -> H -> 1 with ReLU, parameters W1[H][F] b1[H] W2[H][H] b2[H]
+ * W3[H] b3 in one float buffer of impeller_mlp_param_count(F, H) floats,
+ * trained with MSE and Adam.  All buffers are device memory; workspace holds
+ * impeller_mlp_workspace_floats(F, H, B) floats.  loss_sum (nullable)
+ * receives the sum of squared errors of the batch. */
+unsigned int impeller_mlp_param_count(unsigned int F, unsigned int H);
+size_t impeller_mlp_workspace_floats(unsigned int F, unsigned int H, unsigned int B);
+hipError_t impeller_mlp_forward(const float *params, const float *X, unsigned int B,
+                                unsigned int F, unsigned int H, float *Y, float *workspace,
+                                hipStream_t stream);
+hipError_t impeller_mlp_grad(const float *params, const float *X, const float *T,
+                             unsigned int B, unsigned int F, unsigned int H, float *grads,
+                             float *loss_sum, float *workspace, hipStream_t stream);
+hipError_t impeller_mlp_train_step(float *params, float *m, float *v, const float *X,
+                                   const float *T, unsigned int B, unsigned int F,
+                                   unsigned int H, float lr, unsigned int step, float *loss_sum,
+                                   float *workspace, hipStream_t stream);
+
 /* Per-kernel HIP-event timing (off by default).  Kernel ids: 0 NTT
- * bit-reversal, 1 NTT DIT pass, 2 MSM digits, 3 MSM scan, 4 MSM scatter,
- * 5 MSM bucket accumulation, 6 MSM bucket reduction, 7 MSM window sums,
- * 8 MSM final combine, 9 fused gate.  read() synchronises pending events. */
+ * bit-reversal, 1 NTT DIT pass, 2 MSM digits, 3 MSM bucket bounds + task
+ * scan, 5 MSM bucket accumulation, 6 MSM bucket reduction, 7 MSM window
+ * sums, 8 MSM final combine, 9 fused gate, 10 MSM radix sort, 11 MSM
+ * partial-sum tree.  read() synchronises pending events. */
 void impeller_timing_enable(int on);
 void impeller_timing_reset(void);
 int impeller_timing_read(int kernel_id, double *total_ms, unsigned long long *count);

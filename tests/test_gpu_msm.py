@@ -117,3 +117,48 @@ def test_msm_2_20_closed_form(stream):
     expect = O.msm_closed_form(s, k)
     for c in (14, 16):
         assert (M.msm_device(bases, scal, n, c, stream) == expect).all()
+
+
+@pytest.mark.parametrize("c", [12, 14, 18])
+def test_msm_top_window_hot_buckets(stream, bases_small, c):
+    # with these windows the top window holds 2 bits of the scalar, so a few
+    # buckets receive every point: the task split + partial-sum tree must
+    # keep this exact (and fast)
+    bases, k = bases_small
+    n = bases.shape[0]
+    s = O.gen_scalars(n, 1200 + c)
+    got = M.msm_device(to_device(bases), to_device(s), n, c, stream)
+    assert (got == O.msm_closed_form(s, k)).all()
+
+
+@pytest.mark.parametrize("window,target,task", [(16, 1, 64), (13, 1, 7), (12, 3, 1),
+                                                (15, 2, 4096), (16, 0, 33)])
+def test_msm_device_precomputed_tables(stream, bases_small, window, target, task):
+    bases, k = bases_small
+    n = bases.shape[0]
+    nt = M.n_tables(window, target)
+    tables = device_u32(nt * n * 16)
+    tables[: n * 16].copy_(to_device(bases))
+    M.precompute_device(tables, n, window, target, stream)
+    stream.sync()
+    if nt > 1:
+        step = M.MsmConfig(window_size=window, target_window=target or 10 ** 6).n_windows() * window
+        t1 = to_host(tables[n * 16: 2 * n * 16], 16)
+        assert (t1[5] == O.g1_mul(bases[5], 1 << step)).all()
+    for kind in (0, 1, 2):
+        s = O.gen_scalars(n, 1300 + kind + window, kind=kind)
+        got = M.msm_device_cfg(tables, to_device(s), n, window, target, stream, task=task)
+        assert (got == O.msm_closed_form(s, k)).all(), (kind, window, target, task)
+
+
+def test_msm_selectors_2_18(stream):
+    n = 1 << 18
+    bases = device_u32(n * 16)
+    _lib.call("impeller_gen_bases", bases.data_ptr(), n, 31, 0, stream.raw())
+    k = O.gen_scalars(n, 31)
+    for kind in (1, 2):
+        scal = device_u32(n * 8)
+        _lib.call("impeller_gen_scalars", scal.data_ptr(), n, kind, 32 + kind, 0, stream.raw())
+        stream.sync()
+        expect = O.msm_closed_form(to_host(scal), k)
+        assert (M.msm_device(bases, scal, n, 15, stream) == expect).all()

@@ -168,7 +168,9 @@ def test_impeller_ntt_inverse_and_coset(stream, k):
     assert (to_host(d) == O.vec_scale(O.best_fft(x, winv, k), ninv)).all()
 
 
-@pytest.mark.parametrize("passes", [[4, 4, 4, 4], [8, 8], [3, 5, 8], [10, 6], [1] * 16])
+@pytest.mark.parametrize("passes", [[0, 10, 4, 4, 4, 4], [0, 10, 8, 8], [5, 10, 3, 8],
+                                    [0, 11, 10, 6], [0, 10] + [1] * 16, [5, 11, 11],
+                                    [5, 10, 1, 1, 9], [5, 11, 5, 6]])
 def test_pass_splits_agree(stream, passes):
     k = 16
     x = O.gen_scalars(1 << k, 7000)
