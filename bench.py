@@ -1,11 +1,16 @@
-"""Benchmark of the BN254 hot path on one MI355X.
+"""Benchmark of the BN254 hot path on one MI355X.  Prints ONE JSON line.
 
-Workload (the plain single-call production sizes): forward NTT of 2^24 Fr,
-coset NTT of 2^24 Fr (shift = multiplicative generator 7) and one G1 MSM of
-2^22 affine bases k_i*G with uniform scalars.  One step = those three calls
-on device-resident synthetic data.  Prints ONE JSON line.
+Plain single-call path (the metric's first two numbers): forward NTT 2^24,
+coset NTT 2^24 (shift = 7) and one G1 MSM of 2^22 affine bases k_i*G with
+uniform scalars, all device resident; the MSM window c is swept over 12..18.
+Batched vs plain (the third number): 64 uneven NTTs (2^18..2^23, forward /
+inverse / coset-extended x8) and 40 commitment MSMs (2^16..2^22, shared 2^22
+base table, 50% uniform / 30% selector / 20% small scalars), each run as
+sequential single calls and as batched launches replayed from a HIP graph;
+outputs compared bit for bit.  With --chunk, also the fused proving chunk at
+k = 22 (plain vs batched+graph).
 
-  python bench.py [--steps K] [--warmup W] [--window C] [--sweep]
+  python bench.py [--steps K] [--warmup W] [--window C] [--no-cpu] [--no-batch] [--chunk]
 """
 import argparse
 import json
@@ -18,6 +23,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MSM 2^22 ms; NTT 2^24 HBM GB/s; batched-vs-plain speedup (bit-exact)"
 HBM_PEAK_GBS = 8000.0
+TUNED = os.path.join(ROOT, "impeller_amd", "tuned", "picks.json")
 
 
 def parse():
@@ -29,7 +35,16 @@ def parse():
     ap.add_argument("--log-msm", type=int, default=22)
     ap.add_argument("--window", type=int, default=0, help="MSM window c (0: sweep 12..18)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched comparisons")
+    ap.add_argument("--chunk", action="store_true", help="also run the k=22 proving chunk")
     return ap.parse_args()
+
+
+def load_picks():
+    if os.path.exists(TUNED):
+        with open(TUNED) as f:
+            return json.load(f)
+    return {}
 
 
 def main():
@@ -39,73 +54,79 @@ def main():
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
     import torch
     from impeller_amd import _lib
     from impeller_amd import msm as M
     from impeller_amd import ntt as N
+    from impeller_amd import workloads as WL
     from impeller_amd.runtime import Stream, device_u32, to_device, to_host
 
-    r = 0x30644E72E131A029B85045B68181585D2833E84879B9709143E1F593F0000001
-    R = 1 << 256
-
-    def mont(x):
-        v = x * R % r
-        return [(v >> (32 * i)) & 0xFFFFFFFF for i in range(8)]
-
-    import numpy as np
-    dev = torch.device("cuda:0")
+    # graphs cannot be captured on the legacy default stream
+    side = torch.cuda.Stream()
+    torch.cuda.set_stream(side)
     stream = Stream.current(0)
+    r = WL.R_MOD
+    mont = WL.mont_limbs
     k = args.log_ntt
     n = 1 << k
     nm = 1 << args.log_msm
-    root = pow(7, (r - 1) >> 28, r)
-    w = pow(root, 1 << (28 - k), r)
+    w = WL.omega(k)
+    picks = load_picks()
+
+    def ev():
+        return torch.cuda.Event(enable_timing=True)
+
+    def timed(fn, reps):
+        fn()
+        best = None
+        for _ in range(reps):
+            a, b = ev(), ev()
+            a.record()
+            fn()
+            b.record()
+            b.synchronize()
+            t = a.elapsed_time(b)
+            best = t if best is None else min(best, t)
+        return best
 
     # ---------------------------------------------------------------- inputs
     x = device_u32(n * 8)
     _lib.call("impeller_gen_scalars", x.data_ptr(), n, 0, 2024, 0, stream.raw())
-    tw = N.twiddles(np.array(mont(w), np.uint32), n // 2, stream)
-    shift = to_device(np.array(mont(7), np.uint32))
+    tw = N.twiddles(mont(w), n // 2, stream)
+    shift = to_device(mont(WL.GENERATOR))
     bases = device_u32(nm * 16)
     _lib.call("impeller_gen_bases", bases.data_ptr(), nm, 77, 0, stream.raw())
     scal = device_u32(nm * 8)
     _lib.call("impeller_gen_scalars", scal.data_ptr(), nm, 0, 78, 0, stream.raw())
     stream.sync()
     x_orig = x.clone()
+    ntt_plan = None
+    p = picks.get(f"ntt_fwd:{k}")
+    if p:
+        ntt_plan = p["exhaustive"]
 
-    # MSM configurations: precomputed window tables (one bucket set, the
-    # reference's TARGET_WINDOWS = 1 layout, tables built once as setup) and
-    # the plain per-window layout without tables.
+    # ---------------------------------------- MSM 2^22: window sweep + tables
     windows = [args.window] if args.window else list(range(12, 19))
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
     def time_msm(tables, c, target, task, reps=2):
         ws = device_u32((M.cfg_workspace_size(nm, c, target, task) + 3) // 4)
         out = device_u32(16)
-        M.msm_device_cfg(tables, scal, nm, c, target, stream, task=task, workspace=ws, out=out,
-                         sync=False)
-        best = None
-        for _ in range(reps):
-            ev0.record()
-            M.msm_device_cfg(tables, scal, nm, c, target, stream, task=task, workspace=ws,
-                             out=out, sync=False)
-            ev1.record()
-            ev1.synchronize()
-            t = ev0.elapsed_time(ev1)
-            best = t if best is None else min(best, t)
+        t = timed(lambda: M.msm_device_cfg(tables, scal, nm, c, target, stream, task=task,
+                                           workspace=ws, out=out, sync=False), reps)
         del ws
-        return best
+        return t
 
     sweep, sweep_plain, setup_ms = {}, {}, {}
     for c in windows:
-        nt = M.n_tables(c, 1)
-        tables = device_u32(nt * nm * 16)
+        tables = device_u32(M.n_tables(c, 1) * nm * 16)
         tables[: nm * 16].copy_(bases)
-        ev0.record()
+        a, b = ev(), ev()
+        a.record()
         M.precompute_device(tables, nm, c, 1, stream)
-        ev1.record()
-        ev1.synchronize()
-        setup_ms[c] = ev0.elapsed_time(ev1)
+        b.record()
+        b.synchronize()
+        setup_ms[c] = a.elapsed_time(b)
         sweep[c] = time_msm(tables, c, 1, 64)
         del tables
         torch.cuda.empty_cache()
@@ -118,73 +139,153 @@ def main():
     ws = device_u32((M.cfg_workspace_size(nm, best_c, 1, 64) + 3) // 4)
     res_buf = device_u32(16)
 
-    def run_msm(c, _ws=None):
-        return M.msm_device_cfg(tables, scal, nm, c, 1, stream, task=64, workspace=ws,
+    def run_msm():
+        return M.msm_device_cfg(tables, scal, nm, best_c, 1, stream, task=64, workspace=ws,
                                 out=res_buf, sync=False)
 
-    def step():
-        N.ntt(x, tw, k, stream)
-        N.ntt(x, tw, k, stream, coset_shift=shift)
-        return run_msm(best_c, ws)
-
+    # ------------------------------------------------------ plain timed steps
     for _ in range(args.warmup):
-        step()
+        N.ntt(x, tw, k, stream, passes=ntt_plan)
+        N.ntt(x, tw, k, stream, coset_shift=shift, passes=ntt_plan)
+        run_msm()
     torch.cuda.synchronize()
-
-    # ----------------------------------------------------------- timed region
     _lib.timing_reset()
     _lib.timing_enable(True)
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    evs = [[ev() for _ in range(4)] for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
         e = evs[i]
         e[0].record()
-        N.ntt(x, tw, k, stream)
+        N.ntt(x, tw, k, stream, passes=ntt_plan)
         e[1].record()
-        N.ntt(x, tw, k, stream, coset_shift=shift)
+        N.ntt(x, tw, k, stream, coset_shift=shift, passes=ntt_plan)
         e[2].record()
-        res = run_msm(best_c, ws)
+        res = run_msm()
         e[3].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     _lib.timing_enable(False)
-    ntt_ms = sorted(e[0].elapsed_time(e[1]) for e in evs)[args.steps // 2]
-    coset_ms = sorted(e[1].elapsed_time(e[2]) for e in evs)[args.steps // 2]
-    msm_ms = sorted(e[2].elapsed_time(e[3]) for e in evs)[args.steps // 2]
+    med = args.steps // 2
+    ntt_ms = sorted(e[0].elapsed_time(e[1]) for e in evs)[med]
+    coset_ms = sorted(e[1].elapsed_time(e[2]) for e in evs)[med]
+    msm_ms = sorted(e[2].elapsed_time(e[3]) for e in evs)[med]
     kt = {name: _lib.timing_read(name) for name in _lib.KERNEL_IDS}
-
     bytes_ntt = 2 * n * 32
     ntt_gbs = bytes_ntt / (ntt_ms * 1e-3) / 1e9
     dit_ms, dit_cnt = kt["ntt_dit"]
     dit_avg_ms = dit_ms / max(dit_cnt, 1)
     dit_gbs = bytes_ntt / (dit_avg_ms * 1e-3) / 1e9
+    msm_res_host = to_host(res, 16)[0].copy()
+
+    # ---------------------------------------------------- batched vs plain
+    batch = None
+    batch_parity = {}
+    if not args.no_batch:
+        del ws
+        torch.cuda.empty_cache()
+        # batched NTT: 64 uneven transforms
+        spec = WL.ntt_batch_spec()
+        plans = {}
+        for kind, lg in spec:
+            L = lg + 3 if kind == "ext" else lg
+            op = {"fwd": "ntt_fwd", "inv": "ntt_inv", "ext": "ntt_coset"}[kind]
+            pk = picks.get(f"{op}:{L}")
+            if pk:
+                plans[(kind, L)] = pk.get("model") or pk["exhaustive"]
+        ntt_wl = WL.NttWorkload(spec, stream, plans=plans)
+        nb_ = ntt_wl.batch()
+
+        def plain_ntt():
+            ntt_wl.restore()
+            ntt_wl.run_plain()
+
+        def batched_ntt():
+            ntt_wl.restore()
+            nb_.run(stream)
+
+        t_plain_ntt = timed(plain_ntt, 2)
+        plain_out = [it["data"].clone() for it in ntt_wl.items]
+        g_ntt = N.Graph(stream, lambda: nb_.run(stream))
+        t_batch_ntt = timed(lambda: (ntt_wl.restore(), g_ntt.launch(stream)), 2)
+        t_restore = timed(ntt_wl.restore, 2)
+        batch_parity["ntt_batched_equals_plain"] = all(
+            bool(torch.equal(a, it["data"])) for a, it in zip(plain_out, ntt_wl.items))
+        del plain_out, g_ntt, nb_, ntt_wl
+        torch.cuda.empty_cache()
+
+        # batched MSM: 40 commitments on a shared 2^22 base table
+        mspec = WL.msm_batch_spec()
+        configs = []
+        for lg, kind in mspec:
+            pk = picks.get(f"msm:{lg}:{kind}")
+            if pk:
+                configs.append(tuple(pk.get("model") or pk["exhaustive"]))
+            else:
+                configs.append((WL.heuristic_window(lg), 1, 64))
+        msm_wl = WL.MsmWorkload(mspec, stream)
+        wss = msm_wl.workspaces(configs)
+        plain_res = device_u32(16 * len(mspec))
+        t_plain_msm = timed(lambda: msm_wl.run_plain(configs, plain_res, wss), 2)
+        del wss
+        torch.cuda.empty_cache()
+        mb = M.MsmBatch(msm_wl.items(configs), stream)
+        g_msm = N.Graph(stream, lambda: mb.run(stream))
+        t_batch_msm = timed(lambda: g_msm.launch(stream), 2)
+        batch_parity["msm_batched_equals_plain"] = bool(torch.equal(plain_res, mb.results))
+        batch = {
+            "ntt64_plain_ms": round(t_plain_ntt - t_restore, 3),
+            "ntt64_batched_graph_ms": round(t_batch_ntt - t_restore, 3),
+            "msm40_plain_ms": round(t_plain_msm, 3),
+            "msm40_batched_graph_ms": round(t_batch_msm, 3),
+            "ntt_plans": "tuned" if plans else "default",
+            "msm_configs": "tuned" if picks else "heuristic",
+        }
+        batch["ntt64_speedup"] = round(batch["ntt64_plain_ms"] / batch["ntt64_batched_graph_ms"], 3)
+        batch["msm40_speedup"] = round(t_plain_msm / t_batch_msm, 3)
+        batch["speedup"] = round(
+            (batch["ntt64_plain_ms"] + t_plain_msm)
+            / (batch["ntt64_batched_graph_ms"] + t_batch_msm), 3)
+        msm_batch_res = mb.results_host()
+        del g_msm, mb, msm_wl
+        torch.cuda.empty_cache()
+
+    chunk = None
+    if args.chunk:
+        pc = WL.ProvingChunk(stream)
+        cfg = (16, 0, 64)
+        wsc = device_u32((M.cfg_workspace_size(1 << 22, cfg[0], cfg[2] and 0, 64) + 3) // 4)
+        t_plain = timed(lambda: pc.run_plain(cfg, ws=wsc), 1)
+        plain_commit = pc.results.clone()
+        nbs = pc.batches()
+        mbc = pc.msm_batch(cfg)
+        g = N.Graph(stream, lambda: pc.run_batched(nbs, mbc))
+        t_batched = timed(lambda: g.launch(stream), 1)
+        chunk = {"plain_ms": round(t_plain, 2), "batched_graph_ms": round(t_batched, 2),
+                 "speedup": round(t_plain / t_batched, 3),
+                 "commitments_equal": bool(torch.equal(plain_commit, mbc.results))}
+        del pc, g
 
     # ------------------------------------------------ parity + CPU baseline
-    parity = {}
+    parity = dict(batch_parity)
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
-        # MSM closed form: (sum s_i k_i) * G over the same seeded inputs
         kk = O.gen_scalars(nm, 77)
-        s_host = to_host(scal)
-        parity["msm_closed_form"] = bool((to_host(res, 16)[0] == O.msm_closed_form(s_host, kk)).all())
-        # NTT: one fresh forward transform checked at sampled outputs y_j = p(w^j)
+        parity["msm_closed_form"] = bool((msm_res_host == O.msm_closed_form(to_host(scal), kk)).all())
         x.copy_(x_orig)
-        N.ntt(x, tw, k, stream)
+        N.ntt(x, tw, k, stream, passes=ntt_plan)
         stream.sync()
         xh, yh = to_host(x_orig), to_host(x)
         ok = True
         for j in (0, 1, n // 3, n - 1):
-            ok &= bool((O.poly_eval(xh, np.array(mont(pow(w, j, r)), np.uint32)) == yh[j]).all())
+            ok &= bool((O.poly_eval(xh, mont(pow(w, j, r))) == yh[j]).all())
         parity["ntt_sampled_outputs"] = ok
-        # CPU baseline: the oracle's best_fft / best_multiexp restatement on all
-        # host cores, bounded sample sizes
         threads = os.cpu_count() or 1
         kc = min(k, 22)
         xc = O.gen_scalars(1 << kc, 5)
         t = time.perf_counter()
-        O.best_fft(xc, np.array(mont(pow(root, 1 << (28 - kc), r)), np.uint32), kc, threads)
+        O.best_fft(xc, mont(WL.omega(kc)), kc, threads)
         t_fft = time.perf_counter() - t
         mc = min(args.log_msm, 18)
         bc = O.gen_bases(1 << mc, 9, threads=threads)
@@ -198,11 +299,10 @@ def main():
             "cores": threads,
             "kind": "port",
             "sample": f"best_fft 2^{kc} ({t_fft * 1e3:.0f} ms) and best_multiexp 2^{mc} "
-                      f"({t_msm * 1e3:.0f} ms) restated in C, {threads} threads",
-            "ntt_ms": round(t_fft * 1e3, 2),
-            "ntt_log": kc,
-            "msm_ms": round(t_msm * 1e3, 2),
-            "msm_log": mc,
+                      f"({t_msm * 1e3:.0f} ms): C restatement of halo2's CPU path, "
+                      f"{threads} threads",
+            "ntt_ms": round(t_fft * 1e3, 2), "ntt_log": kc,
+            "msm_ms": round(t_msm * 1e3, 2), "msm_log": mc,
         }
 
     line = {
@@ -220,8 +320,9 @@ def main():
         "data": "synthetic (seeded SplitMix64 uniform Fr; bases k_i*G)",
         "config": {
             "workload": f"NTT 2^{k} + coset NTT 2^{k} + G1 MSM 2^{args.log_msm} "
-                        "(plain single-call path)",
+                        "(plain single-call path, device resident)",
             "ntt_log": k, "msm_log": args.log_msm, "msm_window": best_c,
+            "ntt_plan": ntt_plan or "default",
             "msm_layout": "precomputed window tables (one bucket set), task 64",
             "parallelism": "single GPU",
         },
@@ -231,11 +332,13 @@ def main():
         "msm_window_sweep_ms": {str(c): round(v, 3) for c, v in sweep.items()},
         "msm_table_setup_ms": {str(c): round(v, 2) for c, v in setup_ms.items()},
         "msm_no_tables_ms": {str(c): round(v, 3) for c, v in sweep_plain.items()},
-        "batched_vs_plain_speedup": None,
+        "batched_vs_plain_speedup": batch["speedup"] if batch else None,
+        "batched": batch,
+        "proving_chunk": chunk,
         "kernels_ms": {name: {"total": round(v[0], 3), "launches": v[1]}
                        for name, v in kt.items() if v[1]},
         "roofline": {
-            "kernel": "k_dit_pass (NTT DIT pass)",
+            "kernel": "k_ntt_dit (NTT LDS pass)",
             "bound": "hbm",
             "achieved": round(dit_gbs, 2),
             "peak": HBM_PEAK_GBS,

@@ -46,6 +46,20 @@ class ImpNttBatch(ctypes.Structure):
                 ("total", ctypes.c_uint * 9), ("lds", ctypes.c_uint * 9)]
 
 
+class ImpMsmItem(ctypes.Structure):
+    """include/impeller_hip.h ImpMsmItem"""
+    _fields_ = [("tables", ctypes.c_void_p), ("scalars", ctypes.c_void_p),
+                ("len", ctypes.c_ulonglong), ("table_stride", ctypes.c_ulonglong),
+                ("window", ctypes.c_uint), ("target_windows", ctypes.c_uint),
+                ("task", ctypes.c_uint), ("result", ctypes.c_void_p)]
+
+
+class ImpMsmBatch(ctypes.Structure):
+    _fields_ = [("dev", ctypes.c_void_p), ("count", ctypes.c_uint), ("nstages", ctypes.c_uint),
+                ("kind", ctypes.c_uint * 64), ("arg", ctypes.c_uint * 64),
+                ("total", ctypes.c_uint * 64)]
+
+
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint
 _U64 = ctypes.c_ulonglong
@@ -65,6 +79,7 @@ SIGNATURES = {
                        _P, _P]),
     "msm_precompute_cfg": (_ERR, [_U32, _U32, _U64, _P, _U32]),
     "fused_gate_abc_d_qe": (_ERR, [_P, _P, _U64, ctypes.POINTER(PolySlice), _P, _P]),
+    "impeller_gate": (_ERR, [_P, _P, _U64, _P]),
     "impeller_ntt": (_ERR, [_P, _P, _U32, _INT, _P, _P, _U32, _P]),
     "impeller_twiddles": (_ERR, [_P, _U64, _P, _P]),
     "impeller_ntt_batch_bytes": (ctypes.c_size_t, [_U32]),
@@ -80,7 +95,12 @@ SIGNATURES = {
     "impeller_msm_cfg_workspace_size": (ctypes.c_size_t, [_U64, _U32, _U32, _U32]),
     "impeller_msm_tables": (_U32, [_U32, _U32]),
     "impeller_msm_precompute": (_ERR, [_P, _U64, _U32, _U32, _P]),
-    "impeller_msm_cfg": (_ERR, [_P, _P, _U64, _U32, _U32, _U32, _P, _P, ctypes.c_size_t, _P]),
+    "impeller_msm_cfg": (_ERR, [_P, _P, _U64, _U64, _U32, _U32, _U32, _P, _P, ctypes.c_size_t,
+                                _P]),
+    "impeller_msm_batch_bytes": (ctypes.c_size_t, [ctypes.POINTER(ImpMsmItem), _U32]),
+    "impeller_msm_batch_prepare": (_ERR, [ctypes.POINTER(ImpMsmItem), _U32, _P, ctypes.c_size_t,
+                                          ctypes.POINTER(ImpMsmBatch), _P]),
+    "impeller_msm_batch_run": (_ERR, [ctypes.POINTER(ImpMsmBatch), _U32, _P]),
     "impeller_gen_scalars": (_ERR, [_P, _U64, _INT, _U64, _U64, _P]),
     "impeller_gen_bases": (_ERR, [_P, _U64, _U64, _U64, _P]),
     "impeller_field_op": (_ERR, [_INT, _INT, _P, _P, _P, _U64, _P]),

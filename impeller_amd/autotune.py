@@ -1,0 +1,263 @@
+"""Exhaustive autotuner and learned configuration picker.
+
+The tuner times every candidate configuration of an operation on the GPU
+(HIP events, median of a few warm runs) and records (features, ms) rows in a
+timing database.  The cost model (costmodel.CostModel, an FP32 MLP trained
+on those rows by the library's HIP kernels) then ranks candidates for a new
+problem without a sweep; the exhaustive result stays the fallback and the
+judge of the pick's regret.
+
+Configuration spaces:
+  NTT  plan = [fuse, lds_log, d0, d1, ...]  (fuse in {0, 5}, lds_log in
+       {10, 11}, passes of the remaining levels)
+  MSM  (window c, target windows (0 = per-window buckets, 1 = precomputed
+       window tables), task size T)
+"""
+import itertools
+import json
+import os
+
+import numpy as np
+
+OPS = ["ntt_fwd", "ntt_inv", "ntt_coset", "msm"]
+N_FEATURES = 16
+
+
+# ------------------------------------------------------------------ spaces
+def ntt_plans(L, max_passes=4):
+    """Candidate plans for a 2^L transform."""
+    out = []
+    for fuse in ((0, 5) if L >= 10 else (0,)):
+        rest = L - fuse
+        for lds_log in (10, 11):
+            for k in range(1, max_passes + 1):
+                lo = max(1, rest // k - 2)
+                for parts in itertools.product(range(lo, lds_log + 1), repeat=k):
+                    if sum(parts) != rest:
+                        continue
+                    # keep non-increasing splits only (order matters little)
+                    if list(parts) != sorted(parts, reverse=True):
+                        continue
+                    out.append([fuse, lds_log, *parts])
+    return out
+
+
+def default_ntt_plan(L):
+    fuse = 5 if L >= 10 else 0
+    rest = L - fuse
+    np_ = max(1, (rest + 7) // 8)
+    return [fuse, 10] + [rest // np_ + (1 if i < rest % np_ else 0) for i in range(np_)]
+
+
+def msm_candidates(windows=range(10, 19), targets=(0, 1), tasks=(32, 64, 128)):
+    return [(c, t, T) for c in windows for t in targets for T in tasks]
+
+
+DEFAULT_MSM = (16, 0, 64)
+
+
+# --------------------------------------------------------------- features
+def ntt_features(op, L, plan, batch=1):
+    f = np.zeros(N_FEATURES, np.float32)
+    f[OPS.index(op)] = 1
+    f[4] = L
+    f[5] = plan[0] / 5.0
+    f[6] = plan[1] - 10
+    d = plan[2:]
+    f[7] = len(d)
+    f[8] = max(d)
+    f[9] = min(d)
+    f[15] = np.log2(batch)
+    return f
+
+
+def msm_features(logN, cfg, sparsity=(0.0, 0.0, 0.0), batch=1):
+    c, target, task = cfg
+    f = np.zeros(N_FEATURES, np.float32)
+    f[3] = 1
+    f[4] = logN
+    f[10] = c
+    f[11] = 1.0 if target == 1 else 0.0
+    f[12] = np.log2(task)
+    f[13], f[14] = sparsity[0], sparsity[1]
+    f[7] = sparsity[2]
+    f[15] = np.log2(batch)
+    return f
+
+
+def scalar_sparsity(kind):
+    """(fraction zero, fraction one, fraction small) of the seeded generators."""
+    return {0: (0.0, 0.0, 0.0), 1: (0.9, 0.1, 0.0), 2: (0.0, 0.0, 1.0)}[kind]
+
+
+# ---------------------------------------------------------------- database
+class TimingDB:
+    def __init__(self, rows=None):
+        self.rows = list(rows or [])
+
+    def add(self, problem, config, feats, ms):
+        self.rows.append({"problem": problem, "config": config,
+                          "features": [float(x) for x in feats], "ms": float(ms)})
+
+    def save(self, path):
+        with open(path, "w") as f:
+            json.dump(self.rows, f)
+
+    @classmethod
+    def load(cls, path):
+        if not os.path.exists(path):
+            return cls()
+        with open(path) as f:
+            return cls(json.load(f))
+
+    def problems(self):
+        seen = []
+        for r in self.rows:
+            if r["problem"] not in seen:
+                seen.append(r["problem"])
+        return seen
+
+    def for_problem(self, problem):
+        return [r for r in self.rows if r["problem"] == problem]
+
+
+# ----------------------------------------------------------------- timing
+def _median_ms(fn, stream, reps):
+    import torch
+    fn()
+    times = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1))
+    return float(np.median(times))
+
+
+def tune_ntt(db, L, stream, ops=("ntt_fwd",), reps=3, plans=None):
+    """Time every plan of a 2^L transform for each op kind; returns best per op."""
+    from . import _lib
+    from . import ntt as N
+    from .runtime import device_u32
+    n = 1 << L
+    x = device_u32(n * 8)
+    _lib.call("impeller_gen_scalars", x.data_ptr(), n, 0, 1000 + L, 0, stream.raw())
+    w = device_u32(8)
+    _lib.call("impeller_gen_scalars", w.data_ptr(), 1, 0, 7, 0, stream.raw())
+    stream.sync()
+    omega = w.cpu().numpy().view(np.uint32)
+    tw = N.twiddles(omega, max(1, n // 2), stream)
+    best = {}
+    for op in ops:
+        problem = f"{op}:{L}"
+        for plan in (plans or ntt_plans(L)):
+            kw = {}
+            if op == "ntt_inv":
+                kw["inverse"] = True
+            if op == "ntt_coset":
+                kw["coset_shift"] = w
+
+            def run(plan=plan, kw=kw):
+                N.ntt(x, tw, L, stream, passes=plan, **kw)
+
+            ms = _median_ms(run, stream, reps)
+            db.add(problem, plan, ntt_features(op, L, plan), ms)
+            if problem not in best or ms < best[problem][1]:
+                best[problem] = (plan, ms)
+    return best
+
+
+def tune_msm(db, logN, bases, stream, kinds=(0,), cands=None, reps=2):
+    """Time every MSM configuration on the first 2^logN bases for each scalar kind."""
+    import torch
+    from . import _lib
+    from . import msm as M
+    from .runtime import device_u32
+    n = 1 << logN
+    cands = cands or msm_candidates()
+    tables = {}
+    best = {}
+    scal = {}
+    for kind in kinds:
+        s = device_u32(n * 8)
+        _lib.call("impeller_gen_scalars", s.data_ptr(), n, kind, 500 + kind, 0, stream.raw())
+        scal[kind] = s
+    for (c, target, task) in cands:
+        key = (c, target)
+        if key not in tables:
+            tables.clear()
+            torch.cuda.empty_cache()
+            nt = M.n_tables(c, target)
+            t = device_u32(nt * n * 16)
+            t[: n * 16].copy_(bases[: n * 16])
+            M.precompute_device(t, n, c, target, stream)
+            tables[key] = t
+        t = tables[key]
+        ws = device_u32((M.cfg_workspace_size(n, c, target, task) + 3) // 4)
+        out = device_u32(16)
+        for kind in kinds:
+            problem = f"msm:{logN}:{kind}"
+
+            def run(c=c, target=target, task=task, kind=kind):
+                M.msm_device_cfg(t, scal[kind], n, c, target, stream, task=task, workspace=ws,
+                                 out=out, sync=False)
+
+            ms = _median_ms(run, stream, reps)
+            cfg = [c, target, task]
+            db.add(problem, cfg, msm_features(logN, cfg, scalar_sparsity(kind)), ms)
+            if problem not in best or ms < best[problem][1]:
+                best[problem] = (cfg, ms)
+        del ws
+    return best
+
+
+# ------------------------------------------------------------ model picks
+def fit_model(db, stream, rows=None, steps=1500, seed=0):
+    from .costmodel import CostModel
+    rows = rows if rows is not None else db.rows
+    feats = np.array([r["features"] for r in rows], np.float32)
+    ms = np.array([r["ms"] for r in rows])
+    model = CostModel(N_FEATURES, 64, seed)
+    model.fit(feats, ms, stream, steps=steps)
+    return model
+
+
+def pick(model, rows, stream):
+    """The row (configuration) the model predicts fastest among candidates."""
+    feats = np.array([r["features"] for r in rows], np.float32)
+    pred = model.predict_ms(feats, stream)
+    return rows[int(np.argmin(pred))]
+
+
+def regret_report(db, stream, steps=1500, holdout=True):
+    """Per problem: measured ms of the model's pick vs the exhaustive best and
+    vs the library default.  With holdout, the model for a problem is trained
+    without that problem's rows (leave-one-problem-out), i.e. a pick without
+    a sweep of that problem."""
+    report = {}
+    probs = db.problems()
+    full = None if holdout else fit_model(db, stream, steps=steps)
+    for p in probs:
+        rows = db.for_problem(p)
+        best = min(rows, key=lambda r: r["ms"])
+        model = fit_model(db, stream, [r for r in db.rows if r["problem"] != p], steps) \
+            if holdout else full
+        choice = pick(model, rows, stream)
+        op = p.split(":")[0]
+        if op == "msm":
+            default = [r for r in rows if tuple(r["config"]) == DEFAULT_MSM]
+        else:
+            L = int(p.split(":")[1])
+            default = [r for r in rows if r["config"] == default_ntt_plan(L)]
+        d_ms = default[0]["ms"] if default else None
+        report[p] = {
+            "best_ms": best["ms"], "best_config": best["config"],
+            "model_ms": choice["ms"], "model_config": choice["config"],
+            "model_regret": choice["ms"] / best["ms"] - 1.0,
+            "default_ms": d_ms,
+            "default_regret": (d_ms / best["ms"] - 1.0) if d_ms else None,
+        }
+    return report

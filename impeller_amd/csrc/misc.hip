@@ -142,6 +142,29 @@ extern "C" hipError_t fused_gate_abc_d_qe(const ConstPolyPtr* vars, const PolyPt
   return hipGetLastError();
 }
 
+extern "C" hipError_t impeller_gate(const uint32_t* const* inputs, uint32_t* out,
+                                    unsigned long long len, hipStream_t stream) {
+  IMP_REQUIRE(inputs && out, "null pointer");
+  GateViews g;
+  for (int k = 0; k < 6; k++) {
+    IMP_REQUIRE(inputs[k], "gate input");
+    g.in[k] = View{const_cast<u32*>(inputs[k]), 0, len ? len : 1};
+  }
+  g.out = View{out, 0, len ? len : 1};
+  if (len == 0) return hipSuccess;
+  int dev = 0, cus = 256;
+  IMP_CHECK(hipGetDevice(&dev));
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  u64 blocks = (len + 255) / 256;
+  u64 cap = (u64)cus * 8;
+  if (blocks > cap) blocks = cap;
+  {
+    TimedScope ts(K_GATE, stream);
+    hipLaunchKernelGGL(k_gate, dim3((u32)blocks), dim3(256), 0, stream, g, (u64)len);
+  }
+  return hipGetLastError();
+}
+
 extern "C" hipError_t impeller_gen_scalars(uint32_t* out_d, unsigned long long n, int kind,
                                            unsigned long long seed, unsigned long long start,
                                            hipStream_t stream) {

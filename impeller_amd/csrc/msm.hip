@@ -102,20 +102,62 @@ static MsmLayout msm_layout(const MsmShape& s) {
   return l;
 }
 
-// ------------------------------------------------------------------ kernels
-__global__ void k_zero_u32(u32* p, u64 n) {
-  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
+// ----------------------------------------------------------------- job
+// One MSM problem: its shape plus every device pointer it needs.  The same
+// kernels run a single MSM (a batch of one) or many uneven MSMs at once.
+struct MsmJob {
+  MsmShape s;
+  u64 tstride;  // points between consecutive tables (>= N)
+  const u32* bases;
+  const u32* scalars;
+  u32* result;
+  u32 *kA, *vA, *kB, *vB, *hist, *bstart, *bend, *ntask, *toff, *partA, *partB, *rparts, *wsum;
+};
+
+static MsmJob make_msm_job(const MsmShape& s, u64 tstride, const u32* bases, const u32* scalars,
+                           u32* result, char* ws) {
+  MsmLayout l = msm_layout(s);
+  MsmJob J;
+  J.s = s;
+  J.tstride = tstride;
+  J.bases = bases;
+  J.scalars = scalars;
+  J.result = result;
+  J.kA = (u32*)(ws + l.keysA);
+  J.vA = (u32*)(ws + l.valsA);
+  J.kB = (u32*)(ws + l.keysB);
+  J.vB = (u32*)(ws + l.valsB);
+  J.hist = (u32*)(ws + l.hist);
+  J.bstart = (u32*)(ws + l.bstart);
+  J.bend = (u32*)(ws + l.bend);
+  J.ntask = (u32*)(ws + l.ntask);
+  J.toff = (u32*)(ws + l.toff);
+  J.partA = (u32*)(ws + l.partA);
+  J.partB = (u32*)(ws + l.partB);
+  J.rparts = (u32*)(ws + l.rparts);
+  J.wsum = (u32*)(ws + l.wsum);
+  return J;
 }
 
+__device__ __forceinline__ const u32* sorted_keys(const MsmJob& J) {
+  return (J.s.passes & 1) ? J.kB : J.kA;
+}
+__device__ __forceinline__ const u32* sorted_vals(const MsmJob& J) {
+  return (J.s.passes & 1) ? J.vB : J.vA;
+}
+__device__ __forceinline__ const u32* final_parts(const MsmJob& J) {
+  return (J.s.levels & 1) ? J.partB : J.partA;
+}
+
+// ------------------------------------------------------------ stage bodies
 // Signed c-bit digits (msm_impl.cuh:22-33; a digit equal to 2^(c-1) stays
 // positive, so the top window never carries out).  Logical window w goes to
-// physical window w % n_windows at table offset (w / n_windows) * N
-// (msm_impl.cuh:56-67).  key = bucket |digit|, val = point index | sign << 31.
-__global__ void k_msm_digits(const u32* scalars, MsmShape s, u32* keys, u32* vals) {
-  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+// physical window w % n_windows and table w / n_windows (msm_impl.cuh:56-67).
+// key = bucket |digit|, val = point index (table * tstride + i) | sign << 31.
+__device__ void digits_body(const MsmJob& J, u64 i) {
+  const MsmShape& s = J.s;
   if (i >= s.N) return;
-  Fr v = to_number<FrParams>(load_fe<Fr>(scalars + 8 * i));
+  Fr v = to_number<FrParams>(load_fe<Fr>(J.scalars + 8 * i));
   const u32 c = s.c, half = 1u << (c - 1), mask = (1u << c) - 1;
   u32 carry = 0;
   const u32 slots = s.n_windows * s.n_pre;
@@ -137,44 +179,40 @@ __global__ void k_msm_digits(const u32* scalars, MsmShape s, u32* keys, u32* val
         carry = 0;
       }
     }
-    keys[idx] = key;
-    vals[idx] = (u32)((u64)grp * s.N + i) | sign;
+    J.kA[idx] = key;
+    J.vA[idx] = (u32)((u64)grp * J.tstride + i) | sign;
   }
 }
 
-// ---------------------------------------------------- LSD radix sort (4-bit)
-// Stable per pass: tiles of 16 consecutive keys per thread (blocked), block
-// ranks in (digit, thread) order, global bases from the scanned per-tile
-// histograms hist[(pw * 16 + digit) * tiles + tile].
-__global__ __launch_bounds__(256) void k_sort_hist(const u32* keys, MsmShape s, u32 shift,
-                                                   u32* hist) {
-  __shared__ u32 cnt[16];
-  const u32 tile = blockIdx.x, pw = blockIdx.y, t = threadIdx.x;
+// LSD radix sort, 4-bit digits, stable per pass: 16 consecutive keys per
+// thread (blocked), block ranks in (digit, thread) order, global bases from
+// the scanned per-tile histograms hist[(pw * 16 + digit) * tiles + tile].
+__device__ void hist_body(const MsmJob& J, u32 pass, u64 lb, u32* cnt) {
+  const MsmShape& s = J.s;
+  const u32 tile = (u32)(lb % s.tiles), pw = (u32)(lb / s.tiles), t = threadIdx.x;
+  const u32 shift = 4 * pass;
+  const u32* keys = (pass & 1) ? J.kB : J.kA;
+  __syncthreads();
   if (t < 16) cnt[t] = 0;
   __syncthreads();
   const u32* kk = keys + (u64)pw * s.seg_len;
   u64 base = (u64)tile * kSortTile + (u64)t * 16;
-#pragma unroll
   for (u32 j = 0; j < 16; j++) {
     u64 idx = base + j;
     if (idx < s.seg_len) atomicAdd(&cnt[(kk[idx] >> shift) & 15u], 1u);
   }
   __syncthreads();
-  if (t < 16) hist[((u64)pw * 16 + t) * s.tiles + tile] = cnt[t];
+  if (t < 16) J.hist[((u64)pw * 16 + t) * s.tiles + tile] = cnt[t];
 }
 
-// exclusive scan of len entries per segment (in place allowed); with
-// write_total, out[len] = total (out_stride must then exceed len)
-__global__ __launch_bounds__(1024) void k_seg_scan(const u32* in, u32* out, u64 len,
-                                                   u64 in_stride, u64 out_stride,
-                                                   int write_total) {
-  __shared__ u32 part[1024];
+// block-wide exclusive scan of len entries (in place allowed); with
+// write_total, dst[len] = total
+__device__ void scan_body(const u32* src, u32* dst, u64 len, bool write_total, u32* part) {
   const u32 T = blockDim.x, t = threadIdx.x;
-  const u32* src = in + (u64)blockIdx.x * in_stride;
-  u32* dst = out + (u64)blockIdx.x * out_stride;
   const u64 per = (len + T - 1) / T;
   u64 lo = (u64)t * per, hi = lo + per < len ? lo + per : len;
   u32 sum = 0;
+  __syncthreads();
   for (u64 b = lo; b < hi; b++) sum += src[b];
   part[t] = sum;
   __syncthreads();
@@ -193,26 +231,34 @@ __global__ __launch_bounds__(1024) void k_seg_scan(const u32* in, u32* out, u64 
   if (write_total && t == T - 1) dst[len] = part[T - 1];
 }
 
-__global__ __launch_bounds__(256) void k_sort_scatter(const u32* ksrc, const u32* vsrc, u32* kdst,
-                                                      u32* vdst, MsmShape s, u32 shift,
-                                                      const u32* hist) {
-  __shared__ u32 cnt[16 * 256];
-  __shared__ u32 part[256];
-  __shared__ u32 gbase[16];
-  __shared__ u32 dbase[16];
-  const u32 tile = blockIdx.x, pw = blockIdx.y, t = threadIdx.x;
+struct ScatterLds {
+  u32 cnt[16 * 256];
+  u32 part[256];
+  u32 gbase[16];
+  u32 dbase[16];
+};
+
+__device__ void scatter_body(const MsmJob& J, u32 pass, u64 lb, ScatterLds& L) {
+  const MsmShape& s = J.s;
+  const u32 tile = (u32)(lb % s.tiles), pw = (u32)(lb / s.tiles), t = threadIdx.x;
+  const u32 shift = 4 * pass;
+  const u32* ksrc = (pass & 1) ? J.kB : J.kA;
+  const u32* vsrc = (pass & 1) ? J.vB : J.vA;
+  u32* kdst = (pass & 1) ? J.kA : J.kB;
+  u32* vdst = (pass & 1) ? J.vA : J.vB;
   const u64 seg0 = (u64)pw * s.seg_len;
   const u64 base = (u64)tile * kSortTile + (u64)t * 16;
   u32 key[16], val[16];
+  __syncthreads();
 #pragma unroll
-  for (u32 d = 0; d < 16; d++) cnt[d * 256 + t] = 0;
+  for (u32 d = 0; d < 16; d++) L.cnt[d * 256 + t] = 0;
 #pragma unroll
   for (u32 j = 0; j < 16; j++) {
     u64 idx = base + j;
     if (idx < s.seg_len) {
       key[j] = ksrc[seg0 + idx];
       val[j] = vsrc[seg0 + idx];
-      cnt[((key[j] >> shift) & 15u) * 256 + t] += 1;
+      L.cnt[((key[j] >> shift) & 15u) * 256 + t] += 1;
     }
   }
   __syncthreads();
@@ -221,65 +267,69 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const u32* ksrc, const u32
   u32 sum = 0;
 #pragma unroll
   for (u32 j = 0; j < 16; j++) {
-    loc[j] = cnt[t * 16 + j];
+    loc[j] = L.cnt[t * 16 + j];
     sum += loc[j];
   }
-  part[t] = sum;
+  L.part[t] = sum;
   __syncthreads();
   for (u32 d = 1; d < 256; d <<= 1) {
-    u32 v = t >= d ? part[t - d] : 0;
+    u32 v = t >= d ? L.part[t - d] : 0;
     __syncthreads();
-    part[t] += v;
+    L.part[t] += v;
     __syncthreads();
   }
-  u32 run = t ? part[t - 1] : 0;
+  u32 run = t ? L.part[t - 1] : 0;
 #pragma unroll
   for (u32 j = 0; j < 16; j++) {
-    cnt[t * 16 + j] = run;
+    L.cnt[t * 16 + j] = run;
     run += loc[j];
   }
   __syncthreads();
   if (t < 16) {
-    gbase[t] = hist[((u64)pw * 16 + t) * s.tiles + tile];
-    dbase[t] = cnt[t * 256];
+    L.gbase[t] = J.hist[((u64)pw * 16 + t) * s.tiles + tile];
+    L.dbase[t] = L.cnt[t * 256];
   }
   __syncthreads();
-  // rank within digit d = scan(d, t) - scan(d, 0); cnt column t is private
+  // rank within digit d = scan(d, t) - scan(d, 0); column t of cnt is private
 #pragma unroll
   for (u32 j = 0; j < 16; j++) {
     u64 idx = base + j;
     if (idx < s.seg_len) {
       u32 d = (key[j] >> shift) & 15u;
-      u32 r = cnt[d * 256 + t];
-      cnt[d * 256 + t] = r + 1;
-      u64 pos = (u64)gbase[d] + r - dbase[d];
+      u32 r = L.cnt[d * 256 + t];
+      L.cnt[d * 256 + t] = r + 1;
+      u64 pos = (u64)L.gbase[d] + r - L.dbase[d];
       kdst[seg0 + pos] = key[j];
       vdst[seg0 + pos] = val[j];
     }
   }
 }
 
+__device__ void zero_body(const MsmJob& J, u64 i) {
+  u64 nbw = (u64)J.s.n_windows * (J.s.nb + 1);
+  if (i < nbw) J.bstart[i] = 0;
+  else if (i < 2 * nbw) J.bend[i - nbw] = 0;
+}
+
 // bucket run bounds in the sorted keys (bucket 0 = zero digit, skipped)
-__global__ void k_bucket_bounds(const u32* keys, MsmShape s, u32* bstart, u32* bend) {
-  u64 gi = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ void bounds_body(const MsmJob& J, u64 gi) {
+  const MsmShape& s = J.s;
   if (gi >= (u64)s.n_windows * s.seg_len) return;
   u32 pw = (u32)(gi / s.seg_len);
   u64 i = gi - (u64)pw * s.seg_len;
-  const u32* kk = keys + (u64)pw * s.seg_len;
+  const u32* kk = sorted_keys(J) + (u64)pw * s.seg_len;
   u32 k = kk[i];
   if (k == 0) return;
-  u32* st = bstart + (u64)pw * (s.nb + 1);
-  u32* en = bend + (u64)pw * (s.nb + 1);
-  if (i == 0 || kk[i - 1] != k) st[k] = (u32)i;
-  if (i + 1 == s.seg_len || kk[i + 1] != k) en[k] = (u32)(i + 1);
+  if (i == 0 || kk[i - 1] != k) J.bstart[(u64)pw * (s.nb + 1) + k] = (u32)i;
+  if (i + 1 == s.seg_len || kk[i + 1] != k) J.bend[(u64)pw * (s.nb + 1) + k] = (u32)(i + 1);
 }
 
-__global__ void k_task_counts(MsmShape s, const u32* bstart, const u32* bend, u32* ntask) {
-  u64 gi = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ void tcount_body(const MsmJob& J, u64 gi) {
+  const MsmShape& s = J.s;
   if (gi >= (u64)s.n_windows * (s.nb + 1)) return;
   u32 b = (u32)(gi % (s.nb + 1));
-  u32 cnt = bend[gi] - bstart[gi];
-  ntask[gi] = b == 0 ? 0u : (cnt + s.T - 1) / s.T;
+  u32 cnt = J.bend[gi] - J.bstart[gi];
+  J.ntask[gi] = b == 0 ? 0u : (cnt + s.T - 1) / s.T;
 }
 
 // largest b in [1, nb] with toff[b] <= t
@@ -294,48 +344,45 @@ __device__ __forceinline__ u32 find_bucket(const u32* toff, u32 nb, u32 t) {
 }
 
 // one task = up to T consecutive entries of one bucket, mixed XYZZ + affine
-// additions; the next point is loaded while the current one is added.
-__global__ __launch_bounds__(256) void k_bucket_tasks(const u32* bases, const u32* vals,
-                                                      MsmShape s, const u32* bstart,
-                                                      const u32* bend, const u32* toff,
-                                                      u32* parts) {
-  u64 gi = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+// additions; the next point is loaded while the current one is added
+__device__ void tasks_body(const MsmJob& J, u64 gi) {
+  const MsmShape& s = J.s;
   u32 pw = (u32)(gi / s.task_cap);
   if (pw >= s.n_windows) return;
   u32 t = (u32)(gi - (u64)pw * s.task_cap);
-  const u32* to = toff + (u64)pw * (s.nb + 2);
+  const u32* to = J.toff + (u64)pw * (s.nb + 2);
   if (t >= to[s.nb + 1]) return;
   u32 b = find_bucket(to, s.nb, t);
   u32 k = t - to[b];
-  u32 lo = bstart[(u64)pw * (s.nb + 1) + b] + k * s.T;
-  u32 be = bend[(u64)pw * (s.nb + 1) + b];
+  u32 lo = J.bstart[(u64)pw * (s.nb + 1) + b] + k * s.T;
+  u32 be = J.bend[(u64)pw * (s.nb + 1) + b];
   u32 hi = lo + s.T < be ? lo + s.T : be;
-  const u32* vv = vals + (u64)pw * s.seg_len;
+  const u32* vv = sorted_vals(J) + (u64)pw * s.seg_len;
   G1Xyzz acc = G1Xyzz::identity();
   u32 e = vv[lo];
-  G1Aff p = load_aff(bases + 16 * (u64)(e & 0x7fffffffu));
+  G1Aff p = load_aff(J.bases + 16 * (u64)(e & 0x7fffffffu));
   for (u32 j = lo; j < hi; j++) {
     G1Aff cur = p;
     u32 ce = e;
     if (j + 1 < hi) {
       e = vv[j + 1];
-      p = load_aff(bases + 16 * (u64)(e & 0x7fffffffu));
+      p = load_aff(J.bases + 16 * (u64)(e & 0x7fffffffu));
     }
     if (ce & 0x80000000u) cur = aff_neg(cur);
     acc = xyzz_madd(acc, cur);
   }
-  store_xyzz(parts + 32 * ((u64)pw * s.task_cap + t), acc);
+  store_xyzz(J.partA + 32 * ((u64)pw * s.task_cap + t), acc);
 }
 
-// level l of the per-bucket partial-sum tree: group k of bucket b sums
-// src[toff_b + kG .. toff_b + kG + G) (live entries only) into dst[toff_b + k]
-__global__ __launch_bounds__(256) void k_bucket_combine(const u32* src, u32* dst, MsmShape s,
-                                                        const u32* toff, u32 level) {
-  u64 gi = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+// level l of the per-bucket partial-sum tree: group k of bucket b sums the
+// live entries src[toff_b + kG .. + G) into dst[toff_b + k]
+__device__ void combine_body(const MsmJob& J, u32 level, u64 gi) {
+  const MsmShape& s = J.s;
+  if (level > s.levels) return;
   u32 pw = (u32)(gi / s.task_cap);
   if (pw >= s.n_windows) return;
   u32 t = (u32)(gi - (u64)pw * s.task_cap);
-  const u32* to = toff + (u64)pw * (s.nb + 2);
+  const u32* to = J.toff + (u64)pw * (s.nb + 2);
   if (t >= to[s.nb + 1]) return;
   u32 b = find_bucket(to, s.nb, t);
   u32 k = t - to[b];
@@ -344,6 +391,8 @@ __global__ __launch_bounds__(256) void k_bucket_combine(const u32* src, u32* dst
   u32 n_prev = (u32)(((u64)nt + (1ull << sh_prev) - 1) >> sh_prev);
   u32 n_cur = (u32)(((u64)nt + (1ull << sh) - 1) >> sh);
   if (k >= n_cur) return;
+  const u32* src = (level & 1) ? J.partA : J.partB;
+  u32* dst = (level & 1) ? J.partB : J.partA;
   const u64 base = (u64)pw * s.task_cap + to[b];
   const u32 G = 1u << s.g;
   G1Xyzz acc = load_xyzz(src + 32 * (base + (u64)k * G));
@@ -354,13 +403,13 @@ __global__ __launch_bounds__(256) void k_bucket_combine(const u32* src, u32* dst
 
 // segment [lo, lo + seg) of buckets (bucket number b = lo + k + 1):
 // partial = sum_k (lo + k + 1) * B = T + lo * R with running sums
-__global__ void k_msm_reduce(MsmShape s, const u32* parts, const u32* toff, u32* rparts) {
-  u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ void reduce_body(const MsmJob& J, u64 t) {
+  const MsmShape& s = J.s;
   if (t >= (u64)s.n_windows * s.nseg) return;
   u32 pw = (u32)(t / s.nseg), sg = (u32)(t % s.nseg);
   u32 lo = sg * s.seg;
-  const u32* to = toff + (u64)pw * (s.nb + 2);
-  const u32* pp = parts + 32 * (u64)pw * s.task_cap;
+  const u32* to = J.toff + (u64)pw * (s.nb + 2);
+  const u32* pp = final_parts(J) + 32 * (u64)pw * s.task_cap;
   G1Xyzz run = G1Xyzz::identity(), tot = G1Xyzz::identity();
   for (int k = (int)s.seg - 1; k >= 0; k--) {
     u32 b = lo + k + 1;
@@ -368,17 +417,17 @@ __global__ void k_msm_reduce(MsmShape s, const u32* parts, const u32* toff, u32*
     tot = xyzz_add(tot, run);
   }
   if (lo) tot = xyzz_add(tot, xyzz_mul_u32(run, lo));
-  store_xyzz(rparts + 32 * t, tot);
+  store_xyzz(J.rparts + 32 * t, tot);
 }
 
 // one block per physical window: sum of its segment partials
-__global__ __launch_bounds__(256) void k_msm_sum_partials(MsmShape s, const u32* rparts,
-                                                          u32* wsum) {
-  __shared__ u32 red[256 * 32];
-  const u32 pw = blockIdx.x, t = threadIdx.x;
+__device__ void sum_body(const MsmJob& J, u64 pw, u32* red) {
+  const MsmShape& s = J.s;
+  const u32 t = threadIdx.x;
   G1Xyzz acc = G1Xyzz::identity();
+  __syncthreads();
   for (u32 k = t; k < s.nseg; k += blockDim.x)
-    acc = xyzz_add(acc, load_xyzz(rparts + 32 * ((u64)pw * s.nseg + k)));
+    acc = xyzz_add(acc, load_xyzz(J.rparts + 32 * (pw * s.nseg + k)));
   store_xyzz(red + 32 * t, acc);
   __syncthreads();
   for (u32 d = blockDim.x / 2; d > 0; d >>= 1) {
@@ -388,21 +437,103 @@ __global__ __launch_bounds__(256) void k_msm_sum_partials(MsmShape s, const u32*
     }
     __syncthreads();
   }
-  if (t < 32) wsum[32 * pw + t] = red[t];
+  if (t < 32) J.wsum[32 * pw + t] = red[t];
 }
 
-// Horner over physical windows, then affine
-__global__ void k_msm_final(MsmShape s, const u32* wsum, u32* result) {
-  G1Xyzz acc = load_xyzz(wsum + 32 * (s.n_windows - 1));
+// Horner over physical windows, then affine (thread 0 of the job's block)
+__device__ void final_body(const MsmJob& J) {
+  if (threadIdx.x != 0) return;
+  const MsmShape& s = J.s;
+  if (s.N == 0) {
+    for (int i = 0; i < 16; i++) J.result[i] = 0;
+    return;
+  }
+  G1Xyzz acc = load_xyzz(J.wsum + 32 * (s.n_windows - 1));
   for (int pw = (int)s.n_windows - 2; pw >= 0; pw--) {
     for (u32 k = 0; k < s.c; k++) acc = xyzz_dbl(acc);
-    acc = xyzz_add(acc, load_xyzz(wsum + 32 * pw));
+    acc = xyzz_add(acc, load_xyzz(J.wsum + 32 * pw));
   }
-  store_aff(result, xyzz_to_aff(acc));
+  store_aff(J.result, xyzz_to_aff(acc));
 }
 
-__global__ void k_store_identity_aff(u32* result) {
-  if (threadIdx.x < 16) result[threadIdx.x] = 0;
+// ------------------------------------------------------- batched kernels
+// Every stage is one launch over the concatenated block space of all jobs;
+// prefix[j] = first block of job j (jobs without work contribute none), and
+// work-groups stride over that space.
+__device__ __forceinline__ u32 find_job(const u32* prefix, u32 njobs, u32 t) {
+  u32 lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    u32 mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+enum MsmStage : u32 {
+  MS_DIGITS = 0, MS_HIST, MS_HSCAN, MS_SCATTER, MS_ZERO, MS_BOUNDS, MS_TCOUNT, MS_TSCAN,
+  MS_TASKS, MS_COMBINE, MS_REDUCE, MS_SUM, MS_FINAL
+};
+
+#define IMP_BATCH_LOOP(BODY)                                        \
+  const u32 total = prefix[njobs];                                  \
+  for (u32 t = blockIdx.x; t < total; t += gridDim.x) {             \
+    u32 j = find_job(prefix, njobs, t);                             \
+    const MsmJob& J = jobs[j];                                      \
+    const u64 lb = t - prefix[j];                                   \
+    BODY;                                                           \
+  }
+
+__global__ __launch_bounds__(256) void kb_digits(const MsmJob* jobs, u32 njobs, const u32* prefix) {
+  IMP_BATCH_LOOP(digits_body(J, lb * blockDim.x + threadIdx.x))
+}
+__global__ __launch_bounds__(256) void kb_hist(const MsmJob* jobs, u32 njobs, const u32* prefix,
+                                               u32 pass) {
+  __shared__ u32 cnt[16];
+  IMP_BATCH_LOOP(hist_body(J, pass, lb, cnt))
+}
+__global__ __launch_bounds__(1024) void kb_hscan(const MsmJob* jobs, u32 njobs, const u32* prefix) {
+  __shared__ u32 part[1024];
+  IMP_BATCH_LOOP({
+    u32* h = J.hist + lb * 16 * J.s.tiles;
+    scan_body(h, h, (u64)16 * J.s.tiles, false, part);
+  })
+}
+__global__ __launch_bounds__(256) void kb_scatter(const MsmJob* jobs, u32 njobs,
+                                                  const u32* prefix, u32 pass) {
+  __shared__ ScatterLds lds;
+  IMP_BATCH_LOOP(scatter_body(J, pass, lb, lds))
+}
+__global__ __launch_bounds__(256) void kb_zero(const MsmJob* jobs, u32 njobs, const u32* prefix) {
+  IMP_BATCH_LOOP(zero_body(J, lb * blockDim.x + threadIdx.x))
+}
+__global__ __launch_bounds__(256) void kb_bounds(const MsmJob* jobs, u32 njobs, const u32* prefix) {
+  IMP_BATCH_LOOP(bounds_body(J, lb * blockDim.x + threadIdx.x))
+}
+__global__ __launch_bounds__(256) void kb_tcount(const MsmJob* jobs, u32 njobs, const u32* prefix) {
+  IMP_BATCH_LOOP(tcount_body(J, lb * blockDim.x + threadIdx.x))
+}
+__global__ __launch_bounds__(1024) void kb_tscan(const MsmJob* jobs, u32 njobs, const u32* prefix) {
+  __shared__ u32 part[1024];
+  IMP_BATCH_LOOP(scan_body(J.ntask + lb * (J.s.nb + 1), J.toff + lb * (J.s.nb + 2),
+                           (u64)(J.s.nb + 1), true, part))
+}
+__global__ __launch_bounds__(256) void kb_tasks(const MsmJob* jobs, u32 njobs, const u32* prefix) {
+  IMP_BATCH_LOOP(tasks_body(J, lb * blockDim.x + threadIdx.x))
+}
+__global__ __launch_bounds__(256) void kb_combine(const MsmJob* jobs, u32 njobs,
+                                                  const u32* prefix, u32 level) {
+  IMP_BATCH_LOOP(combine_body(J, level, lb * blockDim.x + threadIdx.x))
+}
+__global__ __launch_bounds__(256) void kb_reduce(const MsmJob* jobs, u32 njobs, const u32* prefix) {
+  IMP_BATCH_LOOP(reduce_body(J, lb * blockDim.x + threadIdx.x))
+}
+__global__ __launch_bounds__(256) void kb_sum(const MsmJob* jobs, u32 njobs, const u32* prefix) {
+  __shared__ u32 red[256 * 32];
+  IMP_BATCH_LOOP(sum_body(J, lb, red))
+}
+__global__ __launch_bounds__(64) void kb_final(const MsmJob* jobs, u32 njobs, const u32* prefix) {
+  IMP_BATCH_LOOP(final_body(J))
 }
 
 // table t = 2^(t * step) * table 0 (affine), t = 1 .. n_pre-1
@@ -417,103 +548,137 @@ __global__ void k_msm_precompute(u32* tables, u64 N, u32 n_pre, u32 step) {
   }
 }
 
-// ------------------------------------------------------------------ driver
-static inline dim3 grid1(u64 n, u32 block) { return dim3((u32)((n + block - 1) / block)); }
+// ------------------------------------------------------------------ plan
+// A launch list over a set of jobs: stage kind, argument (pass or level) and
+// per-job block counts.
+struct StageLaunch {
+  u32 kind, arg, total;
+};
 
-// bases: n_pre tables of N affine points (table t = 2^(t * n_windows * c) * P)
-static hipError_t msm_run(const u32* bases, const u32* scalars, const MsmShape& s,
-                          u32* result, char* ws, hipStream_t stream) {
-  if (s.N == 0) {
-    hipLaunchKernelGGL(k_store_identity_aff, dim3(1), dim3(64), 0, stream, result);
-    return hipGetLastError();
+static inline u64 cdiv(u64 a, u64 b) { return (a + b - 1) / b; }
+
+static u32 stage_blocks(const MsmShape& s, u32 kind, u32 arg) {
+  if (s.N == 0) return kind == MS_FINAL ? 1u : 0u;
+  switch (kind) {
+    case MS_DIGITS: return (u32)cdiv(s.N, 256);
+    case MS_HIST:
+    case MS_SCATTER: return arg < s.passes ? s.tiles * s.n_windows : 0;
+    case MS_HSCAN: return arg < s.passes ? s.n_windows : 0;
+    case MS_ZERO: return (u32)cdiv(2ull * s.n_windows * (s.nb + 1), 256);
+    case MS_BOUNDS: return (u32)cdiv((u64)s.n_windows * s.seg_len, 256);
+    case MS_TCOUNT: return (u32)cdiv((u64)s.n_windows * (s.nb + 1), 256);
+    case MS_TSCAN: return s.n_windows;
+    case MS_TASKS: return (u32)cdiv((u64)s.n_windows * s.task_cap, 256);
+    case MS_COMBINE: return arg <= s.levels ? (u32)cdiv((u64)s.n_windows * s.task_cap, 256) : 0;
+    case MS_REDUCE: return (u32)cdiv((u64)s.n_windows * s.nseg, 256);
+    case MS_SUM: return s.n_windows;
+    default: return 1;
   }
-  MsmLayout l = msm_layout(s);
-  u32* kA = (u32*)(ws + l.keysA);
-  u32* vA = (u32*)(ws + l.valsA);
-  u32* kB = (u32*)(ws + l.keysB);
-  u32* vB = (u32*)(ws + l.valsB);
-  u32* hist = (u32*)(ws + l.hist);
-  u32* bstart = (u32*)(ws + l.bstart);
-  u32* bend = (u32*)(ws + l.bend);
-  u32* ntask = (u32*)(ws + l.ntask);
-  u32* toff = (u32*)(ws + l.toff);
-  u32* partA = (u32*)(ws + l.partA);
-  u32* partB = (u32*)(ws + l.partB);
-  u32* rparts = (u32*)(ws + l.rparts);
-  u32* wsum = (u32*)(ws + l.wsum);
-  const u32 TB = 256;
-  {
-    TimedScope ts_(K_MSM_DIGITS, stream);
-    hipLaunchKernelGGL(k_msm_digits, grid1(s.N, TB), dim3(TB), 0, stream, scalars, s, kA, vA);
-  }
-  {
-    TimedScope ts_(K_MSM_SORT, stream);
-    u32 *ks = kA, *vs = vA, *kd = kB, *vd = vB;
-    dim3 tg(s.tiles, s.n_windows);
-    for (u32 p = 0; p < s.passes; p++) {
-      hipLaunchKernelGGL(k_sort_hist, tg, dim3(256), 0, stream, ks, s, 4 * p, hist);
-      hipLaunchKernelGGL(k_seg_scan, dim3(s.n_windows), dim3(1024), 0, stream, hist, hist,
-                         (u64)16 * s.tiles, (u64)16 * s.tiles, (u64)16 * s.tiles, 0);
-      hipLaunchKernelGGL(k_sort_scatter, tg, dim3(256), 0, stream, ks, vs, kd, vd, s, 4 * p,
-                         hist);
-      u32* tk = ks; ks = kd; kd = tk;
-      u32* tv = vs; vs = vd; vd = tv;
-    }
-    kA = ks;  // sorted keys / values
-    vA = vs;
-  }
-  {
-    TimedScope ts_(K_MSM_SCAN, stream);
-    u64 nbw = (u64)s.n_windows * (s.nb + 1);
-    hipLaunchKernelGGL(k_zero_u32, grid1(nbw, TB), dim3(TB), 0, stream, bstart, nbw);
-    hipLaunchKernelGGL(k_zero_u32, grid1(nbw, TB), dim3(TB), 0, stream, bend, nbw);
-    hipLaunchKernelGGL(k_bucket_bounds, grid1((u64)s.n_windows * s.seg_len, TB), dim3(TB), 0,
-                       stream, kA, s, bstart, bend);
-    hipLaunchKernelGGL(k_task_counts, grid1(nbw, TB), dim3(TB), 0, stream, s, bstart, bend,
-                       ntask);
-    hipLaunchKernelGGL(k_seg_scan, dim3(s.n_windows), dim3(1024), 0, stream, ntask, toff,
-                       (u64)(s.nb + 1), (u64)(s.nb + 1), (u64)(s.nb + 2), 1);
-  }
-  const u64 ntasks = (u64)s.n_windows * s.task_cap;
-  {
-    TimedScope ts_(K_MSM_BUCKETS, stream);
-    hipLaunchKernelGGL(k_bucket_tasks, grid1(ntasks, TB), dim3(TB), 0, stream, bases, vA, s,
-                       bstart, bend, toff, partA);
-  }
-  u32* fin = partA;
-  {
-    TimedScope ts_(K_MSM_COMBINE, stream);
-    u32 *src = partA, *dst = partB;
-    for (u32 lv = 1; lv <= s.levels; lv++) {
-      hipLaunchKernelGGL(k_bucket_combine, grid1(ntasks, TB), dim3(TB), 0, stream, src, dst, s,
-                         toff, lv);
-      u32* tmp = src; src = dst; dst = tmp;
-    }
-    fin = src;
-  }
-  {
-    TimedScope ts_(K_MSM_REDUCE, stream);
-    hipLaunchKernelGGL(k_msm_reduce, grid1((u64)s.n_windows * s.nseg, TB), dim3(TB), 0, stream,
-                       s, fin, toff, rparts);
-  }
-  {
-    TimedScope ts_(K_MSM_SUM, stream);
-    hipLaunchKernelGGL(k_msm_sum_partials, dim3(s.n_windows), dim3(256), 0, stream, s, rparts,
-                       wsum);
-  }
-  {
-    TimedScope ts_(K_MSM_FINAL, stream);
-    hipLaunchKernelGGL(k_msm_final, dim3(1), dim3(1), 0, stream, s, wsum, result);
-  }
-  return hipGetLastError();
 }
+
+static void plan_stages(const std::vector<MsmShape>& shapes, std::vector<StageLaunch>& out,
+                        std::vector<u32>& prefix) {
+  u32 maxp = 0, maxl = 0;
+  for (auto& s : shapes) {
+    if (s.passes > maxp) maxp = s.passes;
+    if (s.levels > maxl) maxl = s.levels;
+  }
+  std::vector<std::pair<u32, u32>> kinds;
+  kinds.push_back({MS_DIGITS, 0});
+  for (u32 p = 0; p < maxp; p++) {
+    kinds.push_back({MS_HIST, p});
+    kinds.push_back({MS_HSCAN, p});
+    kinds.push_back({MS_SCATTER, p});
+  }
+  kinds.push_back({MS_ZERO, 0});
+  kinds.push_back({MS_BOUNDS, 0});
+  kinds.push_back({MS_TCOUNT, 0});
+  kinds.push_back({MS_TSCAN, 0});
+  kinds.push_back({MS_TASKS, 0});
+  for (u32 l = 1; l <= maxl; l++) kinds.push_back({MS_COMBINE, l});
+  kinds.push_back({MS_REDUCE, 0});
+  kinds.push_back({MS_SUM, 0});
+  kinds.push_back({MS_FINAL, 0});
+  const u32 n = (u32)shapes.size();
+  out.clear();
+  prefix.assign(kinds.size() * (n + 1), 0);
+  for (size_t k = 0; k < kinds.size(); k++) {
+    u32 run = 0;
+    for (u32 j = 0; j < n; j++) {
+      prefix[k * (n + 1) + j] = run;
+      run += stage_blocks(shapes[j], kinds[k].first, kinds[k].second);
+    }
+    prefix[k * (n + 1) + n] = run;
+    out.push_back(StageLaunch{kinds[k].first, kinds[k].second, run});
+  }
+}
+
+static const u32 kTimerOf[] = {K_MSM_DIGITS, K_MSM_SORT,   K_MSM_SORT,    K_MSM_SORT, K_MSM_SCAN,
+                               K_MSM_SCAN,   K_MSM_SCAN,   K_MSM_SCAN,    K_MSM_BUCKETS,
+                               K_MSM_COMBINE, K_MSM_REDUCE, K_MSM_SUM,    K_MSM_FINAL};
+
+static hipError_t launch_stages(const MsmJob* jobs, u32 njobs, const u32* prefix_dev,
+                                const StageLaunch* st, u32 nst, u32 max_blocks,
+                                hipStream_t stream) {
+  for (u32 k = 0; k < nst; k++) {
+    u32 total = st[k].total;
+    if (total == 0) continue;
+    u32 grid = max_blocks && total > max_blocks ? max_blocks : total;
+    const u32* pf = prefix_dev + (size_t)k * (njobs + 1);
+    TimedScope ts(kTimerOf[st[k].kind], stream);
+    switch (st[k].kind) {
+      case MS_DIGITS: hipLaunchKernelGGL(kb_digits, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
+      case MS_HIST: hipLaunchKernelGGL(kb_hist, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf, st[k].arg); break;
+      case MS_HSCAN: hipLaunchKernelGGL(kb_hscan, dim3(grid), dim3(1024), 0, stream, jobs, njobs, pf); break;
+      case MS_SCATTER: hipLaunchKernelGGL(kb_scatter, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf, st[k].arg); break;
+      case MS_ZERO: hipLaunchKernelGGL(kb_zero, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
+      case MS_BOUNDS: hipLaunchKernelGGL(kb_bounds, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
+      case MS_TCOUNT: hipLaunchKernelGGL(kb_tcount, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
+      case MS_TSCAN: hipLaunchKernelGGL(kb_tscan, dim3(grid), dim3(1024), 0, stream, jobs, njobs, pf); break;
+      case MS_TASKS: hipLaunchKernelGGL(kb_tasks, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
+      case MS_COMBINE: hipLaunchKernelGGL(kb_combine, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf, st[k].arg); break;
+      case MS_REDUCE: hipLaunchKernelGGL(kb_reduce, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
+      case MS_SUM: hipLaunchKernelGGL(kb_sum, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
+      default: hipLaunchKernelGGL(kb_final, dim3(grid), dim3(64), 0, stream, jobs, njobs, pf); break;
+    }
+    IMP_CHECK(hipGetLastError());
+  }
+  return hipSuccess;
+}
+
+// descriptor area at the front of a workspace: jobs, then stage prefixes
+static const size_t kMaxStages = 64;
+static size_t desc_bytes(u32 njobs) {
+  return align256(sizeof(MsmJob) * njobs) + align256(kMaxStages * (njobs + 1) * 4);
+}
+
+// Single MSM: descriptor + prefix uploaded into the workspace, then the stage
+// launches.  The upload is synchronised so the host staging can go away.
+static hipError_t msm_run(const u32* bases, u64 tstride, const u32* scalars, const MsmShape& s,
+                          u32* result, char* ws, hipStream_t stream) {
+  char* body = ws + desc_bytes(1);
+  MsmJob J = make_msm_job(s, tstride, bases, scalars, result, body);
+  std::vector<MsmShape> shapes(1, s);
+  std::vector<StageLaunch> st;
+  std::vector<u32> prefix;
+  plan_stages(shapes, st, prefix);
+  MsmJob* jobs_d = (MsmJob*)ws;
+  u32* prefix_d = (u32*)(ws + align256(sizeof(MsmJob)));
+  IMP_CHECK(hipMemcpyAsync(jobs_d, &J, sizeof(MsmJob), hipMemcpyHostToDevice, stream));
+  IMP_CHECK(hipMemcpyAsync(prefix_d, prefix.data(), prefix.size() * 4, hipMemcpyHostToDevice,
+                           stream));
+  IMP_CHECK(hipStreamSynchronize(stream));
+  return launch_stages(jobs_d, 1, prefix_d, st.data(), (u32)st.size(), 0, stream);
+}
+
+static size_t msm_ws_total(const MsmShape& s) { return desc_bytes(1) + msm_layout(s).total; }
 
 static bool valid_window(u32 c) { return c >= 4 && c <= 20; }
 
 static hipError_t precompute_device(u32* tables, u64 N, const MsmShape& s, hipStream_t stream) {
   if (s.n_pre <= 1 || N == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_msm_precompute, grid1(N, 128), dim3(128), 0, stream, tables, N, s.n_pre,
-                     s.n_windows * s.c);
+  hipLaunchKernelGGL(k_msm_precompute, dim3((u32)cdiv(N, 128)), dim3(128), 0, stream, tables, N,
+                     s.n_pre, s.n_windows * s.c);
   return hipGetLastError();
 }
 
@@ -527,11 +692,10 @@ static hipError_t msm_host(u32 window, u32 target, void* const* buffers,
   IMP_REQUIRE(valid_window(window), "window");
   IMP_REQUIRE(num_cards >= 1 && cards, "cards");
   MsmShape s = msm_shape(len, window, target);
-  MsmLayout l = msm_layout(s);
   // per card: points (n_pre tables), scalars, results, MSM workspace
   size_t pts = align256((size_t)s.n_pre * len * 64), scl = align256((size_t)len * 32),
          res = align256(64);
-  size_t need = pts + scl + res + l.total;
+  size_t need = pts + scl + res + msm_ws_total(s);
   if (buffers == nullptr) {
     IMP_REQUIRE(buffer_sizes, "buffer_sizes");
     // the whole MSM runs on the first card; the others need nothing
@@ -555,7 +719,7 @@ static hipError_t msm_host(u32 window, u32 target, void* const* buffers,
   for (u32 b = 0; b < batches; b++) {
     IMP_CHECK(hipMemcpyAsync(d_scl, h_scaler_batch[b], (size_t)len * 32, hipMemcpyHostToDevice,
                              st));
-    IMP_CHECK(msm_run(d_pts, d_scl, s, d_res, ws, st));
+    IMP_CHECK(msm_run(d_pts, len, d_scl, s, d_res, ws, st));
     IMP_CHECK(hipMemcpyAsync(h_result[b], d_res, 64, hipMemcpyDeviceToHost, st));
     IMP_CHECK(hipStreamSynchronize(st));
   }
@@ -627,7 +791,7 @@ extern "C" hipError_t msm_precompute(unsigned long long len, unsigned int* const
 
 extern "C" size_t impeller_msm_workspace_size(unsigned long long len, unsigned int window) {
   if (!valid_window(window)) return 0;
-  return msm_layout(msm_shape(len, window, 0)).total;
+  return msm_ws_total(msm_shape(len, window, 0));
 }
 
 extern "C" hipError_t impeller_msm(const uint32_t* bases_d, const uint32_t* scalars_d,
@@ -637,15 +801,15 @@ extern "C" hipError_t impeller_msm(const uint32_t* bases_d, const uint32_t* scal
   IMP_REQUIRE(valid_window(window), "window");
   IMP_REQUIRE(result_d && workspace && (len == 0 || (bases_d && scalars_d)), "null pointer");
   MsmShape s = msm_shape(len, window, 0);
-  IMP_REQUIRE(workspace_bytes >= msm_layout(s).total, "workspace too small");
-  return msm_run(bases_d, scalars_d, s, result_d, (char*)workspace, stream);
+  IMP_REQUIRE(workspace_bytes >= msm_ws_total(s), "workspace too small");
+  return msm_run(bases_d, len, scalars_d, s, result_d, (char*)workspace, stream);
 }
 
 extern "C" size_t impeller_msm_cfg_workspace_size(unsigned long long len, unsigned int window,
                                                   unsigned int target_windows,
                                                   unsigned int task) {
   if (!valid_window(window) || task == 0) return 0;
-  return msm_layout(msm_shape(len, window, target_windows, task)).total;
+  return msm_ws_total(msm_shape(len, window, target_windows, task));
 }
 
 extern "C" unsigned int impeller_msm_tables(unsigned int window, unsigned int target_windows) {
@@ -662,13 +826,83 @@ extern "C" hipError_t impeller_msm_precompute(uint32_t* tables_d, unsigned long 
 }
 
 extern "C" hipError_t impeller_msm_cfg(const uint32_t* tables_d, const uint32_t* scalars_d,
-                                       unsigned long long len, unsigned int window,
+                                       unsigned long long len, unsigned long long table_stride,
+                                       unsigned int window,
                                        unsigned int target_windows, unsigned int task,
                                        uint32_t* result_d, void* workspace,
                                        size_t workspace_bytes, hipStream_t stream) {
   IMP_REQUIRE(valid_window(window) && task > 0 && task <= 4096, "window / task");
   IMP_REQUIRE(result_d && workspace && (len == 0 || (tables_d && scalars_d)), "null pointer");
   MsmShape s = msm_shape(len, window, target_windows, task);
-  IMP_REQUIRE(workspace_bytes >= msm_layout(s).total, "workspace too small");
-  return msm_run(tables_d, scalars_d, s, result_d, (char*)workspace, stream);
+  IMP_REQUIRE(workspace_bytes >= msm_ws_total(s), "workspace too small");
+  u64 stride = table_stride ? table_stride : len;
+  IMP_REQUIRE(stride >= len && (u64)s.n_pre * stride < (1ull << 31), "table stride");
+  return msm_run(tables_d, stride, scalars_d, s, result_d, (char*)workspace, stream);
+}
+
+// ------------------------------------------------------------ batched MSM
+static MsmShape item_shape(const ImpMsmItem& it) {
+  return msm_shape(it.len, it.window, it.target_windows, it.task ? it.task : 64);
+}
+
+extern "C" size_t impeller_msm_batch_bytes(const ImpMsmItem* items, unsigned int count) {
+  if (!items || count == 0) return 0;
+  size_t total = desc_bytes(count);
+  for (u32 i = 0; i < count; i++) {
+    if (!valid_window(items[i].window)) return 0;
+    total += align256(msm_layout(item_shape(items[i])).total);
+  }
+  return total;
+}
+
+extern "C" hipError_t impeller_msm_batch_prepare(const ImpMsmItem* items, unsigned int count,
+                                                 void* dev, size_t bytes, ImpMsmBatch* out,
+                                                 hipStream_t stream) {
+  IMP_REQUIRE(items && out && dev && count > 0, "null pointer / empty batch");
+  size_t need = impeller_msm_batch_bytes(items, count);
+  IMP_REQUIRE(need > 0 && bytes >= need, "batch buffer too small / bad item");
+  std::vector<MsmShape> shapes;
+  std::vector<MsmJob> jobs;
+  char* base = (char*)dev;
+  size_t off = desc_bytes(count);
+  for (u32 i = 0; i < count; i++) {
+    const ImpMsmItem& it = items[i];
+    IMP_REQUIRE(it.result && (it.len == 0 || (it.tables && it.scalars)), "batch item");
+    MsmShape s = item_shape(it);
+    u64 stride = it.table_stride ? it.table_stride : it.len;
+    IMP_REQUIRE(stride >= it.len, "table stride");
+    IMP_REQUIRE((u64)s.n_pre * stride < (1ull << 31), "point index range");
+    shapes.push_back(s);
+    jobs.push_back(make_msm_job(s, stride, it.tables, it.scalars, it.result, base + off));
+    off += align256(msm_layout(s).total);
+  }
+  std::vector<StageLaunch> st;
+  std::vector<u32> prefix;
+  plan_stages(shapes, st, prefix);
+  IMP_REQUIRE(st.size() <= kMaxStages && st.size() <= sizeof(out->kind) / sizeof(out->kind[0]),
+              "too many stages");
+  out->dev = dev;
+  out->count = count;
+  out->nstages = (unsigned int)st.size();
+  for (size_t k = 0; k < st.size(); k++) {
+    out->kind[k] = st[k].kind;
+    out->arg[k] = st[k].arg;
+    out->total[k] = st[k].total;
+  }
+  IMP_CHECK(hipMemcpyAsync(base, jobs.data(), sizeof(MsmJob) * count, hipMemcpyHostToDevice,
+                           stream));
+  IMP_CHECK(hipMemcpyAsync(base + align256(sizeof(MsmJob) * count), prefix.data(),
+                           prefix.size() * 4, hipMemcpyHostToDevice, stream));
+  return hipStreamSynchronize(stream);
+}
+
+extern "C" hipError_t impeller_msm_batch_run(const ImpMsmBatch* b, unsigned int max_blocks,
+                                             hipStream_t stream) {
+  IMP_REQUIRE(b && b->dev && b->count > 0, "batch");
+  const char* base = (const char*)b->dev;
+  const MsmJob* jobs = (const MsmJob*)base;
+  const u32* prefix = (const u32*)(base + align256(sizeof(MsmJob) * b->count));
+  std::vector<StageLaunch> st(b->nstages);
+  for (u32 k = 0; k < b->nstages; k++) st[k] = StageLaunch{b->kind[k], b->arg[k], b->total[k]};
+  return launch_stages(jobs, b->count, prefix, st.data(), b->nstages, max_blocks, stream);
 }

@@ -39,3 +39,10 @@ class FusedGate:
                       local_buffer.data_ptr() if local_buffer is not None else None,
                       stream.raw())
         return f
+
+
+def gate_device(inputs, out, length, stream):
+    """out = a*b*c - d + q*e*(1 - q) over contiguous device tensors
+    (impeller_gate; kernel launch only, graph-capturable)."""
+    ptrs = (ctypes.c_void_p * 6)(*[t.data_ptr() for t in inputs])
+    _lib.call("impeller_gate", ptrs, out.data_ptr(), length, stream.raw())

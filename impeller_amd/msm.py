@@ -121,7 +121,7 @@ def precompute_device(tables_t, n, window, target, stream):
 
 
 def msm_device_cfg(tables_t, scalars_t, n, window, target, stream, task=64, workspace=None,
-                   out=None, sync=True):
+                   out=None, sync=True, table_stride=0):
     """Device MSM over precomputed tables; returns the (16,) affine result."""
     from .runtime import device_u32, to_host
     ws_bytes = cfg_workspace_size(n, window, target, task)
@@ -129,8 +129,8 @@ def msm_device_cfg(tables_t, scalars_t, n, window, target, stream, task=64, work
         workspace = device_u32((ws_bytes + 3) // 4)
     if out is None:
         out = device_u32(16)
-    _lib.call("impeller_msm_cfg", tables_t.data_ptr(), scalars_t.data_ptr(), n, window, target,
-              task, out.data_ptr(), workspace.data_ptr(), ws_bytes, stream.raw())
+    _lib.call("impeller_msm_cfg", tables_t.data_ptr(), scalars_t.data_ptr(), n, table_stride,
+              window, target, task, out.data_ptr(), workspace.data_ptr(), ws_bytes, stream.raw())
     if not sync:
         return out
     stream.sync()
@@ -148,3 +148,44 @@ def msm_device(bases_t, scalars_t, n, window, stream, workspace=None):
               out.data_ptr(), workspace.data_ptr(), ws_bytes, stream.raw())
     stream.sync()
     return to_host(out, 16)[0]
+
+
+class MsmBatch:
+    """A prepared batch of MSMs (impeller_msm_batch_*).
+
+    items: dicts with tables (device tensor), scalars (device tensor), len,
+    window, target (0: per-window buckets, 1: precomputed tables), task,
+    table_stride (points between tables of a shared set; default len).
+    Results land in self.results (device, 16 words per MSM).
+    """
+
+    def __init__(self, items, stream):
+        from .runtime import device_u32
+        n = len(items)
+        self.results = device_u32(16 * n)
+        arr = (_lib.ImpMsmItem * n)()
+        self._keep = list(items)
+        for i, it in enumerate(items):
+            a = arr[i]
+            a.tables = it["tables"].data_ptr()
+            a.scalars = it["scalars"].data_ptr()
+            a.len = it["len"]
+            a.table_stride = it.get("table_stride", 0)
+            a.window = it["window"]
+            a.target_windows = it.get("target", 0)
+            a.task = it.get("task", 64)
+            a.result = self.results.data_ptr() + 64 * i
+        nbytes = int(_lib.call("impeller_msm_batch_bytes", arr, n))
+        if nbytes == 0:
+            raise ValueError("invalid MSM batch item")
+        self.dev = device_u32((nbytes + 3) // 4)
+        self.info = _lib.ImpMsmBatch()
+        _lib.call("impeller_msm_batch_prepare", arr, n, self.dev.data_ptr(), nbytes,
+                  ctypes.byref(self.info), stream.raw())
+
+    def run(self, stream, max_blocks=0):
+        _lib.call("impeller_msm_batch_run", ctypes.byref(self.info), max_blocks, stream.raw())
+
+    def results_host(self):
+        from .runtime import to_host
+        return to_host(self.results, 16)

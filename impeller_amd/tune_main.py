@@ -1,0 +1,91 @@
+"""Run the exhaustive autotuner on the GPU, train the cost model on its
+timings and evaluate the model's picks.
+
+  python -m impeller_amd.tune_main [--out DIR] [--ntt-logs 18-25] [--msm-logs 16-22]
+
+Writes DIR/timings.json (every measured configuration), DIR/picks.json
+(exhaustive best and model pick per problem, used by the batched workloads)
+and DIR/report.json (leave-one-problem-out regret of the model vs the
+exhaustive best and vs the library default).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def parse_range(s):
+    a, b = s.split("-")
+    return list(range(int(a), int(b) + 1))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=os.path.join(HERE, "tuned"))
+    ap.add_argument("--ntt-logs", default="18-25")
+    ap.add_argument("--msm-logs", default="16-22")
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=1500)
+    args = ap.parse_args()
+    os.makedirs(args.out, exist_ok=True)
+
+    from . import _lib
+    from . import autotune as A
+    from .runtime import Stream, device_u32
+
+    stream = Stream.current(0)
+    db = A.TimingDB()
+    t0 = time.time()
+    picks = {}
+    for L in parse_range(args.ntt_logs):
+        best = A.tune_ntt(db, L, stream, ops=("ntt_fwd", "ntt_inv", "ntt_coset"), reps=args.reps)
+        for prob, (cfg, ms) in best.items():
+            picks[prob] = {"exhaustive": cfg, "exhaustive_ms": ms}
+        print(f"[tune] ntt 2^{L}: {len(A.ntt_plans(L))} plans x 3 ops, {time.time() - t0:.1f}s",
+              flush=True)
+    msm_logs = parse_range(args.msm_logs)
+    nb = 1 << max(msm_logs)
+    bases = device_u32(nb * 16)
+    _lib.call("impeller_gen_bases", bases.data_ptr(), nb, 77, 0, stream.raw())
+    stream.sync()
+    for lg in msm_logs:
+        best = A.tune_msm(db, lg, bases, stream, kinds=(0, 1, 2), reps=max(1, args.reps - 1))
+        for prob, (cfg, ms) in best.items():
+            picks[prob] = {"exhaustive": cfg, "exhaustive_ms": ms}
+        print(f"[tune] msm 2^{lg}: {len(A.msm_candidates())} configs x 3 kinds, "
+              f"{time.time() - t0:.1f}s", flush=True)
+    db.save(os.path.join(args.out, "timings.json"))
+
+    # cost model: trained on everything for the picks used downstream, and
+    # leave-one-problem-out for the regret report
+    model = A.fit_model(db, stream, steps=args.steps)
+    for prob in db.problems():
+        choice = A.pick(model, db.for_problem(prob), stream)
+        picks[prob]["model"] = choice["config"]
+        picks[prob]["model_ms"] = choice["ms"]
+    with open(os.path.join(args.out, "picks.json"), "w") as f:
+        json.dump(picks, f, indent=1)
+    report = A.regret_report(db, stream, steps=args.steps, holdout=True)
+    regrets = [v["model_regret"] for v in report.values()]
+    dregrets = [v["default_regret"] for v in report.values() if v["default_regret"] is not None]
+    summary = {
+        "problems": len(report),
+        "rows": len(db.rows),
+        "model_regret_mean": sum(regrets) / len(regrets),
+        "model_regret_max": max(regrets),
+        "model_within_5pct": sum(1 for r in regrets if r <= 0.05) / len(regrets),
+        "default_regret_mean": sum(dregrets) / len(dregrets) if dregrets else None,
+        "default_regret_max": max(dregrets) if dregrets else None,
+        "seconds": time.time() - t0,
+    }
+    with open(os.path.join(args.out, "report.json"), "w") as f:
+        json.dump({"summary": summary, "per_problem": report}, f, indent=1)
+    print(json.dumps(summary), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -131,6 +131,12 @@ hipError_t fused_gate_abc_d_qe(const ConstPolyPtr *vars, const PolyPtr *mut_vars
                                void *local_buffer, hipStream_t stream);
 
 /* ======================= device-resident extensions ====================== */
+
+/* The same gate over contiguous device arrays, without the view round trip
+ * (kernel launch only, so it can be captured): inputs is a HOST array of six
+ * device pointers {a, b, c, d, q, e}. */
+hipError_t impeller_gate(const uint32_t *const *inputs, uint32_t *out, unsigned long long len,
+                         hipStream_t stream);
 /* No reference analogue: entry points for callers that keep everything in
  * HBM (the batched / graph-captured proving path and the benchmarks). */
 
@@ -208,7 +214,9 @@ hipError_t impeller_msm(const uint32_t *bases_d, const uint32_t *scalars_d,
  * window) * P with n_windows = min(target_windows, floor(254/window) + 1)
  * (target_windows = 0: one bucket set per window, a single table); the
  * reference's precompute layout (core/src/msm/src/msm_impl.cuh:56-67).
- * impeller_msm_precompute fills tables 1.. from table 0 in place.  task is
+ * impeller_msm_precompute fills tables 1.. from table 0 in place;
+ * table_stride is the number of points between consecutive tables (0 = len,
+ * larger when a shorter MSM uses a prefix of a shared table set).  task is
  * the number of bucket entries one thread accumulates before the per-bucket
  * partial-sum tree takes over. */
 size_t impeller_msm_cfg_workspace_size(unsigned long long len, unsigned int window,
@@ -218,9 +226,41 @@ hipError_t impeller_msm_precompute(uint32_t *tables_d, unsigned long long len,
                                    unsigned int window, unsigned int target_windows,
                                    hipStream_t stream);
 hipError_t impeller_msm_cfg(const uint32_t *tables_d, const uint32_t *scalars_d,
-                            unsigned long long len, unsigned int window,
+                            unsigned long long len, unsigned long long table_stride,
+                            unsigned int window,
                             unsigned int target_windows, unsigned int task, uint32_t *result_d,
                             void *workspace, size_t workspace_bytes, hipStream_t stream);
+
+/* Batched MSM (no reference analogue): count independent MSMs, possibly of
+ * different sizes and configurations and sharing one table set (table_stride
+ * = points between consecutive tables of that set, 0 = len), run as one
+ * launch per pipeline stage over the concatenated work of all of them, so
+ * the latency-bound tail stages of all MSMs overlap.  prepare is
+ * synchronous and carves every MSM's workspace out of dev
+ * (impeller_msm_batch_bytes bytes); run only launches kernels and can be
+ * captured into a HIP graph.  Each result (affine) goes to item.result. */
+typedef struct ImpMsmItem {
+    const uint32_t *tables;
+    const uint32_t *scalars;
+    unsigned long long len;
+    unsigned long long table_stride;
+    unsigned int window, target_windows, task;
+    uint32_t *result;
+} ImpMsmItem;
+
+typedef struct ImpMsmBatch {
+    void *dev;
+    unsigned int count, nstages;
+    unsigned int kind[64];
+    unsigned int arg[64];
+    unsigned int total[64];
+} ImpMsmBatch;
+
+size_t impeller_msm_batch_bytes(const ImpMsmItem *items, unsigned int count);
+hipError_t impeller_msm_batch_prepare(const ImpMsmItem *items, unsigned int count, void *dev,
+                                      size_t bytes, ImpMsmBatch *batch, hipStream_t stream);
+hipError_t impeller_msm_batch_run(const ImpMsmBatch *batch, unsigned int max_blocks,
+                                  hipStream_t stream);
 
 /* Seeded synthetic inputs on the device (the CPU checker has the same
  * definition): kind 0 uniform Fr, 1 selector-like (90% zero, else one),

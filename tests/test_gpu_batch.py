@@ -101,3 +101,68 @@ def test_graph_replay_matches_eager():
             stream.sync()
             for it, e in zip(items, expects):
                 assert (to_host(it["data"]) == e).all(), (rep, it["log_n"])
+
+
+def test_batched_msm_uneven_shared_tables(stream):
+    from impeller_amd import _lib
+    from impeller_amd import msm as M
+    # one 2^14 base set with precomputed tables for c = 13 (target 1), shared
+    # by MSMs of different lengths through the table stride
+    nb = 1 << 14
+    bases = device_u32(nb * 16)
+    _lib.call("impeller_gen_bases", bases.data_ptr(), nb, 91, 0, stream.raw())
+    k = O.gen_scalars(nb, 91)
+    tabs = {}
+    for c, target in ((13, 1), (11, 3)):
+        t = device_u32(M.n_tables(c, target) * nb * 16)
+        t[: nb * 16].copy_(bases)
+        M.precompute_device(t, nb, c, target, stream)
+        tabs[(c, target)] = t
+    items, expects = [], []
+    rng = random.Random(3)
+    for i in range(9):
+        n = rng.choice([1, 37, 1 << 10, 1 << 12, nb])
+        kind = i % 3
+        s = O.gen_scalars(n, 700 + i, kind=kind)
+        c, target = [(13, 1), (11, 3), (12, 0)][i % 3]
+        if target == 0:
+            items.append(dict(tables=bases, scalars=to_device(s), len=n, window=c, target=0,
+                              task=rng.choice([16, 64])))
+        else:
+            items.append(dict(tables=tabs[(c, target)], scalars=to_device(s), len=n, window=c,
+                              target=target, table_stride=nb, task=rng.choice([16, 64])))
+        expects.append(O.msm_closed_form(s, k[:n]))
+    b = M.MsmBatch(items, stream)
+    for max_blocks in (0, 64):
+        b.results.zero_()
+        b.run(stream, max_blocks)
+        stream.sync()
+        got = b.results_host()
+        for i, e in enumerate(expects):
+            assert (got[i] == e).all(), (i, items[i]["len"], max_blocks)
+
+
+def test_batched_msm_graph_replay():
+    import torch
+    from impeller_amd import _lib
+    from impeller_amd import msm as M
+    s_t = torch.cuda.Stream()
+    with torch.cuda.stream(s_t):
+        stream = Stream.current(0)
+        n = 1 << 11
+        bases = device_u32(n * 16)
+        _lib.call("impeller_gen_bases", bases.data_ptr(), n, 5, 0, stream.raw())
+        k = O.gen_scalars(n, 5)
+        ss = [O.gen_scalars(n, 40 + i) for i in range(3)]
+        items = [dict(tables=bases, scalars=to_device(s), len=n, window=w, target=0)
+                 for s, w in zip(ss, (10, 12, 14))]
+        b = M.MsmBatch(items, stream)
+        g = N.Graph(stream, lambda: b.run(stream))
+        for rep in range(3):
+            b.results.zero_()
+            torch.cuda.synchronize()
+            g.launch(stream)
+            stream.sync()
+            got = b.results_host()
+            for i, s in enumerate(ss):
+                assert (got[i] == O.msm_closed_form(s, k)).all(), (rep, i)
