@@ -254,7 +254,7 @@ def main():
     if args.chunk:
         pc = WL.ProvingChunk(stream)
         cfg = (16, 0, 64)
-        wsc = device_u32((M.cfg_workspace_size(1 << 22, cfg[0], cfg[2] and 0, 64) + 3) // 4)
+        wsc = device_u32((M.cfg_workspace_size(1 << pc.k, cfg[0], cfg[1], cfg[2]) + 3) // 4)
         t_plain = timed(lambda: pc.run_plain(cfg, ws=wsc), 1)
         plain_commit = pc.results.clone()
         nbs = pc.batches()

@@ -40,12 +40,49 @@ def _compile(src):
     return obj
 
 
+READELF = os.path.join(os.path.dirname(os.path.realpath(HIPCC)), "..", "llvm", "bin",
+                       "llvm-readelf")
+if not os.path.exists(READELF):
+    READELF = shutil.which("llvm-readelf") or "/opt/rocm/llvm/bin/llvm-readelf"
+
+
+def _device_functions(src):
+    """Out-of-line device functions (FUNC symbols without a kernel descriptor)
+    in the gfx950 code object of one source.  There must be none: in a callee
+    the compiler's long-branch expansion has been seen to reuse the return
+    address registers s[30:31] without saving them, so a wave that takes such
+    a branch never returns.  Every device function here is force-inlined and
+    this check keeps it that way."""
+    obj = os.path.join(OBJDIR, os.path.basename(src) + ".dev.o")
+    cmd = [HIPCC, *COMMON, "--cuda-device-only", "--no-gpu-bundle-output", "-c", src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc (device object) failed for {src}:\n{r.stderr}")
+    r = subprocess.run([READELF, "-s", "--wide", obj], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"llvm-readelf failed on {obj}:\n{r.stderr}")
+    funcs, kds = set(), set()
+    for line in r.stdout.splitlines():
+        f = line.split()
+        if len(f) >= 8 and f[3] == "FUNC":
+            funcs.add(f[7])
+        elif len(f) >= 8 and f[3] == "OBJECT" and f[7].endswith(".kd"):
+            kds.add(f[7][:-3])
+    return sorted(funcs - kds)
+
+
 def build(verbose=True):
     os.makedirs(OBJDIR, exist_ok=True)
     srcs = sources()
-    workers = min(len(srcs), int(os.environ.get("MAX_JOBS", "8")))
+    workers = min(2 * len(srcs), int(os.environ.get("MAX_JOBS", "8")))
     with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs_f = [ex.submit(_compile, s) for s in srcs]
+        calls_f = [ex.submit(_device_functions, s) for s in srcs]
+        objs = [f.result() for f in objs_f]
+        calls = {s: f.result() for s, f in zip(srcs, calls_f)}
+    bad = {os.path.basename(s): c for s, c in calls.items() if c}
+    if bad:
+        raise RuntimeError(f"out-of-line device functions (force-inline them): {bad}")
     tmp = OUT + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)

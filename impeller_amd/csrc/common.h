@@ -57,6 +57,13 @@ inline bool make_view(const PolyPtr& p, View* v) {
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// The launch's dynamic LDS as a u32 pointer (one definition per kernel).
+#ifndef IMP_DYNAMIC_LDS
+#define IMP_DYNAMIC_LDS(name)            \
+  extern __shared__ uint4 name##_lds4[]; \
+  u32* name = reinterpret_cast<u32*>(name##_lds4)
+#endif
+
 __host__ __device__ __forceinline__ u64 bit_reverse(u64 x, u32 bits) {
   u64 r = 0;
   for (u32 i = 0; i < bits; i++) {

@@ -213,12 +213,17 @@ __device__ __forceinline__ Fp<P> mul_lazy_cios(const Fp<P>& a, const Fp<P>& b) {
   return r;
 }
 
+#ifdef IMP_MONT_FIPS
 // (acc, hi) += x * y with a 96-bit accumulator: one v_mad_u64_u32 whose
 // carry-out feeds one v_addc_co_u32.  The 64-bit accumulator is the mad's own
-// addend register pair, so no zero-extension moves are needed.
+// addend register pair, so no zero-extension moves are needed.  gfx950 needs
+// two wait states between a VALU write of an SGPR (the carry-out) and a VALU
+// read of it (the carry-in); the compiler's hazard recognizer does not look
+// inside an asm block, so the s_nop is written here.
 __device__ __forceinline__ void mac96(u64& acc, u32& hi, u32 x, u32 y) {
   u64 c;
   asm("v_mad_u64_u32 %0, %1, %3, %4, %0\n\t"
+      "s_nop 1\n\t"
       "v_addc_co_u32 %2, %1, %2, 0, %1"
       : "+v"(acc), "=&s"(c), "+v"(hi)
       : "v"(x), "v"(y));
@@ -227,6 +232,7 @@ __device__ __forceinline__ void mac96(u64& acc, u32& hi, u32 x, u32 y) {
 __device__ __forceinline__ void mac96s(u64& acc, u32& hi, u32 x, u32 y) {
   u64 c;
   asm("v_mad_u64_u32 %0, %1, %3, %4, %0\n\t"
+      "s_nop 1\n\t"
       "v_addc_co_u32 %2, %1, %2, 0, %1"
       : "+v"(acc), "=&s"(c), "+v"(hi)
       : "v"(x), "s"(y));
@@ -269,12 +275,18 @@ __device__ __forceinline__ Fp<P> mul_lazy_fips(const Fp<P>& a, const Fp<P>& b) {
   return r;
 }
 
+#endif  // IMP_MONT_FIPS
+
+// The library is built with the compiler-scheduled CIOS form.  The inline-asm
+// product-scanning form (IMP_MONT_FIPS) is kept for measurement only: the
+// compiler's hazard recognizer cannot see inside asm blocks, so every wait
+// state in it must be written by hand.
 template <class P>
 __device__ __forceinline__ Fp<P> mul_lazy(const Fp<P>& a, const Fp<P>& b) {
-#ifdef IMP_MONT_CIOS
-  return mul_lazy_cios<P>(a, b);
-#else
+#ifdef IMP_MONT_FIPS
   return mul_lazy_fips<P>(a, b);
+#else
+  return mul_lazy_cios<P>(a, b);
 #endif
 }
 

@@ -154,7 +154,7 @@ __device__ __forceinline__ const u32* final_parts(const MsmJob& J) {
 // positive, so the top window never carries out).  Logical window w goes to
 // physical window w % n_windows and table w / n_windows (msm_impl.cuh:56-67).
 // key = bucket |digit|, val = point index (table * tstride + i) | sign << 31.
-__device__ void digits_body(const MsmJob& J, u64 i) {
+__device__ __forceinline__ void digits_body(const MsmJob& J, u64 i) {
   const MsmShape& s = J.s;
   if (i >= s.N) return;
   Fr v = to_number<FrParams>(load_fe<Fr>(J.scalars + 8 * i));
@@ -187,7 +187,7 @@ __device__ void digits_body(const MsmJob& J, u64 i) {
 // LSD radix sort, 4-bit digits, stable per pass: 16 consecutive keys per
 // thread (blocked), block ranks in (digit, thread) order, global bases from
 // the scanned per-tile histograms hist[(pw * 16 + digit) * tiles + tile].
-__device__ void hist_body(const MsmJob& J, u32 pass, u64 lb, u32* cnt) {
+__device__ __forceinline__ void hist_body(const MsmJob& J, u32 pass, u64 lb, u32* cnt) {
   const MsmShape& s = J.s;
   const u32 tile = (u32)(lb % s.tiles), pw = (u32)(lb / s.tiles), t = threadIdx.x;
   const u32 shift = 4 * pass;
@@ -207,7 +207,7 @@ __device__ void hist_body(const MsmJob& J, u32 pass, u64 lb, u32* cnt) {
 
 // block-wide exclusive scan of len entries (in place allowed); with
 // write_total, dst[len] = total
-__device__ void scan_body(const u32* src, u32* dst, u64 len, bool write_total, u32* part) {
+__device__ __forceinline__ void scan_body(const u32* src, u32* dst, u64 len, bool write_total, u32* part) {
   const u32 T = blockDim.x, t = threadIdx.x;
   const u64 per = (len + T - 1) / T;
   u64 lo = (u64)t * per, hi = lo + per < len ? lo + per : len;
@@ -238,7 +238,7 @@ struct ScatterLds {
   u32 dbase[16];
 };
 
-__device__ void scatter_body(const MsmJob& J, u32 pass, u64 lb, ScatterLds& L) {
+__device__ __forceinline__ void scatter_body(const MsmJob& J, u32 pass, u64 lb, ScatterLds& L) {
   const MsmShape& s = J.s;
   const u32 tile = (u32)(lb % s.tiles), pw = (u32)(lb / s.tiles), t = threadIdx.x;
   const u32 shift = 4 * pass;
@@ -305,14 +305,14 @@ __device__ void scatter_body(const MsmJob& J, u32 pass, u64 lb, ScatterLds& L) {
   }
 }
 
-__device__ void zero_body(const MsmJob& J, u64 i) {
+__device__ __forceinline__ void zero_body(const MsmJob& J, u64 i) {
   u64 nbw = (u64)J.s.n_windows * (J.s.nb + 1);
   if (i < nbw) J.bstart[i] = 0;
   else if (i < 2 * nbw) J.bend[i - nbw] = 0;
 }
 
 // bucket run bounds in the sorted keys (bucket 0 = zero digit, skipped)
-__device__ void bounds_body(const MsmJob& J, u64 gi) {
+__device__ __forceinline__ void bounds_body(const MsmJob& J, u64 gi) {
   const MsmShape& s = J.s;
   if (gi >= (u64)s.n_windows * s.seg_len) return;
   u32 pw = (u32)(gi / s.seg_len);
@@ -324,7 +324,7 @@ __device__ void bounds_body(const MsmJob& J, u64 gi) {
   if (i + 1 == s.seg_len || kk[i + 1] != k) J.bend[(u64)pw * (s.nb + 1) + k] = (u32)(i + 1);
 }
 
-__device__ void tcount_body(const MsmJob& J, u64 gi) {
+__device__ __forceinline__ void tcount_body(const MsmJob& J, u64 gi) {
   const MsmShape& s = J.s;
   if (gi >= (u64)s.n_windows * (s.nb + 1)) return;
   u32 b = (u32)(gi % (s.nb + 1));
@@ -345,7 +345,7 @@ __device__ __forceinline__ u32 find_bucket(const u32* toff, u32 nb, u32 t) {
 
 // one task = up to T consecutive entries of one bucket, mixed XYZZ + affine
 // additions; the next point is loaded while the current one is added
-__device__ void tasks_body(const MsmJob& J, u64 gi) {
+__device__ __forceinline__ void tasks_body(const MsmJob& J, u64 gi) {
   const MsmShape& s = J.s;
   u32 pw = (u32)(gi / s.task_cap);
   if (pw >= s.n_windows) return;
@@ -376,7 +376,7 @@ __device__ void tasks_body(const MsmJob& J, u64 gi) {
 
 // level l of the per-bucket partial-sum tree: group k of bucket b sums the
 // live entries src[toff_b + kG .. + G) into dst[toff_b + k]
-__device__ void combine_body(const MsmJob& J, u32 level, u64 gi) {
+__device__ __forceinline__ void combine_body(const MsmJob& J, u32 level, u64 gi) {
   const MsmShape& s = J.s;
   if (level > s.levels) return;
   u32 pw = (u32)(gi / s.task_cap);
@@ -403,7 +403,7 @@ __device__ void combine_body(const MsmJob& J, u32 level, u64 gi) {
 
 // segment [lo, lo + seg) of buckets (bucket number b = lo + k + 1):
 // partial = sum_k (lo + k + 1) * B = T + lo * R with running sums
-__device__ void reduce_body(const MsmJob& J, u64 t) {
+__device__ __forceinline__ void reduce_body(const MsmJob& J, u64 t) {
   const MsmShape& s = J.s;
   if (t >= (u64)s.n_windows * s.nseg) return;
   u32 pw = (u32)(t / s.nseg), sg = (u32)(t % s.nseg);
@@ -421,7 +421,7 @@ __device__ void reduce_body(const MsmJob& J, u64 t) {
 }
 
 // one block per physical window: sum of its segment partials
-__device__ void sum_body(const MsmJob& J, u64 pw, u32* red) {
+__device__ __forceinline__ void sum_body(const MsmJob& J, u64 pw, u32* red) {
   const MsmShape& s = J.s;
   const u32 t = threadIdx.x;
   G1Xyzz acc = G1Xyzz::identity();
@@ -441,7 +441,7 @@ __device__ void sum_body(const MsmJob& J, u64 pw, u32* red) {
 }
 
 // Horner over physical windows, then affine (thread 0 of the job's block)
-__device__ void final_body(const MsmJob& J) {
+__device__ __forceinline__ void final_body(const MsmJob& J) {
   if (threadIdx.x != 0) return;
   const MsmShape& s = J.s;
   if (s.N == 0) {

@@ -143,7 +143,7 @@ __device__ __forceinline__ Fr src_load(const NttJob& J, u64 q) {
 // work-group owns both tiles, so the permutation can run in place.  In LDS
 // the destination tiles are laid out by position [a * 32 + c], whose low 5
 // bits c are the bits DIT levels 0..4 pair up (exponent (c mod 2^j) 2^(L-j-1)).
-__device__ void br_tile_body(const NttJob& J, u64 b, u32* lds) {
+__device__ __forceinline__ void br_tile_body(const NttJob& J, u64 b, u32* lds) {
   const u32 L = J.L;
   u32* d0 = lds;              // positions of tile b
   u32* d1 = lds + 1024 * 8;   // positions of tile rev(b)
@@ -194,7 +194,7 @@ __device__ void br_tile_body(const NttJob& J, u64 b, u32* lds) {
 }
 
 // whole array (L < 10) in one work-group
-__device__ void br_small_body(const NttJob& J, u32* lds) {
+__device__ __forceinline__ void br_small_body(const NttJob& J, u32* lds) {
   const u32 n = 1u << J.L;
   __syncthreads();
   for (u32 i = threadIdx.x; i < n; i += blockDim.x)
@@ -213,7 +213,7 @@ __device__ __forceinline__ void br_body(const NttJob& J, u64 b, u32* lds) {
 // contiguous (s == 0): work-group = C consecutive groups of G positions.
 // strided (2^s >= C): work-group = fixed hi, C consecutive lo.  The last pass
 // multiplies each output by the epilogue factor (n^-1, coset un-shift).
-__device__ void dit_body(const NttJob& J, const NttStage& S, bool last, u64 wg, u32* t) {
+__device__ __forceinline__ void dit_body(const NttJob& J, const NttStage& S, bool last, u64 wg, u32* t) {
   const u32 L = J.L, s = S.s, D = S.D, logC = S.logC;
   const u32 C = 1u << logC, G = 1u << D, GC = G * C;
   const bool contig = (s == 0), two = J.two != 0;
@@ -289,13 +289,13 @@ __device__ void dit_body(const NttJob& J, const NttStage& S, bool last, u64 wg, 
 
 // --------------------------------------------------------------- kernels
 __global__ __launch_bounds__(256) void k_ntt_br(NttJob J) {
-  extern __shared__ uint4 lds4[];
-  br_body(J, blockIdx.x, reinterpret_cast<u32*>(lds4));
+  IMP_DYNAMIC_LDS(lds);
+  br_body(J, blockIdx.x, lds);
 }
 
 __global__ __launch_bounds__(256) void k_ntt_dit(NttJob J, u32 stage) {
-  extern __shared__ uint4 lds4[];
-  dit_body(J, J.st[stage], stage + 1 == J.nst, blockIdx.x, reinterpret_cast<u32*>(lds4));
+  IMP_DYNAMIC_LDS(lds);
+  dit_body(J, J.st[stage], stage + 1 == J.nst, blockIdx.x, lds);
 }
 
 __device__ __forceinline__ u32 find_job(const u32* prefix, u32 njobs, u32 t) {
@@ -313,22 +313,22 @@ __device__ __forceinline__ u32 find_job(const u32* prefix, u32 njobs, u32 t) {
 // the stage contribute zero blocks.
 __global__ __launch_bounds__(256) void k_ntt_br_batch(const NttJob* jobs, u32 njobs,
                                                       const u32* prefix) {
-  extern __shared__ uint4 lds4[];
+  IMP_DYNAMIC_LDS(lds);
   const u32 total = prefix[njobs];
   for (u32 t = blockIdx.x; t < total; t += gridDim.x) {
     u32 j = find_job(prefix, njobs, t);
-    br_body(jobs[j], t - prefix[j], reinterpret_cast<u32*>(lds4));
+    br_body(jobs[j], t - prefix[j], lds);
   }
 }
 
 __global__ __launch_bounds__(256) void k_ntt_dit_batch(const NttJob* jobs, u32 njobs,
                                                        const u32* prefix, u32 stage) {
-  extern __shared__ uint4 lds4[];
+  IMP_DYNAMIC_LDS(lds);
   const u32 total = prefix[njobs];
   for (u32 t = blockIdx.x; t < total; t += gridDim.x) {
     u32 j = find_job(prefix, njobs, t);
     const NttJob& J = jobs[j];
-    dit_body(J, J.st[stage], stage + 1 == J.nst, t - prefix[j], reinterpret_cast<u32*>(lds4));
+    dit_body(J, J.st[stage], stage + 1 == J.nst, t - prefix[j], lds);
   }
 }
 

@@ -1,0 +1,282 @@
+// CPU emulation of HIP kernel launches (see hip/hip_runtime.h).  One
+// work-group runs at a time; its work-items are ucontext fibers that yield at
+// __syncthreads, and the group's barrier releases once every work-item that
+// has not returned is waiting at it.  Dynamic LDS is a fresh heap block per
+// work-group (so AddressSanitizer sees overruns); static __shared__ arrays are
+// function-local statics.  Stream capture records launches and copies into a
+// replayable list.  Test infrastructure only.
+#include <stdlib.h>
+#include <ucontext.h>
+
+#include <vector>
+
+#include "hip/hip_runtime.h"
+
+#if defined(__has_feature)
+#if __has_feature(address_sanitizer)
+#define EMU_ASAN 1
+#include <sanitizer/common_interface_defs.h>
+#endif
+#endif
+
+namespace emu {
+
+dim3 threadIdx_, blockIdx_, blockDim_, gridDim_;
+
+namespace {
+const size_t kStack = 512 * 1024;
+
+struct Fiber {
+  ucontext_t ctx;
+  char* stack = nullptr;
+  int state = 0;  // 0 runnable, 1 waiting at the barrier, 2 returned
+#ifdef EMU_ASAN
+  void* fake = nullptr;
+#endif
+};
+
+std::vector<Fiber> g_fibers;
+ucontext_t g_sched;
+int g_cur = -1;
+const std::function<void()>* g_body = nullptr;
+char* g_lds = nullptr;
+hipError_t g_last = hipSuccess;
+bool g_capturing = false;
+std::vector<std::function<void()>>* g_capture = nullptr;
+#ifdef EMU_ASAN
+const void* g_sched_bottom = nullptr;
+size_t g_sched_size = 0;
+void* g_sched_fake = nullptr;
+#endif
+
+// switch from the current fiber back to the scheduler
+void to_sched(bool dying) {
+  Fiber& f = g_fibers[g_cur];
+#ifdef EMU_ASAN
+  __sanitizer_start_switch_fiber(dying ? nullptr : &f.fake, g_sched_bottom, g_sched_size);
+#else
+  (void)dying;
+#endif
+  swapcontext(&f.ctx, &g_sched);
+#ifdef EMU_ASAN
+  __sanitizer_finish_switch_fiber(f.fake, &g_sched_bottom, &g_sched_size);
+#endif
+}
+
+void trampoline() {
+#ifdef EMU_ASAN
+  __sanitizer_finish_switch_fiber(nullptr, &g_sched_bottom, &g_sched_size);
+#endif
+  (*g_body)();
+  g_fibers[g_cur].state = 2;
+  to_sched(true);
+}
+
+void run_group(unsigned nthreads) {
+  if (g_fibers.size() < nthreads) g_fibers.resize(nthreads);
+  for (unsigned t = 0; t < nthreads; t++) {
+    Fiber& f = g_fibers[t];
+    if (!f.stack) f.stack = (char*)malloc(kStack);
+    getcontext(&f.ctx);
+    f.ctx.uc_stack.ss_sp = f.stack;
+    f.ctx.uc_stack.ss_size = kStack;
+    f.ctx.uc_link = nullptr;
+    makecontext(&f.ctx, trampoline, 0);
+    f.state = 0;
+  }
+  for (;;) {
+    for (unsigned t = 0; t < nthreads; t++) {
+      Fiber& f = g_fibers[t];
+      if (f.state != 0) continue;
+      g_cur = (int)t;
+      unsigned tx = t % blockDim_.x, ty = (t / blockDim_.x) % blockDim_.y,
+               tz = t / (blockDim_.x * blockDim_.y);
+      threadIdx_ = dim3(tx, ty, tz);
+#ifdef EMU_ASAN
+      __sanitizer_start_switch_fiber(&g_sched_fake, f.stack, kStack);
+#endif
+      swapcontext(&g_sched, &f.ctx);
+#ifdef EMU_ASAN
+      __sanitizer_finish_switch_fiber(g_sched_fake, nullptr, nullptr);
+#endif
+    }
+    unsigned waiting = 0;
+    for (unsigned t = 0; t < nthreads; t++) waiting += g_fibers[t].state == 1;
+    if (waiting == 0) break;
+    for (unsigned t = 0; t < nthreads; t++)
+      if (g_fibers[t].state == 1) g_fibers[t].state = 0;
+  }
+  g_cur = -1;
+}
+
+void run_launch(dim3 grid, dim3 block, size_t shmem, const std::function<void()>& body) {
+  unsigned nthreads = block.x * block.y * block.z;
+  if (nthreads == 0 || nthreads > 1024 || grid.x == 0 || grid.y == 0 || grid.z == 0 ||
+      shmem > 160 * 1024) {
+    fprintf(stderr, "[emu] invalid launch: grid %u %u %u block %u %u %u lds %zu\n", grid.x,
+            grid.y, grid.z, block.x, block.y, block.z, shmem);
+    g_last = hipErrorInvalidValue;
+    return;
+  }
+  const std::function<void()>* prev = g_body;
+  g_body = &body;
+  gridDim_ = grid;
+  blockDim_ = block;
+  for (unsigned z = 0; z < grid.z; z++)
+    for (unsigned y = 0; y < grid.y; y++)
+      for (unsigned x = 0; x < grid.x; x++) {
+        blockIdx_ = dim3(x, y, z);
+        g_lds = shmem ? (char*)malloc(shmem) : nullptr;
+        if (g_lds) memset(g_lds, 0xA5, shmem);  // LDS starts undefined
+        run_group(nthreads);
+        free(g_lds);
+        g_lds = nullptr;
+      }
+  g_body = prev;
+}
+}  // namespace
+
+void syncthreads() {
+  g_fibers[g_cur].state = 1;
+  to_sched(false);
+}
+
+void* dyn_lds() { return g_lds; }
+
+void launch_impl(dim3 grid, dim3 block, size_t shmem, std::function<void()> body) {
+  if (g_capturing) {
+    g_capture->push_back([grid, block, shmem, body]() { run_launch(grid, block, shmem, body); });
+    return;
+  }
+  run_launch(grid, block, shmem, body);
+}
+
+bool capturing() { return g_capturing; }
+void record(std::function<void()> f) { g_capture->push_back(std::move(f)); }
+
+}  // namespace emu
+
+using namespace emu;
+
+struct ihipGraph_t {
+  std::vector<std::function<void()>> ops;
+};
+struct ihipGraphExec_t {
+  std::vector<std::function<void()>> ops;
+};
+struct ihipEvent_t {
+  int dummy;
+};
+
+hipError_t hipMalloc(void** p, size_t n) {
+  *p = malloc(n ? n : 1);
+  return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+hipError_t hipFree(void* p) {
+  free(p);
+  return hipSuccess;
+}
+hipError_t hipMallocAsync(void** p, size_t n, hipStream_t) {
+  if (g_capturing) return hipErrorNotSupported;
+  return hipMalloc(p, n);
+}
+hipError_t hipFreeAsync(void* p, hipStream_t) {
+  if (g_capturing) return hipErrorNotSupported;
+  return hipFree(p);
+}
+hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {
+  if (g_capturing) return hipErrorNotSupported;
+  memmove(d, s, n);
+  return hipSuccess;
+}
+hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) {
+  if (g_capturing) {
+    g_capture->push_back([d, s, n]() { memmove(d, s, n); });
+    return hipSuccess;
+  }
+  memmove(d, s, n);
+  return hipSuccess;
+}
+hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) {
+  // a graph must never hold a memset node (clear with a kernel instead)
+  if (g_capturing) return hipErrorNotSupported;
+  memset(d, v, n);
+  return hipSuccess;
+}
+hipError_t hipStreamSynchronize(hipStream_t) {
+  return g_capturing ? hipErrorNotSupported : hipSuccess;
+}
+hipError_t hipDeviceSynchronize() { return hipSuccess; }
+hipError_t hipGetLastError() {
+  hipError_t e = g_last;
+  g_last = hipSuccess;
+  return e;
+}
+const char* hipGetErrorString(hipError_t e) {
+  switch (e) {
+    case hipSuccess: return "hipSuccess";
+    case hipErrorInvalidValue: return "hipErrorInvalidValue";
+    case hipErrorOutOfMemory: return "hipErrorOutOfMemory";
+    case hipErrorNotSupported: return "hipErrorNotSupported";
+    default: return "hipError (emulated)";
+  }
+}
+hipError_t hipGetDevice(int* d) {
+  *d = 0;
+  return hipSuccess;
+}
+hipError_t hipSetDevice(int) { return hipSuccess; }
+hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) {
+  *v = 8;  // a small "chip": persistent grids stride more often
+  return hipSuccess;
+}
+hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
+  static int token;
+  *s = (hipStream_t)&token;
+  return hipSuccess;
+}
+hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+hipError_t hipEventCreate(hipEvent_t* e) {
+  *e = new ihipEvent_t{0};
+  return hipSuccess;
+}
+hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
+  *ms = 0.f;
+  return hipSuccess;
+}
+hipError_t hipEventDestroy(hipEvent_t e) {
+  delete e;
+  return hipSuccess;
+}
+hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+  if (g_capturing) return hipErrorInvalidValue;
+  g_capturing = true;
+  g_capture = new std::vector<std::function<void()>>();
+  return hipSuccess;
+}
+hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) {
+  if (!g_capturing) return hipErrorInvalidValue;
+  g_capturing = false;
+  *g = new ihipGraph_t{std::move(*g_capture)};
+  delete g_capture;
+  g_capture = nullptr;
+  return hipSuccess;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, hipGraphNode_t*, char*, size_t) {
+  *e = new ihipGraphExec_t{g->ops};
+  return hipSuccess;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+  for (auto& op : e->ops) op();
+  return hipSuccess;
+}
+hipError_t hipGraphExecDestroy(hipGraphExec_t e) {
+  delete e;
+  return hipSuccess;
+}
+hipError_t hipGraphDestroy(hipGraph_t g) {
+  delete g;
+  return hipSuccess;
+}

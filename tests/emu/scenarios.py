@@ -1,0 +1,296 @@
+"""Kernel scenarios on the CPU-emulated library, checked against the oracle.
+The same shapes and entry points as the GPU tests (batched NTT and MSM, graph
+replay, single-call NTT/MSM, gate), at sizes the emulation finishes quickly.
+
+  python tests/emu/scenarios.py [--asan] [name ...]
+"""
+import ctypes
+import random
+import sys
+import time
+
+import numpy as np
+
+from tests.emu.driver import Emu, garbage_u32, ptr, u32
+from impeller_amd import _lib
+from oracle import oracle as O
+
+r = O.R_MOD
+L = O.int_to_limbs
+ROOT = pow(7, (r - 1) >> 28, r)
+
+
+def mont(x):
+    return L(O.to_mont_int(x))
+
+
+def omega(k):
+    return pow(ROOT, 1 << (28 - k), r)
+
+
+def flat(a):
+    return np.ascontiguousarray(a, dtype=np.uint32).reshape(-1)
+
+
+def ntt_items(specs, seed):
+    """(items, expects); kinds fwd / inv / ext / fwd_random as in the GPU tests."""
+    rng = random.Random(seed)
+    items, expects = [], []
+    for i, (kind, k) in enumerate(specs):
+        n = 1 << k
+        if kind == "ext":
+            ks = k - 3
+            coeffs = O.gen_scalars(1 << ks, seed * 100 + i)
+            w, g = mont(omega(k)), mont(7)
+            ext = np.zeros((n, 8), np.uint32)
+            ext[: 1 << ks] = coeffs
+            expects.append(O.best_fft(O.distribute_powers(ext, O.powers(g, n)), w, k))
+            items.append(dict(data=garbage_u32(n * 8), src=flat(coeffs), log_n=k, log_src=ks,
+                              twiddle=flat(O.powers(w, n // 2)), coset_shift=flat(g)))
+        elif kind == "inv":
+            x = O.gen_scalars(n, seed * 100 + i)
+            winv = mont(pow(omega(k), -1, r))
+            expects.append(O.vec_scale(O.best_fft(x, winv, k), mont(pow(n, -1, r))))
+            items.append(dict(data=flat(x).copy(), log_n=k, inverse=True,
+                              twiddle=flat(O.powers(winv, max(1, n // 2)))))
+        else:
+            x = O.gen_scalars(n, seed * 100 + i)
+            w = mont(rng.randrange(r)) if kind == "fwd_random" else mont(omega(k))
+            expects.append(O.best_fft(x, w, k))
+            items.append(dict(data=flat(x).copy(), log_n=k,
+                              twiddle=flat(O.powers(w, max(1, n // 2)))))
+    return items, expects
+
+
+class NttBatch:
+    def __init__(self, E, items):
+        self.E = E
+        n = len(items)
+        arr = (_lib.ImpNttItem * n)()
+        for i, it in enumerate(items):
+            a = arr[i]
+            a.data = ptr(it["data"])
+            a.src = ptr(it.get("src"))
+            a.log_n = it["log_n"]
+            a.log_src = it.get("log_src", it["log_n"])
+            a.twiddle = ptr(it["twiddle"])
+            a.inverse = 1 if it.get("inverse") else 0
+            a.coset_shift = ptr(it.get("coset_shift"))
+            plan = it.get("plan") or []
+            a.plan_len = len(plan)
+            for k, v in enumerate(plan):
+                a.plan[k] = v
+        nbytes = int(E.call("impeller_ntt_batch_bytes", n))
+        self.dev = garbage_u32((nbytes + 3) // 4)
+        self.info = _lib.ImpNttBatch()
+        E.call("impeller_ntt_batch_prepare", arr, n, ptr(self.dev), nbytes,
+               ctypes.byref(self.info), None)
+
+    def run(self, max_blocks=0):
+        self.E.call("impeller_ntt_batch_run", ctypes.byref(self.info), max_blocks, None)
+
+
+class Graph:
+    def __init__(self, E, fn):
+        self.E = E
+        E.call("impeller_graph_begin", None)
+        fn()
+        h = ctypes.c_void_p()
+        E.call("impeller_graph_end", None, ctypes.byref(h))
+        self.h = h
+
+    def launch(self):
+        self.E.call("impeller_graph_launch", self.h, None)
+
+
+def check_items(items, expects, tag):
+    for it, e in zip(items, expects):
+        got = it["data"].reshape(-1, 8)
+        assert (got == e).all(), f"{tag}: log_n {it['log_n']}"
+
+
+SPECS = [("fwd", 12), ("inv", 14), ("ext", 13), ("fwd_random", 10), ("fwd", 3), ("inv", 9),
+         ("ext", 15), ("fwd_random", 16), ("fwd", 1)]
+
+
+def batched_ntt_mixed(E):
+    for max_blocks in (0, 7, 256):
+        items, expects = ntt_items(SPECS, 11 + max_blocks)
+        b = NttBatch(E, items)
+        b.run(max_blocks)
+        check_items(items, expects, f"mixed max_blocks={max_blocks}")
+
+
+def batched_ntt_plans(E):
+    items, expects = ntt_items([("fwd_random", 14), ("inv", 12), ("ext", 14)], 5)
+    items[0]["plan"] = [5, 11, 9]
+    items[1]["plan"] = [0, 10, 2, 3, 7]
+    items[2]["plan"] = [5, 10, 4, 5]
+    NttBatch(E, items).run()
+    check_items(items, expects, "plans")
+
+
+def ntt_graph_replay(E):
+    items, expects = ntt_items([("fwd", 12), ("ext", 13), ("inv", 11)], 9)
+    restore = [(it["data"], it["data"].copy()) for it in items if it.get("src") is None]
+    b = NttBatch(E, items)
+    g = Graph(E, lambda: b.run())
+    for rep in range(3):
+        for dst, src in restore:
+            dst[:] = src
+        g.launch()
+        check_items(items, expects, f"graph rep {rep}")
+
+
+def single_ntt(E):
+    for k in (1, 5, 10, 13, 16):
+        n = 1 << k
+        x = O.gen_scalars(n, 300 + k)
+        w = mont(omega(k))
+        tw = flat(O.powers(w, max(1, n // 2)))
+        for inverse, shift in ((0, None), (0, flat(mont(7))), (1, None), (1, flat(mont(5)))):
+            d = flat(x).copy()
+            E.call("impeller_ntt", ptr(d), ptr(tw), k, inverse, ptr(shift), None, 0, None)
+            if inverse:
+                e = O.best_fft(x, w, k)
+                e = O.vec_scale(e, mont(pow(n, -1, r)))
+                if shift is not None:
+                    e = O.vec_mul(e, O.powers(mont(5), n))
+            else:
+                xin = x if shift is None else O.vec_mul(x, O.powers(mont(7), n))
+                e = O.best_fft(xin, w, k)
+            assert (d.reshape(-1, 8) == e).all(), (k, inverse, shift is not None)
+
+
+def msm_items(E, nb, seed=91):
+    bases = u32(nb * 16)
+    E.call("impeller_gen_bases", ptr(bases), nb, seed, 0, None)
+    k = O.gen_scalars(nb, seed)
+    tabs = {}
+    for c, target in ((13, 1), (11, 3)):
+        nt = int(E.call("impeller_msm_tables", c, target))
+        t = u32(nt * nb * 16)
+        t[: nb * 16] = bases
+        E.call("impeller_msm_precompute", ptr(t), nb, c, target, None)
+        tabs[(c, target)] = t
+    return bases, k, tabs
+
+
+class MsmBatch:
+    def __init__(self, E, items):
+        self.E = E
+        n = len(items)
+        self.results = u32(16 * n)
+        arr = (_lib.ImpMsmItem * n)()
+        for i, it in enumerate(items):
+            a = arr[i]
+            a.tables = ptr(it["tables"])
+            a.scalars = ptr(it["scalars"])
+            a.len = it["len"]
+            a.table_stride = it.get("table_stride", 0)
+            a.window = it["window"]
+            a.target_windows = it.get("target", 0)
+            a.task = it.get("task", 64)
+            a.result = self.results.ctypes.data + 64 * i
+        nbytes = int(E.call("impeller_msm_batch_bytes", arr, n))
+        assert nbytes > 0
+        self.dev = garbage_u32((nbytes + 3) // 4)
+        self.info = _lib.ImpMsmBatch()
+        E.call("impeller_msm_batch_prepare", arr, n, ptr(self.dev), nbytes,
+               ctypes.byref(self.info), None)
+
+    def run(self, max_blocks=0):
+        self.E.call("impeller_msm_batch_run", ctypes.byref(self.info), max_blocks, None)
+
+
+def batched_msm_shared_tables(E, nb=1 << 12):
+    bases, k, tabs = msm_items(E, nb)
+    items, expects = [], []
+    rng = random.Random(3)
+    for i in range(9):
+        n = rng.choice([1, 37, 1 << 8, 1 << 10, nb])
+        kind = i % 3
+        s = O.gen_scalars(n, 700 + i, kind=kind)
+        c, target = [(13, 1), (11, 3), (12, 0)][i % 3]
+        t = bases if target == 0 else tabs[(c, target)]
+        items.append(dict(tables=t, scalars=flat(s), len=n, window=c, target=target,
+                          table_stride=nb if target else 0, task=rng.choice([16, 64])))
+        expects.append(O.msm_closed_form(s, k[:n]))
+    b = MsmBatch(E, items)
+    for max_blocks in (0, 64):
+        b.results[:] = 0
+        b.run(max_blocks)
+        got = b.results.reshape(-1, 16)
+        for i, e in enumerate(expects):
+            assert (got[i] == e).all(), (i, items[i]["len"], max_blocks)
+
+
+def msm_graph_replay(E):
+    n = 1 << 9
+    bases = u32(n * 16)
+    E.call("impeller_gen_bases", ptr(bases), n, 5, 0, None)
+    k = O.gen_scalars(n, 5)
+    ss = [O.gen_scalars(n, 40 + i) for i in range(3)]
+    items = [dict(tables=bases, scalars=flat(s), len=n, window=w, target=0)
+             for s, w in zip(ss, (10, 12, 14))]
+    b = MsmBatch(E, items)
+    g = Graph(E, lambda: b.run())
+    for rep in range(2):
+        b.results[:] = 0
+        g.launch()
+        got = b.results.reshape(-1, 16)
+        for i, s in enumerate(ss):
+            assert (got[i] == O.msm_closed_form(s, k)).all(), (rep, i)
+
+
+def single_msm(E):
+    for n, c, target, task, kind in ((1, 8, 0, 64, 0), (7, 13, 0, 64, 0), (1000, 10, 1, 16, 1),
+                                     (4096, 16, 0, 64, 2), (4096, 12, 2, 32, 0)):
+        bases = u32(n * 16)
+        E.call("impeller_gen_bases", ptr(bases), n, 77, 0, None)
+        nt = int(E.call("impeller_msm_tables", c, target))
+        tables = u32(nt * n * 16)
+        tables[: n * 16] = bases
+        E.call("impeller_msm_precompute", ptr(tables), n, c, target, None)
+        s = O.gen_scalars(n, 12 + n, kind=kind)
+        ws_bytes = int(E.call("impeller_msm_cfg_workspace_size", n, c, target, task))
+        ws = garbage_u32((ws_bytes + 3) // 4)
+        out = garbage_u32(16)
+        E.call("impeller_msm_cfg", ptr(tables), ptr(flat(s)), n, 0, c, target, task, ptr(out),
+               ptr(ws), ws_bytes, None)
+        assert (out == O.msm_closed_form(s, O.gen_scalars(n, 77))).all(), (n, c, target, task)
+
+
+def gate(E):
+    for n in (1, 777, 4096):
+        ins = [O.gen_scalars(n, 40 + i) for i in range(6)]
+        arrs = [flat(a) for a in ins]
+        ptrs = (ctypes.c_void_p * 6)(*[a.ctypes.data for a in arrs])
+        out = u32(n * 8)
+        E.call("impeller_gate", ptrs, ptr(out), n, None)
+        assert (out.reshape(-1, 8) == O.gate(*ins)).all(), n
+
+
+ALL = [("single_ntt", single_ntt), ("batched_ntt_mixed", batched_ntt_mixed),
+       ("batched_ntt_plans", batched_ntt_plans), ("ntt_graph_replay", ntt_graph_replay),
+       ("gate", gate), ("single_msm", single_msm), ("msm_graph_replay", msm_graph_replay),
+       ("batched_msm_shared_tables", batched_msm_shared_tables)]
+
+
+def main(argv):
+    asan = "--asan" in argv
+    names = [a for a in argv if not a.startswith("--")]
+    E = Emu(asan)
+    for name, fn in ALL:
+        if names and name not in names:
+            continue
+        t = time.time()
+        print(f"[emu] {name} ...", flush=True)
+        fn(E)
+        print(f"[emu] {name} ok ({time.time() - t:.1f}s)", flush=True)
+
+
+if __name__ == "__main__":
+    import os
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))))
+    main(sys.argv[1:])
