@@ -204,11 +204,13 @@ def main():
             ntt_wl.restore()
             nb_.run(stream)
 
+        # the transforms run in place: both schedules restore their inputs
+        # first, and the restore time is measured on its own and subtracted
+        t_restore = timed(ntt_wl.restore, 2)
         t_plain_ntt = timed(plain_ntt, 2)
         plain_out = [it["data"].clone() for it in ntt_wl.items]
         g_ntt = N.Graph(stream, lambda: nb_.run(stream))
         t_batch_ntt = timed(lambda: (ntt_wl.restore(), g_ntt.launch(stream)), 2)
-        t_restore = timed(ntt_wl.restore, 2)
         batch_parity["ntt_batched_equals_plain"] = all(
             bool(torch.equal(a, it["data"])) for a, it in zip(plain_out, ntt_wl.items))
         del plain_out, g_ntt, nb_, ntt_wl

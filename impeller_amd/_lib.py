@@ -56,8 +56,8 @@ class ImpMsmItem(ctypes.Structure):
 
 class ImpMsmBatch(ctypes.Structure):
     _fields_ = [("dev", ctypes.c_void_p), ("count", ctypes.c_uint), ("nstages", ctypes.c_uint),
-                ("kind", ctypes.c_uint * 64), ("arg", ctypes.c_uint * 64),
-                ("total", ctypes.c_uint * 64)]
+                ("kind", ctypes.c_uint * 128), ("arg", ctypes.c_uint * 128),
+                ("total", ctypes.c_uint * 128)]
 
 
 _P = ctypes.c_void_p

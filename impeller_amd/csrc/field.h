@@ -213,7 +213,7 @@ __device__ __forceinline__ Fp<P> mul_lazy_cios(const Fp<P>& a, const Fp<P>& b) {
   return r;
 }
 
-#ifdef IMP_MONT_FIPS
+#ifndef IMP_EMU
 // (acc, hi) += x * y with a 96-bit accumulator: one v_mad_u64_u32 whose
 // carry-out feeds one v_addc_co_u32.  The 64-bit accumulator is the mad's own
 // addend register pair, so no zero-extension moves are needed.  gfx950 needs
@@ -275,18 +275,18 @@ __device__ __forceinline__ Fp<P> mul_lazy_fips(const Fp<P>& a, const Fp<P>& b) {
   return r;
 }
 
-#endif  // IMP_MONT_FIPS
+#endif  // !IMP_EMU
 
-// The library is built with the compiler-scheduled CIOS form.  The inline-asm
-// product-scanning form (IMP_MONT_FIPS) is kept for measurement only: the
-// compiler's hazard recognizer cannot see inside asm blocks, so every wait
-// state in it must be written by hand.
+// The library uses the inline-asm product-scanning form (1.43x the
+// compiler-scheduled CIOS form on gfx950, bit-identical).  The CIOS form is
+// kept for the host build of the kernels (IMP_EMU: the CPU launch emulation
+// used by the tests) and as a measurement reference (IMP_MONT_CIOS).
 template <class P>
 __device__ __forceinline__ Fp<P> mul_lazy(const Fp<P>& a, const Fp<P>& b) {
-#ifdef IMP_MONT_FIPS
-  return mul_lazy_fips<P>(a, b);
-#else
+#if defined(IMP_EMU) || defined(IMP_MONT_CIOS)
   return mul_lazy_cios<P>(a, b);
+#else
+  return mul_lazy_fips<P>(a, b);
 #endif
 }
 

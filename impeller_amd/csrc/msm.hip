@@ -15,10 +15,15 @@
 //   k_sort_*          stable LSD radix sort of the keys per physical window
 //                     (4-bit digits, per-tile histograms, block-local ranks)
 //   k_bucket_bounds   run boundaries of each bucket in the sorted keys
+//   kb_scan_a/b/c     device-wide segmented exclusive scans (chunk sums,
+//                     chunk offsets, apply) for the radix histograms, the
+//                     task offsets and the combine levels
 //   k_bucket_tasks    each bucket split into tasks of <= T entries, one
 //                     thread per task, mixed XYZZ + affine additions
-//   k_bucket_combine  per-bucket partial-sum tree (groups of 2^g) so that a
-//                     heavily loaded bucket never serialises on one thread
+//   k_bucket_combine  per-bucket partial-sum tree (groups of 2^g); each level
+//                     runs one thread per live group (compact indices from a
+//                     scan), so a heavily loaded bucket never serialises on one
+//                     thread and idle buckets cost no lanes
 //   k_msm_reduce      sum_b b * B_b via running sums over bucket segments
 //   k_msm_final       window partials -> Horner over windows -> affine
 #include <algorithm>
@@ -43,7 +48,8 @@ struct MsmShape {
   u32 passes;     // radix passes (4-bit digits over c-bit keys)
   u32 T;          // entries per accumulation task
   u32 g;          // log2 of the combine group size
-  u32 levels;     // combine levels
+  u32 levels;     // combine levels (worst case over bucket loads)
+  u32 chunks;     // scan chunks per window (max over the scans)
   u64 task_cap;   // task slots per physical window
   u32 seg;        // buckets per reduce segment
   u32 nseg;       // reduce segments per physical window
@@ -64,19 +70,21 @@ static MsmShape msm_shape(u64 N, u32 c, u32 target_windows, u32 task = 64) {
   if (s.tiles == 0) s.tiles = 1;
   s.passes = (c + 3) / 4;
   s.T = task;
-  s.g = 5;
+  s.g = 4;
   s.task_cap = s.seg_len / s.T + s.nb + 1;
   u64 max_tasks = s.seg_len / s.T + 1;
   s.levels = 0;
-  while ((1ull << (s.g * s.levels)) < max_tasks && s.levels < 6) s.levels++;
+  while ((1ull << (s.g * s.levels)) < max_tasks && s.levels < 8) s.levels++;
+  u64 hl = (u64)16 * s.tiles, bl = (u64)s.nb + 1;
+  s.chunks = (u32)(((hl > bl ? hl : bl) + 4095) / 4096);
   s.seg = s.nb >= 8 ? 8 : s.nb;
   s.nseg = s.nb / s.seg;
   return s;
 }
 
 struct MsmLayout {
-  size_t keysA, valsA, keysB, valsB, hist, bstart, bend, ntask, toff, partA, partB, rparts, wsum,
-      total;
+  size_t keysA, valsA, keysB, valsB, hist, bstart, bend, csum, toff, loff, partA, partB, rparts,
+      wsum, total;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -92,8 +100,9 @@ static MsmLayout msm_layout(const MsmShape& s) {
   l.hist = o; o = align256(o + (size_t)s.n_windows * 16 * s.tiles * 4 + 4 * s.n_windows);
   l.bstart = o; o = align256(o + (size_t)s.n_windows * (s.nb + 1) * 4);
   l.bend = o; o = align256(o + (size_t)s.n_windows * (s.nb + 1) * 4);
-  l.ntask = o; o = align256(o + (size_t)s.n_windows * (s.nb + 1) * 4);
+  l.csum = o; o = align256(o + (size_t)s.n_windows * (s.chunks + 1) * 4);
   l.toff = o; o = align256(o + (size_t)s.n_windows * (s.nb + 2) * 4);
+  l.loff = o; o = align256(o + (size_t)s.n_windows * (s.nb + 2) * 4);
   l.partA = o; o = align256(o + (size_t)s.n_windows * s.task_cap * 128);
   l.partB = o; o = align256(o + (size_t)s.n_windows * s.task_cap * 128);
   l.rparts = o; o = align256(o + (size_t)s.n_windows * s.nseg * 128);
@@ -111,7 +120,8 @@ struct MsmJob {
   const u32* bases;
   const u32* scalars;
   u32* result;
-  u32 *kA, *vA, *kB, *vB, *hist, *bstart, *bend, *ntask, *toff, *partA, *partB, *rparts, *wsum;
+  u32 *kA, *vA, *kB, *vB, *hist, *bstart, *bend, *csum, *toff, *loff, *partA, *partB, *rparts,
+      *wsum;
 };
 
 static MsmJob make_msm_job(const MsmShape& s, u64 tstride, const u32* bases, const u32* scalars,
@@ -130,8 +140,9 @@ static MsmJob make_msm_job(const MsmShape& s, u64 tstride, const u32* bases, con
   J.hist = (u32*)(ws + l.hist);
   J.bstart = (u32*)(ws + l.bstart);
   J.bend = (u32*)(ws + l.bend);
-  J.ntask = (u32*)(ws + l.ntask);
+  J.csum = (u32*)(ws + l.csum);
   J.toff = (u32*)(ws + l.toff);
+  J.loff = (u32*)(ws + l.loff);
   J.partA = (u32*)(ws + l.partA);
   J.partB = (u32*)(ws + l.partB);
   J.rparts = (u32*)(ws + l.rparts);
@@ -145,8 +156,12 @@ __device__ __forceinline__ const u32* sorted_keys(const MsmJob& J) {
 __device__ __forceinline__ const u32* sorted_vals(const MsmJob& J) {
   return (J.s.passes & 1) ? J.vB : J.vA;
 }
-__device__ __forceinline__ const u32* final_parts(const MsmJob& J) {
-  return (J.s.levels & 1) ? J.partB : J.partA;
+// combine levels bucket b needs: smallest l with 2^(g l) >= nt (0 for nt <= 1);
+// its total ends in partA for even l, partB for odd l
+__device__ __forceinline__ u32 bucket_levels(u32 nt, u32 g) {
+  u32 l = 0;
+  while ((1ull << (g * l)) < nt) l++;
+  return l;
 }
 
 // ------------------------------------------------------------ stage bodies
@@ -324,12 +339,99 @@ __device__ __forceinline__ void bounds_body(const MsmJob& J, u64 gi) {
   if (i + 1 == s.seg_len || kk[i + 1] != k) J.bend[(u64)pw * (s.nb + 1) + k] = (u32)(i + 1);
 }
 
-__device__ __forceinline__ void tcount_body(const MsmJob& J, u64 gi) {
+// ----------------------------------------------------------- segmented scans
+// One exclusive scan per physical window over one of three inputs:
+//   SC_HIST   the radix histograms hist[(pw * 16 + digit) * tiles + tile], in place;
+//   SC_TASKS  tasks per bucket ceil(count_b / T) (0 for bucket 0) -> toff, total at [nb+1];
+//   SC_LEVEL  groups of combine level x per bucket (0 for a bucket already reduced
+//             to one entry) -> loff, total at [nb+1].
+// Phase A sums chunks of 4096 inputs, phase B scans the chunk sums of a window
+// (and stores the total), phase C scans each chunk and adds its offset.
+enum ScanKind : u32 { SC_HIST = 0, SC_TASKS = 1, SC_LEVEL = 2 };
+static const u32 kScanChunk = 4096;  // 256 threads x 16 inputs
+
+__host__ __device__ inline u64 scan_len(const MsmShape& s, u32 kind) {
+  return kind == SC_HIST ? (u64)16 * s.tiles : (u64)s.nb + 1;
+}
+__host__ __device__ inline u32 scan_chunks(const MsmShape& s, u32 kind) {
+  return (u32)((scan_len(s, kind) + kScanChunk - 1) / kScanChunk);
+}
+
+__device__ __forceinline__ u32 scan_in(const MsmJob& J, u32 kind, u32 x, u32 pw, u64 i) {
   const MsmShape& s = J.s;
-  if (gi >= (u64)s.n_windows * (s.nb + 1)) return;
-  u32 b = (u32)(gi % (s.nb + 1));
-  u32 cnt = J.bend[gi] - J.bstart[gi];
-  J.ntask[gi] = b == 0 ? 0u : (cnt + s.T - 1) / s.T;
+  if (kind == SC_HIST) return J.hist[(u64)pw * 16 * s.tiles + i];
+  if (i == 0) return 0;
+  const u64 bo = (u64)pw * (s.nb + 1) + i;
+  const u32 nt = (J.bend[bo] - J.bstart[bo] + s.T - 1) / s.T;
+  if (kind == SC_TASKS) return nt;
+  const u32 shp = s.g * (x - 1), sh = s.g * x;
+  const u32 nprev = (u32)(((u64)nt + (1ull << shp) - 1) >> shp);
+  return nprev > 1 ? (u32)(((u64)nt + (1ull << sh) - 1) >> sh) : 0u;
+}
+
+__device__ __forceinline__ u32* scan_out(const MsmJob& J, u32 kind, u32 pw) {
+  if (kind == SC_HIST) return J.hist + (u64)pw * 16 * J.s.tiles;
+  return (kind == SC_TASKS ? J.toff : J.loff) + (u64)pw * (J.s.nb + 2);
+}
+
+// exclusive scan of one value per thread (blockDim.x == 256); returns the
+// thread's prefix, *total the block sum
+__device__ __forceinline__ u32 block_excl_scan(u32 v, u32* part, u32* total) {
+  const u32 t = threadIdx.x;
+  __syncthreads();
+  part[t] = v;
+  __syncthreads();
+  for (u32 d = 1; d < 256; d <<= 1) {
+    u32 w = t >= d ? part[t - d] : 0;
+    __syncthreads();
+    part[t] += w;
+    __syncthreads();
+  }
+  *total = part[255];
+  return part[t] - v;
+}
+
+__device__ __forceinline__ void scan_a_body(const MsmJob& J, u32 kind, u32 x, u64 lb, u32* part) {
+  const u32 nch = scan_chunks(J.s, kind);
+  const u32 pw = (u32)(lb / nch), ch = (u32)(lb % nch);
+  const u64 len = scan_len(J.s, kind);
+  const u64 base = (u64)ch * kScanChunk + (u64)threadIdx.x * 16;
+  u32 sum = 0;
+  for (u32 j = 0; j < 16; j++)
+    if (base + j < len) sum += scan_in(J, kind, x, pw, base + j);
+  u32 total;
+  block_excl_scan(sum, part, &total);
+  if (threadIdx.x == 0) J.csum[(u64)pw * (J.s.chunks + 1) + ch] = total;
+}
+
+__device__ __forceinline__ void scan_b_body(const MsmJob& J, u32 kind, u64 pw, u32* part) {
+  const u32 nch = scan_chunks(J.s, kind);
+  u32* cs = J.csum + pw * (J.s.chunks + 1);
+  scan_body(cs, cs, nch, true, part);
+  __syncthreads();
+  if (threadIdx.x == 0 && kind != SC_HIST) scan_out(J, kind, (u32)pw)[scan_len(J.s, kind)] = cs[nch];
+}
+
+__device__ __forceinline__ void scan_c_body(const MsmJob& J, u32 kind, u32 x, u64 lb, u32* part) {
+  const u32 nch = scan_chunks(J.s, kind);
+  const u32 pw = (u32)(lb / nch), ch = (u32)(lb % nch);
+  const u64 len = scan_len(J.s, kind);
+  const u64 base = (u64)ch * kScanChunk + (u64)threadIdx.x * 16;
+  u32 v[16];
+  u32 sum = 0;
+#pragma unroll
+  for (u32 j = 0; j < 16; j++) {
+    v[j] = base + j < len ? scan_in(J, kind, x, pw, base + j) : 0u;
+    sum += v[j];
+  }
+  u32 total;
+  u32 run = block_excl_scan(sum, part, &total) + J.csum[(u64)pw * (J.s.chunks + 1) + ch];
+  u32* out = scan_out(J, kind, pw);
+#pragma unroll
+  for (u32 j = 0; j < 16; j++) {
+    if (base + j < len) out[base + j] = run;
+    run += v[j];
+  }
 }
 
 // largest b in [1, nb] with toff[b] <= t
@@ -374,23 +476,29 @@ __device__ __forceinline__ void tasks_body(const MsmJob& J, u64 gi) {
   store_xyzz(J.partA + 32 * ((u64)pw * s.task_cap + t), acc);
 }
 
-// level l of the per-bucket partial-sum tree: group k of bucket b sums the
-// live entries src[toff_b + kG .. + G) into dst[toff_b + k]
+// level l of the per-bucket partial-sum tree: compact index o of window pw
+// (o < loff[nb+1]) is group k = o - loff[b] of bucket b; it sums the live
+// entries src[toff_b + kG .. + G) of the previous level into dst[toff_b + k].
+// Threads past the window's live groups exit at once.
+__host__ __device__ inline u64 combine_cap(const MsmShape& s, u32 level) {
+  return (s.task_cap >> (s.g * level)) + s.nb + 1;
+}
+
 __device__ __forceinline__ void combine_body(const MsmJob& J, u32 level, u64 gi) {
   const MsmShape& s = J.s;
   if (level > s.levels) return;
-  u32 pw = (u32)(gi / s.task_cap);
+  const u64 cap = combine_cap(s, level);
+  const u32 pw = (u32)(gi / cap);
   if (pw >= s.n_windows) return;
-  u32 t = (u32)(gi - (u64)pw * s.task_cap);
+  const u32 o = (u32)(gi - (u64)pw * cap);
+  const u32* lo = J.loff + (u64)pw * (s.nb + 2);
+  if (o >= lo[s.nb + 1]) return;
+  const u32 b = find_bucket(lo, s.nb, o);
+  const u32 k = o - lo[b];
   const u32* to = J.toff + (u64)pw * (s.nb + 2);
-  if (t >= to[s.nb + 1]) return;
-  u32 b = find_bucket(to, s.nb, t);
-  u32 k = t - to[b];
-  u32 nt = to[b + 1] - to[b];
-  u32 sh_prev = s.g * (level - 1), sh = s.g * level;
-  u32 n_prev = (u32)(((u64)nt + (1ull << sh_prev) - 1) >> sh_prev);
-  u32 n_cur = (u32)(((u64)nt + (1ull << sh) - 1) >> sh);
-  if (k >= n_cur) return;
+  const u32 nt = to[b + 1] - to[b];
+  const u32 shp = s.g * (level - 1);
+  const u32 n_prev = (u32)(((u64)nt + (1ull << shp) - 1) >> shp);
   const u32* src = (level & 1) ? J.partA : J.partB;
   u32* dst = (level & 1) ? J.partB : J.partA;
   const u64 base = (u64)pw * s.task_cap + to[b];
@@ -409,11 +517,15 @@ __device__ __forceinline__ void reduce_body(const MsmJob& J, u64 t) {
   u32 pw = (u32)(t / s.nseg), sg = (u32)(t % s.nseg);
   u32 lo = sg * s.seg;
   const u32* to = J.toff + (u64)pw * (s.nb + 2);
-  const u32* pp = final_parts(J) + 32 * (u64)pw * s.task_cap;
+  const u64 pbase = (u64)pw * s.task_cap;
   G1Xyzz run = G1Xyzz::identity(), tot = G1Xyzz::identity();
   for (int k = (int)s.seg - 1; k >= 0; k--) {
     u32 b = lo + k + 1;
-    if (to[b + 1] > to[b]) run = xyzz_add(run, load_xyzz(pp + 32 * (u64)to[b]));
+    u32 nt = to[b + 1] - to[b];
+    if (nt) {
+      const u32* pp = (bucket_levels(nt, s.g) & 1) ? J.partB : J.partA;
+      run = xyzz_add(run, load_xyzz(pp + 32 * (pbase + to[b])));
+    }
     tot = xyzz_add(tot, run);
   }
   if (lo) tot = xyzz_add(tot, xyzz_mul_u32(run, lo));
@@ -471,9 +583,11 @@ __device__ __forceinline__ u32 find_job(const u32* prefix, u32 njobs, u32 t) {
 }
 
 enum MsmStage : u32 {
-  MS_DIGITS = 0, MS_HIST, MS_HSCAN, MS_SCATTER, MS_ZERO, MS_BOUNDS, MS_TCOUNT, MS_TSCAN,
+  MS_DIGITS = 0, MS_HIST, MS_SCAN_A, MS_SCAN_B, MS_SCAN_C, MS_SCATTER, MS_ZERO, MS_BOUNDS,
   MS_TASKS, MS_COMBINE, MS_REDUCE, MS_SUM, MS_FINAL
 };
+// scan stage argument: ScanKind | (pass or level) << 8
+static inline u32 scan_arg(u32 kind, u32 x) { return kind | (x << 8); }
 
 #define IMP_BATCH_LOOP(BODY)                                        \
   const u32 total = prefix[njobs];                                  \
@@ -492,12 +606,20 @@ __global__ __launch_bounds__(256) void kb_hist(const MsmJob* jobs, u32 njobs, co
   __shared__ u32 cnt[16];
   IMP_BATCH_LOOP(hist_body(J, pass, lb, cnt))
 }
-__global__ __launch_bounds__(1024) void kb_hscan(const MsmJob* jobs, u32 njobs, const u32* prefix) {
-  __shared__ u32 part[1024];
-  IMP_BATCH_LOOP({
-    u32* h = J.hist + lb * 16 * J.s.tiles;
-    scan_body(h, h, (u64)16 * J.s.tiles, false, part);
-  })
+__global__ __launch_bounds__(256) void kb_scan_a(const MsmJob* jobs, u32 njobs,
+                                                 const u32* prefix, u32 arg) {
+  __shared__ u32 part[256];
+  IMP_BATCH_LOOP(scan_a_body(J, arg & 255, arg >> 8, lb, part))
+}
+__global__ __launch_bounds__(256) void kb_scan_b(const MsmJob* jobs, u32 njobs,
+                                                 const u32* prefix, u32 arg) {
+  __shared__ u32 part[256];
+  IMP_BATCH_LOOP(scan_b_body(J, arg & 255, lb, part))
+}
+__global__ __launch_bounds__(256) void kb_scan_c(const MsmJob* jobs, u32 njobs,
+                                                 const u32* prefix, u32 arg) {
+  __shared__ u32 part[256];
+  IMP_BATCH_LOOP(scan_c_body(J, arg & 255, arg >> 8, lb, part))
 }
 __global__ __launch_bounds__(256) void kb_scatter(const MsmJob* jobs, u32 njobs,
                                                   const u32* prefix, u32 pass) {
@@ -509,14 +631,6 @@ __global__ __launch_bounds__(256) void kb_zero(const MsmJob* jobs, u32 njobs, co
 }
 __global__ __launch_bounds__(256) void kb_bounds(const MsmJob* jobs, u32 njobs, const u32* prefix) {
   IMP_BATCH_LOOP(bounds_body(J, lb * blockDim.x + threadIdx.x))
-}
-__global__ __launch_bounds__(256) void kb_tcount(const MsmJob* jobs, u32 njobs, const u32* prefix) {
-  IMP_BATCH_LOOP(tcount_body(J, lb * blockDim.x + threadIdx.x))
-}
-__global__ __launch_bounds__(1024) void kb_tscan(const MsmJob* jobs, u32 njobs, const u32* prefix) {
-  __shared__ u32 part[1024];
-  IMP_BATCH_LOOP(scan_body(J.ntask + lb * (J.s.nb + 1), J.toff + lb * (J.s.nb + 2),
-                           (u64)(J.s.nb + 1), true, part))
 }
 __global__ __launch_bounds__(256) void kb_tasks(const MsmJob* jobs, u32 njobs, const u32* prefix) {
   IMP_BATCH_LOOP(tasks_body(J, lb * blockDim.x + threadIdx.x))
@@ -563,13 +677,19 @@ static u32 stage_blocks(const MsmShape& s, u32 kind, u32 arg) {
     case MS_DIGITS: return (u32)cdiv(s.N, 256);
     case MS_HIST:
     case MS_SCATTER: return arg < s.passes ? s.tiles * s.n_windows : 0;
-    case MS_HSCAN: return arg < s.passes ? s.n_windows : 0;
+    case MS_SCAN_A:
+    case MS_SCAN_B:
+    case MS_SCAN_C: {
+      const u32 sk = arg & 255, x = arg >> 8;
+      const bool live = sk == SC_HIST ? x < s.passes : (sk == SC_LEVEL ? x <= s.levels : true);
+      if (!live) return 0;
+      return kind == MS_SCAN_B ? s.n_windows : s.n_windows * scan_chunks(s, sk);
+    }
     case MS_ZERO: return (u32)cdiv(2ull * s.n_windows * (s.nb + 1), 256);
     case MS_BOUNDS: return (u32)cdiv((u64)s.n_windows * s.seg_len, 256);
-    case MS_TCOUNT: return (u32)cdiv((u64)s.n_windows * (s.nb + 1), 256);
-    case MS_TSCAN: return s.n_windows;
     case MS_TASKS: return (u32)cdiv((u64)s.n_windows * s.task_cap, 256);
-    case MS_COMBINE: return arg <= s.levels ? (u32)cdiv((u64)s.n_windows * s.task_cap, 256) : 0;
+    case MS_COMBINE:
+      return arg <= s.levels ? (u32)cdiv((u64)s.n_windows * combine_cap(s, arg), 256) : 0;
     case MS_REDUCE: return (u32)cdiv((u64)s.n_windows * s.nseg, 256);
     case MS_SUM: return s.n_windows;
     default: return 1;
@@ -585,17 +705,24 @@ static void plan_stages(const std::vector<MsmShape>& shapes, std::vector<StageLa
   }
   std::vector<std::pair<u32, u32>> kinds;
   kinds.push_back({MS_DIGITS, 0});
+  auto scan3 = [&](u32 sk, u32 x) {
+    kinds.push_back({MS_SCAN_A, scan_arg(sk, x)});
+    kinds.push_back({MS_SCAN_B, scan_arg(sk, x)});
+    kinds.push_back({MS_SCAN_C, scan_arg(sk, x)});
+  };
   for (u32 p = 0; p < maxp; p++) {
     kinds.push_back({MS_HIST, p});
-    kinds.push_back({MS_HSCAN, p});
+    scan3(SC_HIST, p);
     kinds.push_back({MS_SCATTER, p});
   }
   kinds.push_back({MS_ZERO, 0});
   kinds.push_back({MS_BOUNDS, 0});
-  kinds.push_back({MS_TCOUNT, 0});
-  kinds.push_back({MS_TSCAN, 0});
+  scan3(SC_TASKS, 0);
   kinds.push_back({MS_TASKS, 0});
-  for (u32 l = 1; l <= maxl; l++) kinds.push_back({MS_COMBINE, l});
+  for (u32 l = 1; l <= maxl; l++) {
+    scan3(SC_LEVEL, l);
+    kinds.push_back({MS_COMBINE, l});
+  }
   kinds.push_back({MS_REDUCE, 0});
   kinds.push_back({MS_SUM, 0});
   kinds.push_back({MS_FINAL, 0});
@@ -613,9 +740,10 @@ static void plan_stages(const std::vector<MsmShape>& shapes, std::vector<StageLa
   }
 }
 
-static const u32 kTimerOf[] = {K_MSM_DIGITS, K_MSM_SORT,   K_MSM_SORT,    K_MSM_SORT, K_MSM_SCAN,
-                               K_MSM_SCAN,   K_MSM_SCAN,   K_MSM_SCAN,    K_MSM_BUCKETS,
-                               K_MSM_COMBINE, K_MSM_REDUCE, K_MSM_SUM,    K_MSM_FINAL};
+// timer of each stage kind (scans count with the sort)
+static const u32 kTimerOf[] = {K_MSM_DIGITS,  K_MSM_SORT,   K_MSM_SORT, K_MSM_SORT, K_MSM_SORT,
+                               K_MSM_SORT,    K_MSM_SCAN,   K_MSM_SCAN, K_MSM_BUCKETS,
+                               K_MSM_COMBINE, K_MSM_REDUCE, K_MSM_SUM,  K_MSM_FINAL};
 
 static hipError_t launch_stages(const MsmJob* jobs, u32 njobs, const u32* prefix_dev,
                                 const StageLaunch* st, u32 nst, u32 max_blocks,
@@ -629,12 +757,12 @@ static hipError_t launch_stages(const MsmJob* jobs, u32 njobs, const u32* prefix
     switch (st[k].kind) {
       case MS_DIGITS: hipLaunchKernelGGL(kb_digits, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
       case MS_HIST: hipLaunchKernelGGL(kb_hist, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf, st[k].arg); break;
-      case MS_HSCAN: hipLaunchKernelGGL(kb_hscan, dim3(grid), dim3(1024), 0, stream, jobs, njobs, pf); break;
+      case MS_SCAN_A: hipLaunchKernelGGL(kb_scan_a, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf, st[k].arg); break;
+      case MS_SCAN_B: hipLaunchKernelGGL(kb_scan_b, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf, st[k].arg); break;
+      case MS_SCAN_C: hipLaunchKernelGGL(kb_scan_c, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf, st[k].arg); break;
       case MS_SCATTER: hipLaunchKernelGGL(kb_scatter, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf, st[k].arg); break;
       case MS_ZERO: hipLaunchKernelGGL(kb_zero, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
       case MS_BOUNDS: hipLaunchKernelGGL(kb_bounds, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
-      case MS_TCOUNT: hipLaunchKernelGGL(kb_tcount, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
-      case MS_TSCAN: hipLaunchKernelGGL(kb_tscan, dim3(grid), dim3(1024), 0, stream, jobs, njobs, pf); break;
       case MS_TASKS: hipLaunchKernelGGL(kb_tasks, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
       case MS_COMBINE: hipLaunchKernelGGL(kb_combine, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf, st[k].arg); break;
       case MS_REDUCE: hipLaunchKernelGGL(kb_reduce, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
@@ -647,7 +775,7 @@ static hipError_t launch_stages(const MsmJob* jobs, u32 njobs, const u32* prefix
 }
 
 // descriptor area at the front of a workspace: jobs, then stage prefixes
-static const size_t kMaxStages = 64;
+static const size_t kMaxStages = 128;
 static size_t desc_bytes(u32 njobs) {
   return align256(sizeof(MsmJob) * njobs) + align256(kMaxStages * (njobs + 1) * 4);
 }

@@ -251,9 +251,9 @@ typedef struct ImpMsmItem {
 typedef struct ImpMsmBatch {
     void *dev;
     unsigned int count, nstages;
-    unsigned int kind[64];
-    unsigned int arg[64];
-    unsigned int total[64];
+    unsigned int kind[128];
+    unsigned int arg[128];
+    unsigned int total[128];
 } ImpMsmBatch;
 
 size_t impeller_msm_batch_bytes(const ImpMsmItem *items, unsigned int count);

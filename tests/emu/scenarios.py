@@ -162,6 +162,25 @@ def single_ntt(E):
             assert (d.reshape(-1, 8) == e).all(), (k, inverse, shift is not None)
 
 
+PLANS = [[0, 10, 4, 4, 4, 4], [0, 10, 8, 8], [5, 10, 3, 8], [0, 11, 10, 6], [0, 10] + [2] * 8,
+         [5, 11, 11], [5, 10, 1, 1, 9], [5, 11, 5, 6], [5, 10] + [1] * 3 + [2] * 4]
+
+
+def ntt_plans(E):
+    k = 16
+    n = 1 << k
+    x = O.gen_scalars(n, 7000)
+    w = mont(random.Random(70).randrange(r))
+    expect = O.best_fft(x, w, k)
+    tw = flat(O.powers(w, n // 2))
+    for plan in PLANS:
+        d = flat(x).copy()
+        arr = (ctypes.c_uint * len(plan))(*plan)
+        E.call("impeller_ntt", ptr(d), ptr(tw), k, 0, None, ctypes.cast(arr, ctypes.c_void_p),
+               len(plan), None)
+        assert (d.reshape(-1, 8) == expect).all(), plan
+
+
 def msm_items(E, nb, seed=91):
     bases = u32(nb * 16)
     E.call("impeller_gen_bases", ptr(bases), nb, seed, 0, None)
@@ -261,6 +280,27 @@ def single_msm(E):
         assert (out == O.msm_closed_form(s, O.gen_scalars(n, 77))).all(), (n, c, target, task)
 
 
+def msm_hot_bucket(E):
+    """Thousands of entries in one bucket: multi-level combine trees (G = 16)."""
+    n = 4096
+    bases = u32(n * 16)
+    E.call("impeller_gen_bases", ptr(bases), n, 77, 0, None)
+    k = O.gen_scalars(n, 77)
+    one = O.ints_to_array([O.to_mont_int(1)])[0]
+    three = O.ints_to_array([O.to_mont_int(3)])[0]
+    cases = []
+    s1 = np.tile(one, (n, 1))                                  # every entry in bucket 1
+    s2 = np.tile(one, (n, 1)); s2[::3] = three                 # two hot buckets
+    s3 = O.gen_scalars(n, 5, kind=1)                           # selector-like
+    for s, c, task in ((s1, 8, 4), (s2, 10, 1), (s3, 12, 2), (s1, 16, 7)):
+        ws_bytes = int(E.call("impeller_msm_cfg_workspace_size", n, c, 0, task))
+        ws = garbage_u32((ws_bytes + 3) // 4)
+        out = garbage_u32(16)
+        E.call("impeller_msm_cfg", ptr(bases), ptr(flat(s)), n, 0, c, 0, task, ptr(out), ptr(ws),
+               ws_bytes, None)
+        assert (out == O.msm_closed_form(s, k)).all(), (c, task)
+
+
 def gate(E):
     for n in (1, 777, 4096):
         ins = [O.gen_scalars(n, 40 + i) for i in range(6)]
@@ -271,9 +311,9 @@ def gate(E):
         assert (out.reshape(-1, 8) == O.gate(*ins)).all(), n
 
 
-ALL = [("single_ntt", single_ntt), ("batched_ntt_mixed", batched_ntt_mixed),
+ALL = [("single_ntt", single_ntt), ("ntt_plans", ntt_plans), ("batched_ntt_mixed", batched_ntt_mixed),
        ("batched_ntt_plans", batched_ntt_plans), ("ntt_graph_replay", ntt_graph_replay),
-       ("gate", gate), ("single_msm", single_msm), ("msm_graph_replay", msm_graph_replay),
+       ("gate", gate), ("single_msm", single_msm), ("msm_hot_bucket", msm_hot_bucket), ("msm_graph_replay", msm_graph_replay),
        ("batched_msm_shared_tables", batched_msm_shared_tables)]
 
 

@@ -169,8 +169,8 @@ def test_impeller_ntt_inverse_and_coset(stream, k):
 
 
 @pytest.mark.parametrize("passes", [[0, 10, 4, 4, 4, 4], [0, 10, 8, 8], [5, 10, 3, 8],
-                                    [0, 11, 10, 6], [0, 10] + [1] * 16, [5, 11, 11],
-                                    [5, 10, 1, 1, 9], [5, 11, 5, 6]])
+                                    [0, 11, 10, 6], [0, 10] + [2] * 8, [5, 11, 11],
+                                    [5, 10, 1, 1, 9], [5, 11, 5, 6], [5, 10] + [1] * 3 + [2] * 4])
 def test_pass_splits_agree(stream, passes):
     k = 16
     x = O.gen_scalars(1 << k, 7000)
@@ -180,6 +180,16 @@ def test_pass_splits_agree(stream, passes):
     N.ntt(d, to_device(host_twiddles(w, k)), k, stream, passes=passes)
     stream.sync()
     assert (to_host(d) == expect).all()
+
+
+@pytest.mark.parametrize("passes", [[0, 10] + [1] * 16, [0, 12, 8, 8], [5, 10, 11],
+                                    [3, 10, 13], [0, 10, 0, 16], [5, 10, 4, 4]])
+def test_invalid_plans_rejected(stream, passes):
+    # at most 8 LDS passes of <= lds_log levels, fuse 0 or 5, levels summing to log n
+    k = 16
+    d = to_device(O.gen_scalars(1 << k, 7001))
+    with pytest.raises(_lib.HipError):
+        N.ntt(d, to_device(host_twiddles(random_omega(71), k)), k, stream, passes=passes)
 
 
 def test_full_size_2_20_vs_oracle(stream):
