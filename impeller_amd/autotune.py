@@ -8,7 +8,7 @@ problem without a sweep; the exhaustive result stays the fallback and the
 judge of the pick's regret.
 
 Configuration spaces:
-  NTT  plan = [fuse, lds_log, d0, d1, ...]  (fuse in {0, 5}, lds_log in
+  NTT  plan = [fuse, lds_log, d0, d1, ...]  (fuse in {0, 4, 5}, lds_log in
        {10, 11}, passes of the remaining levels)
   MSM  (window c, target windows (0 = per-window buckets, 1 = precomputed
        window tables), task size T)
@@ -27,7 +27,7 @@ N_FEATURES = 16
 def ntt_plans(L, max_passes=4):
     """Candidate plans for a 2^L transform."""
     out = []
-    for fuse in ((0, 5) if L >= 10 else (0,)):
+    for fuse in ((0, 4, 5) if L >= 10 else ((0, 4) if L >= 8 else (0,))):
         rest = L - fuse
         for lds_log in (10, 11):
             for k in range(1, max_passes + 1):
@@ -43,7 +43,7 @@ def ntt_plans(L, max_passes=4):
 
 
 def default_ntt_plan(L):
-    fuse = 5 if L >= 10 else 0
+    fuse = 4 if L >= 8 else 0
     rest = L - fuse
     np_ = max(1, (rest + 7) // 8)
     return [fuse, 10] + [rest // np_ + (1 if i < rest % np_ else 0) for i in range(np_)]

@@ -251,8 +251,15 @@ struct ScatterLds {
   u32 part[256];
   u32 gbase[16];
   u32 dbase[16];
+  u32 sk[kSortTile];  // the tile in (digit, input order): staged for coalesced writes
+  u32 sv[kSortTile];
 };
 
+// One tile of 4096 entries: ranks in (digit, input order) from per-thread
+// digit counts (column t of cnt belongs to thread t), entries staged in LDS in
+// that order, then written out so that consecutive threads write consecutive
+// positions of each digit's run: position gbase[d] + (i - dbase[d]) for the
+// i-th staged entry of digit d.
 __device__ __forceinline__ void scatter_body(const MsmJob& J, u32 pass, u64 lb, ScatterLds& L) {
   const MsmShape& s = J.s;
   const u32 tile = (u32)(lb % s.tiles), pw = (u32)(lb / s.tiles), t = threadIdx.x;
@@ -262,7 +269,9 @@ __device__ __forceinline__ void scatter_body(const MsmJob& J, u32 pass, u64 lb, 
   u32* kdst = (pass & 1) ? J.kA : J.kB;
   u32* vdst = (pass & 1) ? J.vA : J.vB;
   const u64 seg0 = (u64)pw * s.seg_len;
-  const u64 base = (u64)tile * kSortTile + (u64)t * 16;
+  const u64 tile0 = (u64)tile * kSortTile;
+  const u64 base = tile0 + (u64)t * 16;
+  const u32 n_tile = (u32)(s.seg_len - tile0 < kSortTile ? s.seg_len - tile0 : kSortTile);
   u32 key[16], val[16];
   __syncthreads();
 #pragma unroll
@@ -305,18 +314,25 @@ __device__ __forceinline__ void scatter_body(const MsmJob& J, u32 pass, u64 lb, 
     L.dbase[t] = L.cnt[t * 256];
   }
   __syncthreads();
-  // rank within digit d = scan(d, t) - scan(d, 0); column t of cnt is private
+  // stage: entry j of thread t goes to its rank in (digit, input order)
 #pragma unroll
   for (u32 j = 0; j < 16; j++) {
-    u64 idx = base + j;
-    if (idx < s.seg_len) {
+    if (base + j < s.seg_len) {
       u32 d = (key[j] >> shift) & 15u;
       u32 r = L.cnt[d * 256 + t];
       L.cnt[d * 256 + t] = r + 1;
-      u64 pos = (u64)L.gbase[d] + r - L.dbase[d];
-      kdst[seg0 + pos] = key[j];
-      vdst[seg0 + pos] = val[j];
+      L.sk[r] = key[j];
+      L.sv[r] = val[j];
     }
+  }
+  __syncthreads();
+  // write out: staged entry i of digit d to gbase[d] + (i - dbase[d])
+  for (u32 i = t; i < n_tile; i += blockDim.x) {
+    u32 k = L.sk[i];
+    u32 d = (k >> shift) & 15u;
+    u64 pos = (u64)L.gbase[d] + i - L.dbase[d];
+    kdst[seg0 + pos] = k;
+    vdst[seg0 + pos] = L.sv[i];
   }
 }
 

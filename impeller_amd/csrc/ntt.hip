@@ -10,8 +10,9 @@
 // (a + w*b, a - w*b).  Only the grouping of levels into HBM passes differs.
 //
 // Passes (all in place, no workspace):
-//   1. bit-reversal permutation: tile pairs (b, rev b) of 32 x 32 elements
-//      swapped through LDS, optionally multiplying x[q] by shift^q (coset);
+//   1. bit-reversal permutation: tile pairs (b, rev b) of 16 x 16 (or 32 x 32)
+//      elements swapped through LDS, optionally multiplying x[q] by shift^q
+//      (coset) and running the first 4 (or 5) DIT levels;
 //   2. ceil(L / 8) DIT passes; each work-group stages a tile of G = 2^D
 //      positions x C columns in LDS and runs D consecutive levels, columns
 //      chosen so that every global access is a run of >= 4 consecutive
@@ -27,31 +28,65 @@
 namespace imp {
 
 // --------------------------------------------------------- on-the-fly factors
-// factor(i) = T0[i & 1023] * T1[(i >> 10) & 1023] * T2[i >> 20]; digits the
-// transform size never reaches are skipped.  The tables hold exact powers, so
-// the product equals the direct power bit for bit.
+// factor(i) = T0[i mod 2^h] * T1[(i >> h) mod 2^h] * T2[i >> 2h] with h-bit
+// digits, h = ceil(L / 2) (at most kFactorDigit): one table for L <= 13,
+// one multiply for L <= 26, two above.  The tables hold exact powers, so the
+// product equals the direct power bit for bit.
+static const u32 kFactorDigit = 13;
+static const u32 kFactorTab = 1u << kFactorDigit;  // entries per digit table
+
 struct Factor {
-  const u32* tab;  // 3 * 1024 elements, or nullptr (factor 1)
-  u32 ndig;        // number of 10-bit digits used (1..3)
+  const u32* tab;  // 3 * kFactorTab elements, or nullptr (factor 1)
+  u32 h;           // digit bits
+  u32 ndig;        // digits used (1..3)
 };
 
+__host__ __device__ inline Factor factor_shape(const u32* tab, u32 L) {
+  Factor f;
+  f.tab = tab;
+  if (L <= kFactorDigit) {
+    f.h = L ? L : 1;
+    f.ndig = 1;
+  } else if (L <= 2 * kFactorDigit) {
+    f.h = (L + 1) / 2;
+    f.ndig = 2;
+  } else {
+    f.h = kFactorDigit;
+    f.ndig = 3;
+  }
+  return f;
+}
+
 __device__ __forceinline__ Fr factor_at(const Factor& f, u64 i) {
-  Fr r = load_fe<Fr>(f.tab + 8 * (i & 1023));
-  if (f.ndig > 1) r = mul<FrParams>(r, load_fe<Fr>(f.tab + 8 * (1024 + ((i >> 10) & 1023))));
-  if (f.ndig > 2) r = mul<FrParams>(r, load_fe<Fr>(f.tab + 8 * (2048 + ((i >> 20) & 1023))));
+  const u64 mask = (1ull << f.h) - 1;
+  Fr r = load_fe<Fr>(f.tab + 8 * (i & mask));
+  if (f.ndig > 1)
+    r = mul<FrParams>(r, load_fe<Fr>(f.tab + 8 * (kFactorTab + ((i >> f.h) & mask))));
+  if (f.ndig > 2)
+    r = mul<FrParams>(r, load_fe<Fr>(f.tab + 8 * (2 * kFactorTab + (i >> (2 * f.h)))));
   return r;
 }
 
-// tab[d * 1024 + j] = scale^(d==0) * base^(j * 1024^d)
-__global__ void k_factor_tables(u32* tab, const u32* base_p, const u32* scale_p, u32 ndig) {
+// tab[d * kFactorTab + j] = scale^(d==0) * base^(j * 2^(h d)), j < 2^h
+__global__ void k_factor_tables(u32* tab, const u32* base_p, const u32* scale_p, u32 h,
+                                u32 ndig) {
   u32 t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 1024 * ndig) return;
-  u32 d = t >> 10, j = t & 1023;
+  const u32 per = 1u << h;
+  if (t >= per * ndig) return;
+  u32 d = t / per, j = t % per;
   Fr base = base_p ? load_fe<Fr>(base_p) : Fr::one();
-  for (u32 k = 0; k < 10 * d; k++) base = sqr<FrParams>(base);
+  for (u32 k = 0; k < h * d; k++) base = sqr<FrParams>(base);
   Fr r = pow_u32<FrParams>(base, j);
   if (d == 0 && scale_p) r = mul<FrParams>(r, load_fe<Fr>(scale_p));
-  store_fe<Fr>(tab + 8 * t, r);
+  store_fe<Fr>(tab + 8 * ((u64)d * kFactorTab + j), r);
+}
+
+static hipError_t launch_factor_tables(const Factor& f, const u32* base_d, const u32* scale_d,
+                                       hipStream_t stream) {
+  const u32 n = (1u << f.h) * f.ndig;
+  hipLaunchKernelGGL(k_factor_tables, dim3((n + 255) / 256), dim3(256), 0, stream,
+                     const_cast<u32*>(f.tab), base_d, scale_d, f.h, f.ndig);
+  return hipGetLastError();
 }
 
 // --------------------------------------------------------------- twiddles
@@ -78,20 +113,26 @@ __device__ __forceinline__ Fr tw_load(const Twiddle& tw, bool two, u64 e) {
   return two ? twiddle_at<true>(tw, e) : twiddle_at<false>(tw, e);
 }
 
+__device__ __forceinline__ void bfly2_v(Fr& a, Fr& b, u64 e, const Twiddle& tw, bool two) {
+  if (e != 0) b = mul<FrParams>(b, tw_load(tw, two, e));
+  Fr s = add<FrParams>(a, b);
+  b = sub<FrParams>(a, b);
+  a = s;
+}
+
 __device__ __forceinline__ void bfly2(u32* pa, u32* pb, u64 e, const Twiddle& tw, bool two) {
   Fr a = load_fe<Fr>(pa), b = load_fe<Fr>(pb);
-  if (e != 0) b = mul<FrParams>(b, tw_load(tw, two, e));
-  store_fe<Fr>(pa, add<FrParams>(a, b));
-  store_fe<Fr>(pb, sub<FrParams>(a, b));
+  bfly2_v(a, b, e, tw, two);
+  store_fe<Fr>(pa, a);
+  store_fe<Fr>(pb, b);
 }
 
 // Two consecutive DIT levels on positions g0 < g1 = g0 + 2^j < g2 = g0 +
 // 2^(j+1) < g3: level j pairs (g0,g1),(g2,g3) with exponent e1; level j+1
 // pairs (g0,g2) with e2 and (g1,g3) with e3.  The same butterflies, in the
 // same order per element, as two radix-2 levels.
-__device__ __forceinline__ void bfly4(u32* p0, u32* p1, u32* p2, u32* p3, u64 e1, u64 e2, u64 e3,
-                                      const Twiddle& tw, bool two) {
-  Fr a0 = load_fe<Fr>(p0), a1 = load_fe<Fr>(p1), a2 = load_fe<Fr>(p2), a3 = load_fe<Fr>(p3);
+__device__ __forceinline__ void bfly4_v(Fr& a0, Fr& a1, Fr& a2, Fr& a3, u64 e1, u64 e2, u64 e3,
+                                        const Twiddle& tw, bool two) {
   if (e1 != 0) {
     Fr w1 = tw_load(tw, two, e1);
     a1 = mul<FrParams>(a1, w1);
@@ -101,10 +142,44 @@ __device__ __forceinline__ void bfly4(u32* p0, u32* p1, u32* p2, u32* p3, u64 e1
   Fr b2 = add<FrParams>(a2, a3), b3 = sub<FrParams>(a2, a3);
   if (e2 != 0) b2 = mul<FrParams>(b2, tw_load(tw, two, e2));
   b3 = mul<FrParams>(b3, tw_load(tw, two, e3));
-  store_fe<Fr>(p0, add<FrParams>(b0, b2));
-  store_fe<Fr>(p2, sub<FrParams>(b0, b2));
-  store_fe<Fr>(p1, add<FrParams>(b1, b3));
-  store_fe<Fr>(p3, sub<FrParams>(b1, b3));
+  a0 = add<FrParams>(b0, b2);
+  a2 = sub<FrParams>(b0, b2);
+  a1 = add<FrParams>(b1, b3);
+  a3 = sub<FrParams>(b1, b3);
+}
+
+__device__ __forceinline__ void bfly4(u32* p0, u32* p1, u32* p2, u32* p3, u64 e1, u64 e2, u64 e3,
+                                      const Twiddle& tw, bool two) {
+  Fr a0 = load_fe<Fr>(p0), a1 = load_fe<Fr>(p1), a2 = load_fe<Fr>(p2), a3 = load_fe<Fr>(p3);
+  bfly4_v(a0, a1, a2, a3, e1, e2, e3, tw, two);
+  store_fe<Fr>(p0, a0);
+  store_fe<Fr>(p1, a1);
+  store_fe<Fr>(p2, a2);
+  store_fe<Fr>(p3, a3);
+}
+
+// Planar LDS tiles: slot s of a buffer of ns slots keeps its low 16 bytes at
+// u32 offset 4 s and its high 16 bytes at 4 (ns + s), so the 16-byte accesses
+// of 16 consecutive lanes hit distinct banks whenever their slots differ
+// mod 16 (a 32-byte element stored whole would span two bank groups).
+struct Planar {
+  u32* p;
+  u32 ns;
+};
+
+__device__ __forceinline__ Fr pl_load(const Planar& t, u32 s) {
+  const uint4* q = reinterpret_cast<const uint4*>(t.p);
+  uint4 a = q[s], b = q[t.ns + s];
+  Fr r;
+  r.v[0] = a.x; r.v[1] = a.y; r.v[2] = a.z; r.v[3] = a.w;
+  r.v[4] = b.x; r.v[5] = b.y; r.v[6] = b.z; r.v[7] = b.w;
+  return r;
+}
+
+__device__ __forceinline__ void pl_store(const Planar& t, u32 s, const Fr& v) {
+  uint4* q = reinterpret_cast<uint4*>(t.p);
+  q[s] = make_uint4(v.v[0], v.v[1], v.v[2], v.v[3]);
+  q[t.ns + s] = make_uint4(v.v[4], v.v[5], v.v[6], v.v[7]);
 }
 
 // ------------------------------------------------------------ job descriptor
@@ -122,7 +197,9 @@ struct NttJob {
   View src;
   u64 src_len;
   u32 L;
-  u32 fuse;  // 0 or 5 (L >= 10)
+  u32 fuse;  // DIT levels fused into the bit reversal: 0, or tb (L >= 2 tb)
+  u32 tb;    // bit-reversal tile side 2^tb (4: 16 x 16, two tile pairs per
+             // 256-thread work-group; 5: 32 x 32, one pair)
   u32 two;   // two-level twiddles
   Twiddle tw;
   Factor pro, epi;  // tab == nullptr: none
@@ -138,62 +215,90 @@ __device__ __forceinline__ Fr src_load(const NttJob& J, u64 q) {
   return v;
 }
 
-// Bit reversal on tile pairs: q = (a | b | c) (5 | L-10 | 5 bits) moves to
-// p = rev(q) = (rev c | rev b | rev a), i.e. from tile b to tile rev(b); a
-// work-group owns both tiles, so the permutation can run in place.  In LDS
-// the destination tiles are laid out by position [a * 32 + c], whose low 5
-// bits c are the bits DIT levels 0..4 pair up (exponent (c mod 2^j) 2^(L-j-1)).
-__device__ __forceinline__ void br_tile_body(const NttJob& J, u64 b, u32* lds) {
-  const u32 L = J.L;
-  u32* d0 = lds;              // positions of tile b
-  u32* d1 = lds + 1024 * 8;   // positions of tile rev(b)
-  const u64 rb = bit_reverse(b, L - 10);
-  if (b > rb) return;
-  const bool self = (b == rb);
+// Bit reversal on tile pairs with tiles of NT x NT = 2^TB x 2^TB elements:
+// q = (a | b | c) (TB | L-2TB | TB bits) moves to p = rev(q) = (rev c | rev b |
+// rev a), i.e. from tile b to tile rev(b); the pair is owned by one group of P
+// threads, so the permutation runs in place.  In LDS a destination tile is
+// row-major by p's high TB bits (row) and low TB bits (column), the column
+// XOR-swizzled by the row (planar storage, conflict-free for the transposing
+// writes, the row-wise butterflies and the row-wise reads).  Fused DIT levels
+// 0..TB-1 pair up columns: level j has exponent (col mod 2^j) 2^(L-j-1).
+// Every thread of the work-group reaches every barrier (valid/active only
+// gate the work), since two pairs can share one work-group.
+template <u32 TB>
+__device__ __forceinline__ void br_tile_pair(const NttJob& J, u64 b, bool valid, u32 tid, u32 P,
+                                             const Planar& T) {
+  const u32 L = J.L, NT = 1u << TB, TS = NT * NT;
+  const u64 rb = valid ? bit_reverse(b, L - 2 * TB) : 0;
+  const bool active = valid && b <= rb;
+  const bool self = b == rb;
   __syncthreads();
-  for (u32 idx = threadIdx.x; idx < 1024; idx += blockDim.x) {
-    u32 a = idx >> 5, c = idx & 31;
-    u32 slot = (brev_bits(c, 5) << 5) | brev_bits(a, 5);
-    store_fe<Fr>((self ? d0 : d1) + 8 * slot, src_load(J, ((u64)a << (L - 5)) | (b << 5) | c));
-    if (!self) store_fe<Fr>(d0 + 8 * slot, src_load(J, ((u64)a << (L - 5)) | (rb << 5) | c));
+  if (active) {
+    for (u32 idx = tid; idx < TS; idx += P) {
+      const u32 a = idx >> TB, c = idx & (NT - 1);
+      const u32 row = brev_bits(c, TB), col = brev_bits(a, TB);
+      const u32 slot = row * NT + (col ^ row);
+      pl_store(T, (self ? 0 : TS) + slot, src_load(J, ((u64)a << (L - TB)) | (b << TB) | c));
+      if (!self) pl_store(T, slot, src_load(J, ((u64)a << (L - TB)) | (rb << TB) | c));
+    }
   }
   __syncthreads();
-  if (J.fuse == 5) {
+  if (J.fuse == TB) {
     const bool two = J.two != 0;
-    const u32 nbuf = self ? 1 : 2;
-    for (u32 j = 0; j < 4; j += 2) {
-      for (u32 qi = threadIdx.x; qi < nbuf * 256; qi += blockDim.x) {
-        u32* row = ((qi >> 8) ? d1 : d0) + 8 * (((qi & 255) >> 3) * 32);
-        u32 k = qi & 7;
-        u32 glo = k & ((1u << j) - 1);
-        u32 c0 = ((k >> j) << (j + 2)) | glo;
-        u64 e1 = (u64)glo << (L - j - 1);
-        u64 e2 = (u64)glo << (L - j - 2);
-        u64 e3 = ((u64)glo + (1u << j)) << (L - j - 2);
-        bfly4(row + 8 * c0, row + 8 * (c0 | (1u << j)), row + 8 * (c0 | (2u << j)),
-              row + 8 * (c0 | (3u << j)), e1, e2, e3, J.tw, two);
+    const u32 nt = self ? 1 : 2;
+    for (u32 j = 0; j + 1 < TB; j += 2) {
+      if (active) {
+        for (u32 qi = tid; qi < nt * (TS / 4); qi += P) {
+          const u32 tl = qi / (TS / 4), q = qi % (TS / 4);
+          const u32 row = q / (NT / 4), k = q % (NT / 4);
+          const u32 glo = k & ((1u << j) - 1);
+          const u32 c0 = ((k >> j) << (j + 2)) | glo;
+          const u32 rs = tl * TS + row * NT;
+          const u32 s0 = rs + (c0 ^ row), s1 = rs + ((c0 | (1u << j)) ^ row);
+          const u32 s2 = rs + ((c0 | (2u << j)) ^ row), s3 = rs + ((c0 | (3u << j)) ^ row);
+          Fr a0 = pl_load(T, s0), a1 = pl_load(T, s1), a2 = pl_load(T, s2), a3 = pl_load(T, s3);
+          bfly4_v(a0, a1, a2, a3, (u64)glo << (L - j - 1), (u64)glo << (L - j - 2),
+                  ((u64)glo + (1u << j)) << (L - j - 2), J.tw, two);
+          pl_store(T, s0, a0);
+          pl_store(T, s1, a1);
+          pl_store(T, s2, a2);
+          pl_store(T, s3, a3);
+        }
       }
       __syncthreads();
     }
-    for (u32 bi = threadIdx.x; bi < nbuf * 512; bi += blockDim.x) {
-      u32* row = ((bi >> 9) ? d1 : d0) + 8 * (((bi & 511) >> 4) * 32);
-      u32 k = bi & 15;
-      bfly2(row + 8 * k, row + 8 * (k | 16u), (u64)k << (L - 5), J.tw, two);
+    if (TB & 1) {
+      const u32 j = TB - 1;
+      if (active) {
+        for (u32 bi = tid; bi < nt * (TS / 2); bi += P) {
+          const u32 tl = bi / (TS / 2), q = bi % (TS / 2);
+          const u32 row = q / (NT / 2), k = q % (NT / 2);
+          const u32 rs = tl * TS + row * NT;
+          const u32 s0 = rs + (k ^ row), s1 = rs + ((k | (1u << j)) ^ row);
+          Fr a0 = pl_load(T, s0), a1 = pl_load(T, s1);
+          bfly2_v(a0, a1, (u64)k << (L - TB), J.tw, two);
+          pl_store(T, s0, a0);
+          pl_store(T, s1, a1);
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
-  for (u32 idx = threadIdx.x; idx < 1024; idx += blockDim.x) {
-    u32 a = idx >> 5, c = idx & 31;
-    u64 p0 = ((u64)a << (L - 5)) | (b << 5) | c;
-    store_fe<Fr>(J.x.base + 8 * view_phys(J.x, p0), load_fe<Fr>(d0 + 8 * idx));
-    if (!self) {
-      u64 p1 = ((u64)a << (L - 5)) | (rb << 5) | c;
-      store_fe<Fr>(J.x.base + 8 * view_phys(J.x, p1), load_fe<Fr>(d1 + 8 * idx));
+  if (active) {
+    for (u32 idx = tid; idx < TS; idx += P) {
+      const u32 row = idx >> TB, col = idx & (NT - 1);
+      const u32 slot = row * NT + (col ^ row);
+      const u64 p0 = ((u64)row << (L - TB)) | (b << TB) | col;
+      store_fe<Fr>(J.x.base + 8 * view_phys(J.x, p0), pl_load(T, slot));
+      if (!self) {
+        const u64 p1 = ((u64)row << (L - TB)) | (rb << TB) | col;
+        store_fe<Fr>(J.x.base + 8 * view_phys(J.x, p1), pl_load(T, TS + slot));
+      }
     }
   }
 }
 
-// whole array (L < 10) in one work-group
+// whole array (L < 2 tb) in one work-group
 __device__ __forceinline__ void br_small_body(const NttJob& J, u32* lds) {
   const u32 n = 1u << J.L;
   __syncthreads();
@@ -204,9 +309,18 @@ __device__ __forceinline__ void br_small_body(const NttJob& J, u32* lds) {
     store_fe<Fr>(J.x.base + 8 * view_phys(J.x, i), load_fe<Fr>(lds + 8 * i));
 }
 
-__device__ __forceinline__ void br_body(const NttJob& J, u64 b, u32* lds) {
-  if (J.L >= 10) br_tile_body(J, b, lds);
-  else br_small_body(J, lds);
+// block lb of the bit-reversal stage (256 threads)
+__device__ __forceinline__ void br_body(const NttJob& J, u64 lb, u32* lds) {
+  if (J.L < 2 * J.tb) {
+    br_small_body(J, lds);
+  } else if (J.tb == 5) {
+    br_tile_pair<5>(J, lb, true, threadIdx.x, 256, Planar{lds, 2 * 1024});
+  } else {
+    const u32 half = threadIdx.x >> 7;
+    const u64 b = 2 * lb + half;
+    br_tile_pair<4>(J, b, b < (1ull << (J.L - 8)), threadIdx.x & 127, 128,
+                    Planar{lds + half * (2 * 256 * 8), 2 * 256});
+  }
 }
 
 // LDS pass over levels s .. s+D-1 on positions p = hi*2^(s+D) + g*2^s + lo.
@@ -392,7 +506,8 @@ __global__ void k_ninv(u32* out, u32 L) {
 }
 
 // ----------------------------------------------------------------- planner
-// fuse: DIT levels done inside the bit-reversal kernel (0 or 5, needs L >= 10);
+// fuse: DIT levels done inside the bit-reversal kernel (0; 4 with 16 x 16 tiles,
+// needs L >= 8; 5 with 32 x 32 tiles, needs L >= 10);
 // lds_log: log2 of the elements one work-group stages (10: 32 KiB, 11: 64 KiB);
 // d[]: level counts of the following LDS passes.
 struct PassPlan {
@@ -403,7 +518,7 @@ struct PassPlan {
 };
 
 static void default_plan(u32 L, PassPlan* p) {
-  p->fuse = L >= 10 ? 5 : 0;
+  p->fuse = L >= 8 ? 4 : 0;
   p->lds_log = 10;
   u32 rest = L - p->fuse;
   u32 np = (rest + 7) / 8;
@@ -421,7 +536,7 @@ static bool plan_from(const unsigned int* plan, unsigned int n, u32 L, PassPlan*
   if (n < 3 || n > 10) return false;
   p->fuse = plan[0];
   p->lds_log = plan[1];
-  if (!(p->fuse == 0 || (p->fuse == 5 && L >= 10))) return false;
+  if (!(p->fuse == 0 || (p->fuse == 4 && L >= 8) || (p->fuse == 5 && L >= 10))) return false;
   if (p->lds_log != 10 && p->lds_log != 11) return false;
   u32 sum = p->fuse;
   p->n = n - 2;
@@ -437,7 +552,10 @@ static bool plan_from(const unsigned int* plan, unsigned int n, u32 L, PassPlan*
 static size_t stage_lds(const NttJob& J, u32 stage) {
   return ((size_t)32 << J.st[stage].D) << J.st[stage].logC;
 }
-static size_t br_lds(const NttJob& J) { return J.L >= 10 ? 2 * 1024 * 32 : ((size_t)32 << J.L); }
+static size_t br_lds(const NttJob& J) {
+  if (J.L < 2 * J.tb) return (size_t)32 << J.L;
+  return J.tb == 5 ? (size_t)2 * 1024 * 32 : (size_t)2 * 2 * 256 * 32;
+}
 
 static NttJob make_job(View x, View src, u64 src_len, u32 L, const Twiddle& tw, bool two,
                        Factor pro, Factor epi, const PassPlan& plan) {
@@ -446,12 +564,14 @@ static NttJob make_job(View x, View src, u64 src_len, u32 L, const Twiddle& tw, 
   J.src = src;
   J.src_len = src_len;
   J.L = L;
-  J.fuse = (L >= 10 && plan.fuse == 5) ? 5 : 0;
+  J.tb = plan.fuse == 5 ? 5 : 4;
+  J.fuse = (plan.fuse == J.tb && L >= 2 * J.tb) ? J.tb : 0;
   J.two = two ? 1 : 0;
   J.tw = tw;
   J.pro = pro;
   J.epi = epi;
-  J.br_blocks = L >= 10 ? (u32)(1ull << (L - 10)) : 1;
+  if (L < 2 * J.tb) J.br_blocks = 1;
+  else J.br_blocks = J.tb == 5 ? (u32)(1ull << (L - 10)) : (u32)(L > 8 ? 1ull << (L - 9) : 1);
   J.nst = 0;
   u32 s = J.fuse;
   for (u32 pi = 0; pi < plan.n && L > 0; pi++) {
@@ -487,14 +607,9 @@ static hipError_t run_job(const NttJob& J, hipStream_t stream) {
 // factor tables in stream-ordered temporary memory
 static hipError_t make_factor(const u32* base_d, const u32* scale_d, u32 L, hipStream_t stream,
                               Factor* f, u32** tmp) {
-  u32 ndig = L <= 10 ? 1 : (L <= 20 ? 2 : 3);
-  IMP_CHECK(hipMallocAsync((void**)tmp, (size_t)3 * 1024 * 32, stream));
-  hipLaunchKernelGGL(k_factor_tables, dim3((1024 * ndig + 255) / 256), dim3(256), 0, stream,
-                     *tmp, base_d, scale_d, ndig);
-  IMP_CHECK(hipGetLastError());
-  f->tab = *tmp;
-  f->ndig = ndig;
-  return hipSuccess;
+  IMP_CHECK(hipMallocAsync((void**)tmp, (size_t)3 * kFactorTab * 32, stream));
+  *f = factor_shape(*tmp, L);
+  return launch_factor_tables(*f, base_d, scale_d, stream);
 }
 
 static std::mutex g_host_mu;
@@ -514,7 +629,7 @@ extern "C" hipError_t ssip_ntt(PolyPtr x, const unsigned int* twiddle, unsigned 
   Twiddle tw = {twiddle, nullptr, nullptr, 0};
   PassPlan plan;
   default_plan(log_len, &plan);
-  Factor none = {nullptr, 0};
+  Factor none = {nullptr, 1, 1};
   return run_job(make_job(v, v, x.len, log_len, tw, false, none, none, plan), stream);
 }
 
@@ -530,7 +645,7 @@ extern "C" hipError_t recompute_ntt(PolyPtr x, const unsigned int* pq_d, unsigne
   IMP_REQUIRE(make_view(x, &v), "PolyPtr view");
   PassPlan plan;
   default_plan(log_len, &plan);
-  Factor none = {nullptr, 0};
+  Factor none = {nullptr, 1, 1};
   Twiddle tw = {omegas_d, nullptr, nullptr, 0};  // n <= 2: only omega^0 = 1 is used
   bool two = false;
   u32* tabs = nullptr;
@@ -639,7 +754,7 @@ extern "C" hipError_t impeller_ntt(uint32_t* x, const uint32_t* twiddle, unsigne
   Twiddle tw = {twiddle, nullptr, nullptr, 0};
   PassPlan plan;
   IMP_REQUIRE(plan_from(plan_passes, n_passes, log_len, &plan), "pass plan");
-  Factor pro = {nullptr, 0}, epi = {nullptr, 0};
+  Factor pro = {nullptr, 1, 1}, epi = {nullptr, 1, 1};
   u32* pro_tab = nullptr;
   u32* epi_tab = nullptr;
   u32* ninv = nullptr;
@@ -651,7 +766,7 @@ extern "C" hipError_t impeller_ntt(uint32_t* x, const uint32_t* twiddle, unsigne
     IMP_CHECK(hipGetLastError());
     // factor(i) = n^-1 * shift^i (shift given as its inverse by the caller)
     // or the constant n^-1 (base 1, one digit)
-    IMP_CHECK(make_factor(coset_shift, ninv, coset_shift ? log_len : 10, stream, &epi, &epi_tab));
+    IMP_CHECK(make_factor(coset_shift, ninv, coset_shift ? log_len : 1, stream, &epi, &epi_tab));
   }
   hipError_t e = run_job(make_job(v, v, v.wl, log_len, tw, false, pro, epi, plan), stream);
   if (pro_tab) IMP_CHECK(hipFreeAsync(pro_tab, stream));
@@ -662,7 +777,7 @@ extern "C" hipError_t impeller_ntt(uint32_t* x, const uint32_t* twiddle, unsigne
 
 // ------------------------------------------------------------ batched NTT
 namespace {
-const size_t kFactorBytes = (size_t)3 * 1024 * 32;
+const size_t kFactorBytes = (size_t)3 * kFactorTab * 32;
 const size_t kJobArea = 2 * kFactorBytes + 256;  // pro table, epi table, n^-1
 size_t batch_jobs_off() { return 0; }
 size_t batch_prefix_off(u32 count) { return align_up((size_t)count * sizeof(NttJob), 256); }
@@ -699,20 +814,17 @@ extern "C" hipError_t impeller_ntt_batch_prepare(const ImpNttItem* items, unsign
     u32* pro_tab = (u32*)area;
     u32* epi_tab = (u32*)(area + kFactorBytes);
     u32* ninv = (u32*)(area + 2 * kFactorBytes);
-    Factor pro = {nullptr, 0}, epi = {nullptr, 0};
-    u32 ndig = it.log_n <= 10 ? 1 : (it.log_n <= 20 ? 2 : 3);
+    Factor pro = {nullptr, 1, 1}, epi = {nullptr, 1, 1};
     if (!it.inverse) {
       if (it.coset_shift) {
-        hipLaunchKernelGGL(k_factor_tables, dim3((1024 * ndig + 255) / 256), dim3(256), 0,
-                           stream, pro_tab, it.coset_shift, (const u32*)nullptr, ndig);
-        pro = Factor{pro_tab, ndig};
+        pro = factor_shape(pro_tab, it.log_n);
+        IMP_CHECK(launch_factor_tables(pro, it.coset_shift, nullptr, stream));
       }
     } else {
       hipLaunchKernelGGL(k_ninv, dim3(1), dim3(1), 0, stream, ninv, it.log_n);
-      u32 nd = it.coset_shift ? ndig : 1;
-      hipLaunchKernelGGL(k_factor_tables, dim3((1024 * nd + 255) / 256), dim3(256), 0, stream,
-                         epi_tab, it.coset_shift, (const u32*)ninv, nd);
-      epi = Factor{epi_tab, nd};
+      // factor(i) = n^-1 * shift^i, or the constant n^-1 (one 2-entry table)
+      epi = factor_shape(epi_tab, it.coset_shift ? it.log_n : 1);
+      IMP_CHECK(launch_factor_tables(epi, it.coset_shift, ninv, stream));
     }
     IMP_CHECK(hipGetLastError());
     jobs[i] = make_job(x, src, src_len, it.log_n, tw, false, pro, epi, plan);

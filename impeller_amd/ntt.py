@@ -113,7 +113,7 @@ def ntt(x_tensor, twiddle_tensor, log_len, stream: Stream, inverse=False, coset_
     """In-place NTT of a contiguous device tensor (impeller_ntt).
 
     passes: optional plan [fuse, lds_log, d0, d1, ...] -- levels fused into the
-    bit-reversal kernel (0 or 5), log2 elements staged per work-group (10 or
+    bit-reversal kernel (0, 4 or 5), log2 elements staged per work-group (10 or
     11) and the level count of each following pass (fuse + sum(d) == log_len).
     """
     plan = None

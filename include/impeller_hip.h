@@ -147,7 +147,8 @@ hipError_t impeller_gate(const uint32_t *const *inputs, uint32_t *out, unsigned 
  * or by shift^-i after an inverse one (shift passed as its inverse then).
  * plan_passes (n_passes entries, or NULL for the library default):
  * {fuse, lds_log, d0, d1, ...}: fuse = DIT levels done inside the
- * bit-reversal kernel (0, or 5 when log_len >= 10), lds_log = log2 of the
+ * bit-reversal kernel (0; 4 when log_len >= 8, 16 x 16 tiles; 5 when
+ * log_len >= 10, 32 x 32 tiles), lds_log = log2 of the
  * elements one work-group stages in LDS (10 or 11), d_i = levels of each
  * following pass; fuse + sum(d_i) == log_len. */
 hipError_t impeller_ntt(uint32_t *x, const uint32_t *twiddle, unsigned int log_len,

@@ -122,10 +122,11 @@ def batched_ntt_mixed(E):
 
 
 def batched_ntt_plans(E):
-    items, expects = ntt_items([("fwd_random", 14), ("inv", 12), ("ext", 14)], 5)
+    items, expects = ntt_items([("fwd_random", 14), ("inv", 12), ("ext", 14), ("fwd", 9)], 5)
     items[0]["plan"] = [5, 11, 9]
     items[1]["plan"] = [0, 10, 2, 3, 7]
     items[2]["plan"] = [5, 10, 4, 5]
+    items[3]["plan"] = [4, 10, 5]
     NttBatch(E, items).run()
     check_items(items, expects, "plans")
 
@@ -143,7 +144,7 @@ def ntt_graph_replay(E):
 
 
 def single_ntt(E):
-    for k in (1, 5, 10, 13, 16):
+    for k in (1, 5, 8, 9, 10, 13, 16):
         n = 1 << k
         x = O.gen_scalars(n, 300 + k)
         w = mont(omega(k))
@@ -163,7 +164,8 @@ def single_ntt(E):
 
 
 PLANS = [[0, 10, 4, 4, 4, 4], [0, 10, 8, 8], [5, 10, 3, 8], [0, 11, 10, 6], [0, 10] + [2] * 8,
-         [5, 11, 11], [5, 10, 1, 1, 9], [5, 11, 5, 6], [5, 10] + [1] * 3 + [2] * 4]
+         [5, 11, 11], [5, 10, 1, 1, 9], [5, 11, 5, 6], [5, 10] + [1] * 3 + [2] * 4,
+         [4, 10, 6, 6], [4, 11, 11, 1], [4, 10, 2, 5, 5]]
 
 
 def ntt_plans(E):

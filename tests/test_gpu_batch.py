@@ -70,11 +70,12 @@ def test_batched_ntt_mixed(stream, max_blocks):
 
 
 def test_batched_ntt_plans(stream):
-    specs = [("fwd_random", 14), ("inv", 12), ("ext", 14)]
+    specs = [("fwd_random", 14), ("inv", 12), ("ext", 14), ("fwd", 9)]
     items, expects = build_items(specs, 5)
     items[0]["plan"] = [5, 11, 9]
     items[1]["plan"] = [0, 10, 2, 3, 7]
     items[2]["plan"] = [5, 10, 4, 5]
+    items[3]["plan"] = [4, 10, 5]
     b = N.NttBatch(items, stream)
     b.run(stream)
     stream.sync()

@@ -170,7 +170,8 @@ def test_impeller_ntt_inverse_and_coset(stream, k):
 
 @pytest.mark.parametrize("passes", [[0, 10, 4, 4, 4, 4], [0, 10, 8, 8], [5, 10, 3, 8],
                                     [0, 11, 10, 6], [0, 10] + [2] * 8, [5, 11, 11],
-                                    [5, 10, 1, 1, 9], [5, 11, 5, 6], [5, 10] + [1] * 3 + [2] * 4])
+                                    [5, 10, 1, 1, 9], [5, 11, 5, 6], [5, 10] + [1] * 3 + [2] * 4,
+                                    [4, 10, 6, 6], [4, 11, 11, 1], [4, 10, 2, 5, 5]])
 def test_pass_splits_agree(stream, passes):
     k = 16
     x = O.gen_scalars(1 << k, 7000)
@@ -223,3 +224,39 @@ def test_full_size_2_24_properties(stream):
     N.ntt(x, twi, k, stream, inverse=True)
     stream.sync()
     assert bool((x == orig).all())
+
+
+@pytest.mark.parametrize("k", [24, 26, 27])
+def test_coset_sparse_inputs_closed_form(stream, k):
+    # x has a few nonzero coefficients, so y_j = sum_q x_q (g w^j)^q is a short
+    # closed form: checks the coset factor shift^q at large q (all factor-table
+    # digits) and the transform at sizes past the CPU oracle's reach; then the
+    # inverse coset transform must give x back
+    n = 1 << k
+    g = 7
+    w = omega(k)
+    rng = random.Random(k)
+    pos = [0, 1, 12345, (n >> 1) + 77, n - 1]
+    vals = [rng.randrange(r) for _ in pos]
+    x = device_u32(n * 8)
+    x.zero_()
+    host = np.zeros((len(pos), 8), np.uint32)
+    for i, (q, v) in enumerate(zip(pos, vals)):
+        host[i] = mont(v)
+        x[q * 8:(q + 1) * 8] = to_device(host[i])
+    tw = N.twiddles(mont(w), n // 2, stream)
+    shift = to_device(mont(g))
+    N.ntt(x, tw, k, stream, coset_shift=shift)
+    stream.sync()
+    for j in (0, 1, 999, n - 2):
+        t = g * pow(w, j, r) % r
+        expect = sum(v * pow(t, q, r) for q, v in zip(pos, vals)) % r
+        got = O.limbs_to_int(to_host(x[j * 8:(j + 1) * 8])[0])
+        assert got == O.to_mont_int(expect), j
+    twi = N.twiddles(mont(pow(w, -1, r)), n // 2, stream)
+    N.ntt(x, twi, k, stream, inverse=True, coset_shift=to_device(mont(pow(g, -1, r))))
+    stream.sync()
+    back = to_host(x)
+    nz = np.flatnonzero(back.any(axis=1))
+    assert list(nz) == pos
+    assert (back[pos] == host).all()
