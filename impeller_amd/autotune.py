@@ -49,7 +49,7 @@ def default_ntt_plan(L):
     return [fuse, 10] + [rest // np_ + (1 if i < rest % np_ else 0) for i in range(np_)]
 
 
-def msm_candidates(windows=range(10, 19), targets=(0, 1), tasks=(32, 64, 128)):
+def msm_candidates(windows=range(10, 21), targets=(0, 1), tasks=(32, 64, 128)):
     return [(c, t, T) for c in windows for t in targets for T in tasks]
 
 
@@ -232,7 +232,7 @@ def pick(model, rows, stream):
     return rows[int(np.argmin(pred))]
 
 
-def regret_report(db, stream, steps=1500, holdout=True):
+def regret_report(db, stream, steps=1500, holdout=True, log=None):
     """Per problem: measured ms of the model's pick vs the exhaustive best and
     vs the library default.  With holdout, the model for a problem is trained
     without that problem's rows (leave-one-problem-out), i.e. a pick without
@@ -260,4 +260,7 @@ def regret_report(db, stream, steps=1500, holdout=True):
             "default_ms": d_ms,
             "default_regret": (d_ms / best["ms"] - 1.0) if d_ms else None,
         }
+        if log:
+            log(f"[regret] {p}: model {report[p]['model_regret']:.3f} "
+                f"default {report[p]['default_regret'] if d_ms else float('nan'):.3f}")
     return report

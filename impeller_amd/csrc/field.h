@@ -238,10 +238,41 @@ __device__ __forceinline__ void mac96s(u64& acc, u32& hi, u32 x, u32 y) {
       : "v"(x), "s"(y));
 }
 
+// Two and four products per asm block, each mad with its own carry SGPR pair,
+// ordered so that every carry read comes at least two wait states after its
+// mad: one s_nop for the pair, none for the quad.  Factors: x*y with y in a
+// VGPR, then u*w with w wave-uniform (a modulus limb).
+__device__ __forceinline__ void mac96_pair(u64& acc, u32& hi, u32 x, u32 y, u32 u, u32 w) {
+  u64 c0, c1;
+  asm("v_mad_u64_u32 %0, %2, %4, %5, %0\n\t"
+      "v_mad_u64_u32 %0, %3, %6, %7, %0\n\t"
+      "s_nop 0\n\t"
+      "v_addc_co_u32 %1, %2, %1, 0, %2\n\t"
+      "v_addc_co_u32 %1, %3, %1, 0, %3"
+      : "+v"(acc), "+v"(hi), "=&s"(c0), "=&s"(c1)
+      : "v"(x), "v"(y), "v"(u), "s"(w));
+}
+__device__ __forceinline__ void mac96_quad(u64& acc, u32& hi, u32 x0, u32 y0, u32 u0, u32 w0,
+                                           u32 x1, u32 y1, u32 u1, u32 w1) {
+  u64 c0, c1, c2, c3;
+  asm("v_mad_u64_u32 %0, %2, %6, %7, %0\n\t"
+      "v_mad_u64_u32 %0, %3, %8, %9, %0\n\t"
+      "v_mad_u64_u32 %0, %4, %10, %11, %0\n\t"
+      "v_mad_u64_u32 %0, %5, %12, %13, %0\n\t"
+      "v_addc_co_u32 %1, %2, %1, 0, %2\n\t"
+      "v_addc_co_u32 %1, %3, %1, 0, %3\n\t"
+      "v_addc_co_u32 %1, %4, %1, 0, %4\n\t"
+      "v_addc_co_u32 %1, %5, %1, 0, %5"
+      : "+v"(acc), "+v"(hi), "=&s"(c0), "=&s"(c1), "=&s"(c2), "=&s"(c3)
+      : "v"(x0), "v"(y0), "v"(u0), "s"(w0), "v"(x1), "v"(y1), "v"(u1), "s"(w1));
+}
+
 // Product-scanning Montgomery multiplication (FIPS, Koc-Acar-Kaliski 1996):
 // column k accumulates a_j*b_(k-j) and m_j*q_(k-j) into a 96-bit accumulator;
 // m_k = acc * q' mod 2^32 for k < 8.  For a, b < 2m and 4m < 2^256 the result
-// is < 2m and fits 8 limbs.
+// is < 2m and fits 8 limbs.  The (a_j b, m_j q) pairs of a column go through
+// the quad and pair blocks above; the column's last products (a_k b_0, then
+// m_k q_0 once m_k is known) through the single-product forms.
 template <class P>
 __device__ __forceinline__ Fp<P> mul_lazy_fips(const Fp<P>& a, const Fp<P>& b) {
   u32 m[8];
@@ -251,9 +282,12 @@ __device__ __forceinline__ Fp<P> mul_lazy_fips(const Fp<P>& a, const Fp<P>& b) {
 #pragma unroll
   for (int i = 0; i < 8; i++) {
 #pragma unroll
-    for (int j = 0; j < i; j++) {
-      mac96(acc, hi, a.v[j], b.v[i - j]);
-      mac96s(acc, hi, m[j], P::m[i - j]);
+    for (int j = 0; j < i; j += 2) {
+      if (j + 1 < i)
+        mac96_quad(acc, hi, a.v[j], b.v[i - j], m[j], P::m[i - j], a.v[j + 1], b.v[i - j - 1],
+                   m[j + 1], P::m[i - j - 1]);
+      else
+        mac96_pair(acc, hi, a.v[j], b.v[i - j], m[j], P::m[i - j]);
     }
     mac96(acc, hi, a.v[i], b.v[0]);
     m[i] = (u32)acc * P::np;
@@ -264,9 +298,12 @@ __device__ __forceinline__ Fp<P> mul_lazy_fips(const Fp<P>& a, const Fp<P>& b) {
 #pragma unroll
   for (int i = 8; i < 16; i++) {
 #pragma unroll
-    for (int j = i - 7; j < 8; j++) {
-      mac96(acc, hi, a.v[j], b.v[i - j]);
-      mac96s(acc, hi, m[j], P::m[i - j]);
+    for (int j = i - 7; j < 8; j += 2) {
+      if (j + 1 < 8)
+        mac96_quad(acc, hi, a.v[j], b.v[i - j], m[j], P::m[i - j], a.v[j + 1], b.v[i - j - 1],
+                   m[j + 1], P::m[i - j - 1]);
+      else
+        mac96_pair(acc, hi, a.v[j], b.v[i - j], m[j], P::m[i - j]);
     }
     r.v[i - 8] = (u32)acc;
     acc = (acc >> 32) | ((u64)hi << 32);
@@ -335,9 +372,10 @@ __device__ __forceinline__ Fp<P> pow_big(const Fp<P>& a, const u32 e[8]) {
   return acc;
 }
 
-// inversion by Fermat: a^(m-2) (field.cuh:1172-1175); inv(0) = 0
+// inversion by Fermat: a^(m-2) (field.cuh:1172-1175); inv(0) = 0.  Kept as
+// the cross-check of inv() below.
 template <class P>
-__device__ __forceinline__ Fp<P> inv(const Fp<P>& a) {
+__device__ __forceinline__ Fp<P> inv_fermat(const Fp<P>& a) {
   u32 e[8];
   u32 borrow = 0;
 #pragma unroll
@@ -347,6 +385,121 @@ __device__ __forceinline__ Fp<P> inv(const Fp<P>& a) {
     borrow = (u32)(t >> 63);
   }
   return pow_big<P>(a, e);
+}
+
+// x / 2 mod m for x < m (m odd): x even -> x >> 1, else (x + m) >> 1 (< 2^255)
+template <class P>
+__device__ __forceinline__ void half_mod(u32 x[8]) {
+  const u32 odd = x[0] & 1u;
+  u32 c = 0;
+  u32 s[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    u64 t = (u64)x[i] + (odd ? P::m[i] : 0u) + c;
+    s[i] = (u32)t;
+    c = (u32)(t >> 32);
+  }
+#pragma unroll
+  for (int i = 0; i < 7; i++) x[i] = (s[i] >> 1) | (s[i + 1] << 31);
+  x[7] = s[7] >> 1;
+}
+
+// x = x - y mod m for x, y < m
+template <class P>
+__device__ __forceinline__ void sub_mod_limbs(u32 x[8], const u32 y[8]) {
+  u32 borrow = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    u64 t = (u64)x[i] - y[i] - borrow;
+    x[i] = (u32)t;
+    borrow = (u32)(t >> 63);
+  }
+  u32 c = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    u64 t = (u64)x[i] + (borrow ? P::m[i] : 0u) + c;
+    x[i] = (u32)t;
+    c = (u32)(t >> 32);
+  }
+}
+
+// Inversion by the binary extended Euclidean algorithm (Hankerson, Menezes
+// and Vanstone, Guide to Elliptic Curve Cryptography, Alg. 2.22, with the
+// two odd/odd cases folded into one step).  Invariants: x1 * A = u and
+// x2 * A = v (mod m) for the input representative A = a R; every step halves
+// u or v, so the loop ends within 2 * 254 steps.  The result A^-1 = a^-1 R^-1
+// is brought to Montgomery form a^-1 R by two products with R^2.  Same value
+// as inv_fermat (the inverse is unique); inv(0) = 0.
+template <class P>
+__device__ __forceinline__ Fp<P> inv(const Fp<P>& a_in) {
+  const Fp<P> a = reduce_once<P>(a_in);  // inputs below 2m
+  if (a.is_zero()) return a;
+  u32 u[8], v[8], x1[8], x2[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    u[i] = a.v[i];
+    v[i] = P::m[i];
+    x1[i] = i == 0 ? 1u : 0u;
+    x2[i] = 0;
+  }
+  // at most 2 * 256 steps for u, v < 2^256; the bound only guards the loop
+  for (int step = 0; step < 1024; step++) {
+    u32 u_hi = 0, v_hi = 0;
+#pragma unroll
+    for (int i = 1; i < 8; i++) {
+      u_hi |= u[i];
+      v_hi |= v[i];
+    }
+    if ((u_hi == 0 && u[0] == 1) || (v_hi == 0 && v[0] == 1)) break;
+    if ((u[0] & 1u) == 0) {
+#pragma unroll
+      for (int i = 0; i < 7; i++) u[i] = (u[i] >> 1) | (u[i + 1] << 31);
+      u[7] >>= 1;
+      half_mod<P>(x1);
+    } else if ((v[0] & 1u) == 0) {
+#pragma unroll
+      for (int i = 0; i < 7; i++) v[i] = (v[i] >> 1) | (v[i + 1] << 31);
+      v[7] >>= 1;
+      half_mod<P>(x2);
+    } else {
+      // both odd: subtract the smaller from the larger (the difference is even)
+      u32 d[8], borrow = 0;
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        u64 t = (u64)u[i] - v[i] - borrow;
+        d[i] = (u32)t;
+        borrow = (u32)(t >> 63);
+      }
+      if (!borrow) {
+#pragma unroll
+        for (int i = 0; i < 7; i++) u[i] = (d[i] >> 1) | (d[i + 1] << 31);
+        u[7] = d[7] >> 1;
+        sub_mod_limbs<P>(x1, x2);
+        half_mod<P>(x1);
+      } else {
+        borrow = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          u64 t = (u64)v[i] - u[i] - borrow;
+          d[i] = (u32)t;
+          borrow = (u32)(t >> 63);
+        }
+#pragma unroll
+        for (int i = 0; i < 7; i++) v[i] = (d[i] >> 1) | (d[i + 1] << 31);
+        v[7] = d[7] >> 1;
+        sub_mod_limbs<P>(x2, x1);
+        half_mod<P>(x2);
+      }
+    }
+  }
+  u32 u_hi = 0;
+#pragma unroll
+  for (int i = 1; i < 8; i++) u_hi |= u[i];
+  const bool use1 = u_hi == 0 && u[0] == 1;
+  Fp<P> r;
+#pragma unroll
+  for (int i = 0; i < 8; i++) r.v[i] = use1 ? x1[i] : x2[i];
+  return mul<P>(mul<P>(r, Fp<P>::r2()), Fp<P>::r2());
 }
 
 // ------------------------------------------------------------------ G1 curve

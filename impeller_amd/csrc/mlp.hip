@@ -51,19 +51,28 @@ __global__ void k_linear_bwd_x(const float* dY, const float* W, const float* A, 
   dX[t] = A[t] > 0.0f ? acc : 0.0f;
 }
 
-// dW[o][i] = sum_b dY[b][o] X[b][i]; db[o] = sum_b dY[b][o]
-__global__ void k_linear_bwd_w(const float* X, const float* dY, float* dW, float* db, u32 B,
-                               u32 in, u32 out) {
-  u32 t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= out * (in + 1)) return;
-  u32 o = t / (in + 1), i = t % (in + 1);
+// dW[o][i] = sum_b dY[b][o] X[b][i] (i < in); db[o] = sum_b dY[b][o] (i == in).
+// One work-group per (o, i): 256 strided partial sums over the batch, then a
+// fixed-order tree, so the result does not depend on scheduling.
+__global__ __launch_bounds__(256) void k_linear_bwd_w(const float* X, const float* dY, float* dW,
+                                                      float* db, u32 B, u32 in, u32 out) {
+  __shared__ float red[256];
+  const u32 o = blockIdx.x / (in + 1), i = blockIdx.x % (in + 1), t = threadIdx.x;
   float acc = 0.0f;
   if (i < in) {
-    for (u32 b = 0; b < B; b++) acc = fmaf(dY[b * out + o], X[b * in + i], acc);
-    dW[o * in + i] = acc;
+    for (u32 b = t; b < B; b += 256) acc = fmaf(dY[b * out + o], X[b * in + i], acc);
   } else {
-    for (u32 b = 0; b < B; b++) acc += dY[b * out + o];
-    db[o] = acc;
+    for (u32 b = t; b < B; b += 256) acc += dY[b * out + o];
+  }
+  red[t] = acc;
+  __syncthreads();
+  for (u32 d = 128; d > 0; d >>= 1) {
+    if (t < d) red[t] += red[t + d];
+    __syncthreads();
+  }
+  if (t == 0) {
+    if (i < in) dW[o * in + i] = red[0];
+    else db[o] = red[0];
   }
 }
 
@@ -144,17 +153,17 @@ extern "C" hipError_t impeller_mlp_grad(const float* params, const float* X, con
   IMP_CHECK(mlp_forward(params, X, B, d, A1, A2, Y, stream));
   hipLaunchKernelGGL(k_mse_grad, g1(B), dim3(256), 0, stream, Y, T, dY, sq, B);
   // layer 3: dW3, db3; dA2 = dY W3 masked by A2
-  hipLaunchKernelGGL(k_linear_bwd_w, g1(d.H + 1), dim3(256), 0, stream, A2, dY, grads + d.oW3,
+  hipLaunchKernelGGL(k_linear_bwd_w, dim3(d.H + 1), dim3(256), 0, stream, A2, dY, grads + d.oW3,
                      grads + d.ob3, B, d.H, 1u);
   hipLaunchKernelGGL(k_linear_bwd_x, g1((u64)B * d.H), dim3(256), 0, stream, dY, params + d.oW3,
                      A2, D2, B, d.H, 1u);
   // layer 2
-  hipLaunchKernelGGL(k_linear_bwd_w, g1((u64)d.H * (d.H + 1)), dim3(256), 0, stream, A1, D2,
+  hipLaunchKernelGGL(k_linear_bwd_w, dim3(d.H * (d.H + 1)), dim3(256), 0, stream, A1, D2,
                      grads + d.oW2, grads + d.ob2, B, d.H, d.H);
   hipLaunchKernelGGL(k_linear_bwd_x, g1((u64)B * d.H), dim3(256), 0, stream, D2, params + d.oW2,
                      A1, D1, B, d.H, d.H);
   // layer 1
-  hipLaunchKernelGGL(k_linear_bwd_w, g1((u64)d.H * (d.F + 1)), dim3(256), 0, stream, X, D1,
+  hipLaunchKernelGGL(k_linear_bwd_w, dim3(d.H * (d.F + 1)), dim3(256), 0, stream, X, D1,
                      grads + d.oW1, grads + d.ob1, B, d.F, d.H);
   if (loss_sum) {
     // sum of squared errors / B via the same column-sum kernel (1 output)

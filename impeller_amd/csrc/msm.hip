@@ -13,7 +13,8 @@
 // Pipeline (all on one stream, device resident):
 //   k_msm_digits      scalar -> signed digits -> (bucket key, point | sign)
 //   k_sort_*          stable LSD radix sort of the keys per physical window
-//                     (4-bit digits, per-tile histograms, block-local ranks)
+//                     (8-bit digits, per-tile histograms, two stable 4-bit
+//                     splits per tile in LDS, coalesced run writes)
 //   k_bucket_bounds   run boundaries of each bucket in the sorted keys
 //   kb_scan_a/b/c     device-wide segmented exclusive scans (chunk sums,
 //                     chunk offsets, apply) for the radix histograms, the
@@ -24,8 +25,9 @@
 //                     runs one thread per live group (compact indices from a
 //                     scan), so a heavily loaded bucket never serialises on one
 //                     thread and idle buckets cost no lanes
-//   k_msm_reduce      sum_b b * B_b via running sums over bucket segments
-//   k_msm_final       window partials -> Horner over windows -> affine
+//   k_msm_reduce      running sums over segments of 4 buckets
+//   k_msm_sum         segment sums split by segment-index bit
+//   k_msm_final       doubling tree per window -> Horner over windows -> affine
 #include <algorithm>
 #include <mutex>
 #include <vector>
@@ -45,17 +47,21 @@ struct MsmShape {
   u64 N;          // points per table
   u64 seg_len;    // sort entries per physical window = n_pre * N
   u32 tiles;      // radix-sort tiles per physical window
-  u32 passes;     // radix passes (4-bit digits over c-bit keys)
+  u32 passes;     // radix passes (8-bit digits over c-bit keys)
   u32 T;          // entries per accumulation task
   u32 g;          // log2 of the combine group size
   u32 levels;     // combine levels (worst case over bucket loads)
   u32 chunks;     // scan chunks per window (max over the scans)
   u64 task_cap;   // task slots per physical window
-  u32 seg;        // buckets per reduce segment
-  u32 nseg;       // reduce segments per physical window
+  u32 seg;        // buckets per reduce segment (4)
+  u32 nseg;       // reduce segments per physical window (nb / 4)
+  u32 lbits;      // log2(nseg)
+  u32 sumg;       // work-groups per window sum (power of two)
 };
 
 static const u32 kSortTile = 4096;  // 256 threads x 16 keys
+static const u32 kRadixBits = 8;     // digit bits per sort pass
+static const u32 kRadix = 1u << kRadixBits;
 
 static MsmShape msm_shape(u64 N, u32 c, u32 target_windows, u32 task = 64) {
   MsmShape s;
@@ -68,23 +74,31 @@ static MsmShape msm_shape(u64 N, u32 c, u32 target_windows, u32 task = 64) {
   s.seg_len = (u64)s.n_pre * N;
   s.tiles = (u32)((s.seg_len + kSortTile - 1) / kSortTile);
   if (s.tiles == 0) s.tiles = 1;
-  s.passes = (c + 3) / 4;
+  s.passes = (c + kRadixBits - 1) / kRadixBits;  // keys <= 2^(c-1) need c bits
   s.T = task;
   s.g = 4;
   s.task_cap = s.seg_len / s.T + s.nb + 1;
   u64 max_tasks = s.seg_len / s.T + 1;
   s.levels = 0;
   while ((1ull << (s.g * s.levels)) < max_tasks && s.levels < 8) s.levels++;
-  u64 hl = (u64)16 * s.tiles, bl = (u64)s.nb + 1;
+  u64 hl = (u64)kRadix * s.tiles, bl = (u64)s.nb + 1;
   s.chunks = (u32)(((hl > bl ? hl : bl) + 4095) / 4096);
-  s.seg = s.nb >= 8 ? 8 : s.nb;
-  s.nseg = s.nb / s.seg;
+  s.seg = 4;
+  s.nseg = s.nb / s.seg;  // nb >= 8 for c >= 4
+  s.lbits = c - 3;
+  // window sums: lbits + 1 of them (the T_s and one per segment-index bit),
+  // each split over sumg work-groups; the final block holds every partial
+  const u32 nv = s.lbits + 2;
+  s.sumg = 1;
+  while (s.sumg < 8 && (u64)s.sumg * 2 * 256 <= s.nseg &&
+         (u64)s.n_windows * nv * s.sumg * 2 <= 256)
+    s.sumg *= 2;
   return s;
 }
 
 struct MsmLayout {
   size_t keysA, valsA, keysB, valsB, hist, bstart, bend, csum, toff, loff, partA, partB, rparts,
-      wsum, total;
+      sparts, total;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -97,7 +111,7 @@ static MsmLayout msm_layout(const MsmShape& s) {
   l.valsA = o; o = align256(o + ent);
   l.keysB = o; o = align256(o + ent);
   l.valsB = o; o = align256(o + ent);
-  l.hist = o; o = align256(o + (size_t)s.n_windows * 16 * s.tiles * 4 + 4 * s.n_windows);
+  l.hist = o; o = align256(o + (size_t)s.n_windows * kRadix * s.tiles * 4 + 4 * s.n_windows);
   l.bstart = o; o = align256(o + (size_t)s.n_windows * (s.nb + 1) * 4);
   l.bend = o; o = align256(o + (size_t)s.n_windows * (s.nb + 1) * 4);
   l.csum = o; o = align256(o + (size_t)s.n_windows * (s.chunks + 1) * 4);
@@ -105,8 +119,8 @@ static MsmLayout msm_layout(const MsmShape& s) {
   l.loff = o; o = align256(o + (size_t)s.n_windows * (s.nb + 2) * 4);
   l.partA = o; o = align256(o + (size_t)s.n_windows * s.task_cap * 128);
   l.partB = o; o = align256(o + (size_t)s.n_windows * s.task_cap * 128);
-  l.rparts = o; o = align256(o + (size_t)s.n_windows * s.nseg * 128);
-  l.wsum = o; o = align256(o + (size_t)s.n_windows * 128);
+  l.rparts = o; o = align256(o + (size_t)s.n_windows * 2 * s.nseg * 128);
+  l.sparts = o; o = align256(o + (size_t)s.n_windows * (s.lbits + 1) * s.sumg * 128);
   l.total = o;
   return l;
 }
@@ -121,7 +135,7 @@ struct MsmJob {
   const u32* scalars;
   u32* result;
   u32 *kA, *vA, *kB, *vB, *hist, *bstart, *bend, *csum, *toff, *loff, *partA, *partB, *rparts,
-      *wsum;
+      *sparts;
 };
 
 static MsmJob make_msm_job(const MsmShape& s, u64 tstride, const u32* bases, const u32* scalars,
@@ -146,7 +160,7 @@ static MsmJob make_msm_job(const MsmShape& s, u64 tstride, const u32* bases, con
   J.partA = (u32*)(ws + l.partA);
   J.partB = (u32*)(ws + l.partB);
   J.rparts = (u32*)(ws + l.rparts);
-  J.wsum = (u32*)(ws + l.wsum);
+  J.sparts = (u32*)(ws + l.sparts);
   return J;
 }
 
@@ -199,25 +213,25 @@ __device__ __forceinline__ void digits_body(const MsmJob& J, u64 i) {
   }
 }
 
-// LSD radix sort, 4-bit digits, stable per pass: 16 consecutive keys per
-// thread (blocked), block ranks in (digit, thread) order, global bases from
-// the scanned per-tile histograms hist[(pw * 16 + digit) * tiles + tile].
+// LSD radix sort, 8-bit digits, stable per pass.  Tile histograms go to
+// hist[(pw * 256 + digit) * tiles + tile]; their exclusive scan in that order
+// is each (digit, tile) run's global start.
 __device__ __forceinline__ void hist_body(const MsmJob& J, u32 pass, u64 lb, u32* cnt) {
   const MsmShape& s = J.s;
   const u32 tile = (u32)(lb % s.tiles), pw = (u32)(lb / s.tiles), t = threadIdx.x;
-  const u32 shift = 4 * pass;
+  const u32 shift = kRadixBits * pass;
   const u32* keys = (pass & 1) ? J.kB : J.kA;
   __syncthreads();
-  if (t < 16) cnt[t] = 0;
+  cnt[t] = 0;  // blockDim.x == kRadix
   __syncthreads();
   const u32* kk = keys + (u64)pw * s.seg_len;
-  u64 base = (u64)tile * kSortTile + (u64)t * 16;
-  for (u32 j = 0; j < 16; j++) {
-    u64 idx = base + j;
-    if (idx < s.seg_len) atomicAdd(&cnt[(kk[idx] >> shift) & 15u], 1u);
+  const u64 tile0 = (u64)tile * kSortTile;
+  for (u32 j = 0; j < kSortTile / 256; j++) {
+    u64 idx = tile0 + (u64)j * 256 + t;
+    if (idx < s.seg_len) atomicAdd(&cnt[(kk[idx] >> shift) & (kRadix - 1)], 1u);
   }
   __syncthreads();
-  if (t < 16) J.hist[((u64)pw * 16 + t) * s.tiles + tile] = cnt[t];
+  J.hist[((u64)pw * kRadix + t) * s.tiles + tile] = cnt[t];
 }
 
 // block-wide exclusive scan of len entries (in place allowed); with
@@ -246,47 +260,45 @@ __device__ __forceinline__ void scan_body(const u32* src, u32* dst, u64 len, boo
   if (write_total && t == T - 1) dst[len] = part[T - 1];
 }
 
+// exclusive scan of one value per thread (blockDim.x == 256); returns the
+// thread's prefix, *total the block sum
+__device__ __forceinline__ u32 block_excl_scan(u32 v, u32* part, u32* total) {
+  const u32 t = threadIdx.x;
+  __syncthreads();
+  part[t] = v;
+  __syncthreads();
+  for (u32 d = 1; d < 256; d <<= 1) {
+    u32 w = t >= d ? part[t - d] : 0;
+    __syncthreads();
+    part[t] += w;
+    __syncthreads();
+  }
+  *total = part[255];
+  return part[t] - v;
+}
+
 struct ScatterLds {
-  u32 cnt[16 * 256];
+  u32 cnt[16 * 256];  // per (nibble, thread) counts, then running ranks
   u32 part[256];
-  u32 gbase[16];
-  u32 dbase[16];
+  u32 gbase[kRadix];  // global start of each digit's run of this tile
+  u32 dbase[kRadix];  // first staged position of each digit
   u32 sk[kSortTile];  // the tile in (digit, input order): staged for coalesced writes
   u32 sv[kSortTile];
 };
 
-// One tile of 4096 entries: ranks in (digit, input order) from per-thread
-// digit counts (column t of cnt belongs to thread t), entries staged in LDS in
-// that order, then written out so that consecutive threads write consecutive
-// positions of each digit's run: position gbase[d] + (i - dbase[d]) for the
-// i-th staged entry of digit d.
-__device__ __forceinline__ void scatter_body(const MsmJob& J, u32 pass, u64 lb, ScatterLds& L) {
-  const MsmShape& s = J.s;
-  const u32 tile = (u32)(lb % s.tiles), pw = (u32)(lb / s.tiles), t = threadIdx.x;
-  const u32 shift = 4 * pass;
-  const u32* ksrc = (pass & 1) ? J.kB : J.kA;
-  const u32* vsrc = (pass & 1) ? J.vB : J.vA;
-  u32* kdst = (pass & 1) ? J.kA : J.kB;
-  u32* vdst = (pass & 1) ? J.vA : J.vB;
-  const u64 seg0 = (u64)pw * s.seg_len;
-  const u64 tile0 = (u64)tile * kSortTile;
-  const u64 base = tile0 + (u64)t * 16;
-  const u32 n_tile = (u32)(s.seg_len - tile0 < kSortTile ? s.seg_len - tile0 : kSortTile);
-  u32 key[16], val[16];
-  __syncthreads();
+// Stable split of the tile by the 4-bit field at bit sh.  Thread t holds tile
+// positions 16t .. 16t + n - 1 in key/val; ranks come from per-thread field
+// counts scanned in (field, thread) order, and the entries land in sk/sv.
+__device__ __forceinline__ void split_nibble(ScatterLds& L, const u32 key[16], const u32 val[16],
+                                             u32 n, u32 sh) {
+  const u32 t = threadIdx.x;
 #pragma unroll
   for (u32 d = 0; d < 16; d++) L.cnt[d * 256 + t] = 0;
 #pragma unroll
-  for (u32 j = 0; j < 16; j++) {
-    u64 idx = base + j;
-    if (idx < s.seg_len) {
-      key[j] = ksrc[seg0 + idx];
-      val[j] = vsrc[seg0 + idx];
-      L.cnt[((key[j] >> shift) & 15u) * 256 + t] += 1;
-    }
-  }
+  for (u32 j = 0; j < 16; j++)
+    if (j < n) L.cnt[((key[j] >> sh) & 15u) * 256 + t] += 1;
   __syncthreads();
-  // exclusive scan of cnt in (digit, thread) order: thread t owns [16t, 16t+16)
+  // thread t scans the flattened counts [16t, 16t + 16)
   u32 loc[16];
   u32 sum = 0;
 #pragma unroll
@@ -294,31 +306,18 @@ __device__ __forceinline__ void scatter_body(const MsmJob& J, u32 pass, u64 lb, 
     loc[j] = L.cnt[t * 16 + j];
     sum += loc[j];
   }
-  L.part[t] = sum;
-  __syncthreads();
-  for (u32 d = 1; d < 256; d <<= 1) {
-    u32 v = t >= d ? L.part[t - d] : 0;
-    __syncthreads();
-    L.part[t] += v;
-    __syncthreads();
-  }
-  u32 run = t ? L.part[t - 1] : 0;
+  u32 total;
+  u32 run = block_excl_scan(sum, L.part, &total);
 #pragma unroll
   for (u32 j = 0; j < 16; j++) {
     L.cnt[t * 16 + j] = run;
     run += loc[j];
   }
   __syncthreads();
-  if (t < 16) {
-    L.gbase[t] = J.hist[((u64)pw * 16 + t) * s.tiles + tile];
-    L.dbase[t] = L.cnt[t * 256];
-  }
-  __syncthreads();
-  // stage: entry j of thread t goes to its rank in (digit, input order)
 #pragma unroll
   for (u32 j = 0; j < 16; j++) {
-    if (base + j < s.seg_len) {
-      u32 d = (key[j] >> shift) & 15u;
+    if (j < n) {
+      u32 d = (key[j] >> sh) & 15u;
       u32 r = L.cnt[d * 256 + t];
       L.cnt[d * 256 + t] = r + 1;
       L.sk[r] = key[j];
@@ -326,11 +325,55 @@ __device__ __forceinline__ void scatter_body(const MsmJob& J, u32 pass, u64 lb, 
     }
   }
   __syncthreads();
-  // write out: staged entry i of digit d to gbase[d] + (i - dbase[d])
+}
+
+// One tile of 4096 entries: two stable nibble splits order it by the 8-bit
+// digit (input order kept within a digit), staged in LDS; then consecutive
+// threads write consecutive positions of each digit's run: staged entry i of
+// digit d goes to gbase[d] + (i - dbase[d]).
+__device__ __forceinline__ void scatter_body(const MsmJob& J, u32 pass, u64 lb, ScatterLds& L) {
+  const MsmShape& s = J.s;
+  const u32 tile = (u32)(lb % s.tiles), pw = (u32)(lb / s.tiles), t = threadIdx.x;
+  const u32 shift = kRadixBits * pass;
+  const u32* ksrc = (pass & 1) ? J.kB : J.kA;
+  const u32* vsrc = (pass & 1) ? J.vB : J.vA;
+  u32* kdst = (pass & 1) ? J.kA : J.kB;
+  u32* vdst = (pass & 1) ? J.vA : J.vB;
+  const u64 seg0 = (u64)pw * s.seg_len;
+  const u64 tile0 = (u64)tile * kSortTile;
+  const u32 n_tile = (u32)(s.seg_len - tile0 < kSortTile ? s.seg_len - tile0 : kSortTile);
+  const u32 n = n_tile > 16 * t ? (n_tile - 16 * t < 16 ? n_tile - 16 * t : 16) : 0;
+  u32 key[16], val[16];
+  __syncthreads();
+#pragma unroll
+  for (u32 j = 0; j < 16; j++) {
+    if (j < n) {
+      key[j] = ksrc[seg0 + tile0 + 16 * t + j];
+      val[j] = vsrc[seg0 + tile0 + 16 * t + j];
+    }
+  }
+  split_nibble(L, key, val, n, shift);
+#pragma unroll
+  for (u32 j = 0; j < 16; j++) {
+    if (j < n) {
+      key[j] = L.sk[16 * t + j];
+      val[j] = L.sv[16 * t + j];
+    }
+  }
+  __syncthreads();
+  split_nibble(L, key, val, n, shift + 4);
+  // digit run starts in the staged order, and the runs' global starts
+  L.gbase[t] = J.hist[((u64)pw * kRadix + t) * s.tiles + tile];
+  for (u32 j = 0; j < n; j++) {
+    const u32 i = 16 * t + j;
+    const u32 d = (L.sk[i] >> shift) & (kRadix - 1);
+    if (i == 0 || ((L.sk[i - 1] >> shift) & (kRadix - 1)) != d) L.dbase[d] = i;
+  }
+  __syncthreads();
   for (u32 i = t; i < n_tile; i += blockDim.x) {
-    u32 k = L.sk[i];
-    u32 d = (k >> shift) & 15u;
-    u64 pos = (u64)L.gbase[d] + i - L.dbase[d];
+    const u32 k = L.sk[i];
+    const u32 d = (k >> shift) & (kRadix - 1);
+    const u64 pos = (u64)L.gbase[d] + i - L.dbase[d];
     kdst[seg0 + pos] = k;
     vdst[seg0 + pos] = L.sv[i];
   }
@@ -357,7 +400,7 @@ __device__ __forceinline__ void bounds_body(const MsmJob& J, u64 gi) {
 
 // ----------------------------------------------------------- segmented scans
 // One exclusive scan per physical window over one of three inputs:
-//   SC_HIST   the radix histograms hist[(pw * 16 + digit) * tiles + tile], in place;
+//   SC_HIST   the radix histograms hist[(pw * 256 + digit) * tiles + tile], in place;
 //   SC_TASKS  tasks per bucket ceil(count_b / T) (0 for bucket 0) -> toff, total at [nb+1];
 //   SC_LEVEL  groups of combine level x per bucket (0 for a bucket already reduced
 //             to one entry) -> loff, total at [nb+1].
@@ -367,7 +410,7 @@ enum ScanKind : u32 { SC_HIST = 0, SC_TASKS = 1, SC_LEVEL = 2 };
 static const u32 kScanChunk = 4096;  // 256 threads x 16 inputs
 
 __host__ __device__ inline u64 scan_len(const MsmShape& s, u32 kind) {
-  return kind == SC_HIST ? (u64)16 * s.tiles : (u64)s.nb + 1;
+  return kind == SC_HIST ? (u64)kRadix * s.tiles : (u64)s.nb + 1;
 }
 __host__ __device__ inline u32 scan_chunks(const MsmShape& s, u32 kind) {
   return (u32)((scan_len(s, kind) + kScanChunk - 1) / kScanChunk);
@@ -375,7 +418,7 @@ __host__ __device__ inline u32 scan_chunks(const MsmShape& s, u32 kind) {
 
 __device__ __forceinline__ u32 scan_in(const MsmJob& J, u32 kind, u32 x, u32 pw, u64 i) {
   const MsmShape& s = J.s;
-  if (kind == SC_HIST) return J.hist[(u64)pw * 16 * s.tiles + i];
+  if (kind == SC_HIST) return J.hist[(u64)pw * kRadix * s.tiles + i];
   if (i == 0) return 0;
   const u64 bo = (u64)pw * (s.nb + 1) + i;
   const u32 nt = (J.bend[bo] - J.bstart[bo] + s.T - 1) / s.T;
@@ -386,25 +429,8 @@ __device__ __forceinline__ u32 scan_in(const MsmJob& J, u32 kind, u32 x, u32 pw,
 }
 
 __device__ __forceinline__ u32* scan_out(const MsmJob& J, u32 kind, u32 pw) {
-  if (kind == SC_HIST) return J.hist + (u64)pw * 16 * J.s.tiles;
+  if (kind == SC_HIST) return J.hist + (u64)pw * kRadix * J.s.tiles;
   return (kind == SC_TASKS ? J.toff : J.loff) + (u64)pw * (J.s.nb + 2);
-}
-
-// exclusive scan of one value per thread (blockDim.x == 256); returns the
-// thread's prefix, *total the block sum
-__device__ __forceinline__ u32 block_excl_scan(u32 v, u32* part, u32* total) {
-  const u32 t = threadIdx.x;
-  __syncthreads();
-  part[t] = v;
-  __syncthreads();
-  for (u32 d = 1; d < 256; d <<= 1) {
-    u32 w = t >= d ? part[t - d] : 0;
-    __syncthreads();
-    part[t] += w;
-    __syncthreads();
-  }
-  *total = part[255];
-  return part[t] - v;
 }
 
 __device__ __forceinline__ void scan_a_body(const MsmJob& J, u32 kind, u32 x, u64 lb, u32* part) {
@@ -525,61 +551,118 @@ __device__ __forceinline__ void combine_body(const MsmJob& J, u32 level, u64 gi)
   store_xyzz(dst + 32 * (base + k), acc);
 }
 
-// segment [lo, lo + seg) of buckets (bucket number b = lo + k + 1):
-// partial = sum_k (lo + k + 1) * B = T + lo * R with running sums
+// Window sum S = sum_b b * B_b (b = 1 .. nb) in three shallow stages.
+// Segment s holds buckets 4s + 1 .. 4s + 4:
+//   T_s = sum_k (k + 1) B_(4s+k+1),  R_s = sum_k B_(4s+k+1),
+//   S = sum_s T_s + 4 sum_s s R_s = sum_s T_s + 4 sum_l 2^l V_l,
+//   V_l = sum of R_s over the s with bit l set (l < lbits = log2 nseg).
+// reduce: one thread per segment (running sums over 4 buckets);
+// sum:    lbits + 1 sums (the T_s, then each V_l), sumg work-groups each;
+// final:  partials -> a_0 = sum T, a_1 = 0, a_(l+2) = V_l, S = sum_i 2^i a_i
+//         by a doubling tree, then Horner over physical windows and affine.
 __device__ __forceinline__ void reduce_body(const MsmJob& J, u64 t) {
   const MsmShape& s = J.s;
   if (t >= (u64)s.n_windows * s.nseg) return;
-  u32 pw = (u32)(t / s.nseg), sg = (u32)(t % s.nseg);
-  u32 lo = sg * s.seg;
+  const u32 pw = (u32)(t / s.nseg), sg = (u32)(t % s.nseg);
+  const u32 lo = sg * s.seg;
   const u32* to = J.toff + (u64)pw * (s.nb + 2);
   const u64 pbase = (u64)pw * s.task_cap;
   G1Xyzz run = G1Xyzz::identity(), tot = G1Xyzz::identity();
   for (int k = (int)s.seg - 1; k >= 0; k--) {
-    u32 b = lo + k + 1;
-    u32 nt = to[b + 1] - to[b];
+    const u32 b = lo + k + 1;
+    const u32 nt = to[b + 1] - to[b];
     if (nt) {
       const u32* pp = (bucket_levels(nt, s.g) & 1) ? J.partB : J.partA;
       run = xyzz_add(run, load_xyzz(pp + 32 * (pbase + to[b])));
     }
     tot = xyzz_add(tot, run);
   }
-  if (lo) tot = xyzz_add(tot, xyzz_mul_u32(run, lo));
-  store_xyzz(J.rparts + 32 * t, tot);
+  store_xyzz(J.rparts + 32 * ((u64)(2 * pw) * s.nseg + sg), tot);
+  store_xyzz(J.rparts + 32 * ((u64)(2 * pw + 1) * s.nseg + sg), run);
 }
 
-// one block per physical window: sum of its segment partials
-__device__ __forceinline__ void sum_body(const MsmJob& J, u64 pw, u32* red) {
-  const MsmShape& s = J.s;
+// pairwise tree over the 256 XYZZ values red[0..255] with the given stride
+// pattern: after it red[0] is their sum (blockDim.x == 256)
+__device__ __forceinline__ void block_tree_sum(u32* red) {
   const u32 t = threadIdx.x;
-  G1Xyzz acc = G1Xyzz::identity();
-  __syncthreads();
-  for (u32 k = t; k < s.nseg; k += blockDim.x)
-    acc = xyzz_add(acc, load_xyzz(J.rparts + 32 * (pw * s.nseg + k)));
-  store_xyzz(red + 32 * t, acc);
-  __syncthreads();
-  for (u32 d = blockDim.x / 2; d > 0; d >>= 1) {
+  for (u32 d = 128; d > 0; d >>= 1) {
     if (t < d) {
       G1Xyzz a = load_xyzz(red + 32 * t), b = load_xyzz(red + 32 * (t + d));
       store_xyzz(red + 32 * t, xyzz_add(a, b));
     }
     __syncthreads();
   }
-  if (t < 32) J.wsum[32 * pw + t] = red[t];
 }
 
-// Horner over physical windows, then affine (thread 0 of the job's block)
-__device__ __forceinline__ void final_body(const MsmJob& J) {
-  if (threadIdx.x != 0) return;
+// work-group lb: window pw, sum q (0: the T_s; q = l + 1: V_l), share g
+__device__ __forceinline__ void sum_body(const MsmJob& J, u64 lb, u32* red) {
   const MsmShape& s = J.s;
+  const u32 t = threadIdx.x;
+  const u32 g = (u32)(lb % s.sumg);
+  const u32 q = (u32)((lb / s.sumg) % (s.lbits + 1));
+  const u32 pw = (u32)(lb / ((u64)s.sumg * (s.lbits + 1)));
+  const u32 count = q == 0 ? s.nseg : s.nseg / 2;
+  const u32* src = J.rparts + 32 * ((u64)(2 * pw + (q ? 1 : 0)) * s.nseg);
+  const u32 l = q ? q - 1 : 0;
+  G1Xyzz acc = G1Xyzz::identity();
+  __syncthreads();
+  for (u32 i = g * 256 + t; i < count; i += s.sumg * 256) {
+    // q > 0: the i-th segment index with bit l set
+    const u32 sg = q == 0 ? i : (((i >> l) << (l + 1)) | (1u << l) | (i & ((1u << l) - 1)));
+    acc = xyzz_add(acc, load_xyzz(src + 32 * (u64)sg));
+  }
+  store_xyzz(red + 32 * t, acc);
+  __syncthreads();
+  block_tree_sum(red);
+  if (t < 32) J.sparts[32 * (((u64)pw * (s.lbits + 1) + q) * s.sumg + g) + t] = red[t];
+}
+
+// one work-group per job (blockDim.x == 256); the host guarantees
+// n_windows * (lbits + 2) * sumg <= 256
+__device__ __forceinline__ void final_body(const MsmJob& J, u32* red) {
+  const MsmShape& s = J.s;
+  const u32 t = threadIdx.x;
+  __syncthreads();
   if (s.N == 0) {
-    for (int i = 0; i < 16; i++) J.result[i] = 0;
+    if (t < 16) J.result[t] = 0;
     return;
   }
-  G1Xyzz acc = load_xyzz(J.wsum + 32 * (s.n_windows - 1));
+  const u32 nv = s.lbits + 2, G = s.sumg, nval = s.n_windows * nv;
+  // value (pw, i) at red[(pw * nv + i) * G], its sum partials in the next G - 1 slots
+  if (t < nval * G) {
+    const u32 gi = t % G, i = (t / G) % nv, pw = t / (G * nv);
+    G1Xyzz v = G1Xyzz::identity();
+    if (i != 1) {
+      const u32 q = i == 0 ? 0 : i - 1;
+      v = load_xyzz(J.sparts + 32 * (((u64)pw * (s.lbits + 1) + q) * G + gi));
+    }
+    store_xyzz(red + 32 * t, v);
+  }
+  __syncthreads();
+  for (u32 d = 1; d < G; d <<= 1) {
+    if (t < nval * G && (t % (2 * d)) == 0) {
+      G1Xyzz a = load_xyzz(red + 32 * t), b = load_xyzz(red + 32 * (t + d));
+      store_xyzz(red + 32 * t, xyzz_add(a, b));
+    }
+    __syncthreads();
+  }
+  // doubling tree per window: a_i += 2^h a_(i+h) for i = 0 mod 2h
+  for (u32 h = 1; h < nv; h <<= 1) {
+    if (t < nval) {
+      const u32 i = t % nv;
+      if ((i % (2 * h)) == 0 && i + h < nv) {
+        G1Xyzz b = load_xyzz(red + 32 * ((t + h) * G));
+        for (u32 k = 0; k < h; k++) b = xyzz_dbl(b);
+        store_xyzz(red + 32 * (t * G), xyzz_add(load_xyzz(red + 32 * (t * G)), b));
+      }
+    }
+    __syncthreads();
+  }
+  if (t != 0) return;
+  G1Xyzz acc = load_xyzz(red + 32 * ((s.n_windows - 1) * nv * G));
   for (int pw = (int)s.n_windows - 2; pw >= 0; pw--) {
     for (u32 k = 0; k < s.c; k++) acc = xyzz_dbl(acc);
-    acc = xyzz_add(acc, load_xyzz(J.wsum + 32 * pw));
+    acc = xyzz_add(acc, load_xyzz(red + 32 * ((u32)pw * nv * G)));
   }
   store_aff(J.result, xyzz_to_aff(acc));
 }
@@ -619,7 +702,7 @@ __global__ __launch_bounds__(256) void kb_digits(const MsmJob* jobs, u32 njobs, 
 }
 __global__ __launch_bounds__(256) void kb_hist(const MsmJob* jobs, u32 njobs, const u32* prefix,
                                                u32 pass) {
-  __shared__ u32 cnt[16];
+  __shared__ u32 cnt[kRadix];
   IMP_BATCH_LOOP(hist_body(J, pass, lb, cnt))
 }
 __global__ __launch_bounds__(256) void kb_scan_a(const MsmJob* jobs, u32 njobs,
@@ -662,8 +745,9 @@ __global__ __launch_bounds__(256) void kb_sum(const MsmJob* jobs, u32 njobs, con
   __shared__ u32 red[256 * 32];
   IMP_BATCH_LOOP(sum_body(J, lb, red))
 }
-__global__ __launch_bounds__(64) void kb_final(const MsmJob* jobs, u32 njobs, const u32* prefix) {
-  IMP_BATCH_LOOP(final_body(J))
+__global__ __launch_bounds__(256) void kb_final(const MsmJob* jobs, u32 njobs, const u32* prefix) {
+  __shared__ u32 red[256 * 32];
+  IMP_BATCH_LOOP(final_body(J, red))
 }
 
 // table t = 2^(t * step) * table 0 (affine), t = 1 .. n_pre-1
@@ -707,7 +791,7 @@ static u32 stage_blocks(const MsmShape& s, u32 kind, u32 arg) {
     case MS_COMBINE:
       return arg <= s.levels ? (u32)cdiv((u64)s.n_windows * combine_cap(s, arg), 256) : 0;
     case MS_REDUCE: return (u32)cdiv((u64)s.n_windows * s.nseg, 256);
-    case MS_SUM: return s.n_windows;
+    case MS_SUM: return s.n_windows * (s.lbits + 1) * s.sumg;
     default: return 1;
   }
 }
@@ -783,7 +867,7 @@ static hipError_t launch_stages(const MsmJob* jobs, u32 njobs, const u32* prefix
       case MS_COMBINE: hipLaunchKernelGGL(kb_combine, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf, st[k].arg); break;
       case MS_REDUCE: hipLaunchKernelGGL(kb_reduce, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
       case MS_SUM: hipLaunchKernelGGL(kb_sum, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
-      default: hipLaunchKernelGGL(kb_final, dim3(grid), dim3(64), 0, stream, jobs, njobs, pf); break;
+      default: hipLaunchKernelGGL(kb_final, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
     }
     IMP_CHECK(hipGetLastError());
   }
@@ -817,7 +901,9 @@ static hipError_t msm_run(const u32* bases, u64 tstride, const u32* scalars, con
 
 static size_t msm_ws_total(const MsmShape& s) { return desc_bytes(1) + msm_layout(s).total; }
 
-static bool valid_window(u32 c) { return c >= 4 && c <= 20; }
+// c in [4, 20]; the final work-group holds lbits + 2 = c - 1 values for each of
+// up to W = 254 / c + 1 windows, at most 255 over that range
+static bool valid_window(u32 c) { return c >= 4 && c <= 20 && (254 / c + 1) * (c - 1) <= 256; }
 
 static hipError_t precompute_device(u32* tables, u64 N, const MsmShape& s, hipStream_t stream) {
   if (s.n_pre <= 1 || N == 0) return hipSuccess;

@@ -61,14 +61,17 @@ def main():
 
     # cost model: trained on everything for the picks used downstream, and
     # leave-one-problem-out for the regret report
+    t1 = time.time()
     model = A.fit_model(db, stream, steps=args.steps)
+    print(f"[tune] cost model trained on {len(db.rows)} rows, {time.time() - t1:.1f}s", flush=True)
     for prob in db.problems():
         choice = A.pick(model, db.for_problem(prob), stream)
         picks[prob]["model"] = choice["config"]
         picks[prob]["model_ms"] = choice["ms"]
     with open(os.path.join(args.out, "picks.json"), "w") as f:
         json.dump(picks, f, indent=1)
-    report = A.regret_report(db, stream, steps=args.steps, holdout=True)
+    report = A.regret_report(db, stream, steps=args.steps, holdout=True,
+                             log=lambda m: print(m, flush=True))
     regrets = [v["model_regret"] for v in report.values()]
     dregrets = [v["default_regret"] for v in report.values() if v["default_regret"] is not None]
     summary = {

@@ -29,7 +29,10 @@ const size_t kStack = 512 * 1024;
 struct Fiber {
   ucontext_t ctx;
   char* stack = nullptr;
-  int state = 0;  // 0 runnable, 1 waiting at the barrier, 2 returned
+  int state = 0;  // 0 runnable, 1 waiting at the barrier, 2 returned, 3 in a wave collective
+  int wkind = 0;  // collective: 1 ballot, 2 shuffle
+  unsigned long long wval = 0, wres = 0;
+  int warg = 0;
 #ifdef EMU_ASAN
   void* fake = nullptr;
 #endif
@@ -100,6 +103,37 @@ void run_group(unsigned nthreads) {
       __sanitizer_finish_switch_fiber(g_sched_fake, nullptr, nullptr);
 #endif
     }
+    // wave collectives: complete once every lane of the wave that has not
+    // returned is in one
+    bool released = false;
+    for (unsigned w0 = 0; w0 < nthreads; w0 += 64) {
+      unsigned w1 = w0 + 64 < nthreads ? w0 + 64 : nthreads;
+      unsigned in_op = 0, done = 0, at_bar = 0;
+      for (unsigned t = w0; t < w1; t++) {
+        in_op += g_fibers[t].state == 3;
+        done += g_fibers[t].state == 2;
+        at_bar += g_fibers[t].state == 1;
+      }
+      if (in_op == 0) continue;
+      if (at_bar || in_op + done != w1 - w0) {
+        fprintf(stderr, "[emu] divergent wave collective in block %u, wave %u\n", blockIdx_.x,
+                w0 / 64);
+        abort();
+      }
+      unsigned long long mask = 0;
+      for (unsigned t = w0; t < w1; t++)
+        if (g_fibers[t].state == 3 && g_fibers[t].wkind == 1 && g_fibers[t].wval)
+          mask |= 1ull << (t - w0);
+      for (unsigned t = w0; t < w1; t++) {
+        Fiber& f = g_fibers[t];
+        if (f.state != 3) continue;
+        if (f.wkind == 1) f.wres = mask;
+        else f.wres = g_fibers[w0 + ((unsigned)f.warg & 63)].wval;
+        f.state = 0;
+      }
+      released = true;
+    }
+    if (released) continue;
     unsigned waiting = 0;
     for (unsigned t = 0; t < nthreads; t++) waiting += g_fibers[t].state == 1;
     if (waiting == 0) break;
@@ -142,6 +176,20 @@ void syncthreads() {
 }
 
 void* dyn_lds() { return g_lds; }
+
+static unsigned long long wave_op(int kind, unsigned long long v, int arg) {
+  Fiber& f = g_fibers[g_cur];
+  f.wkind = kind;
+  f.wval = v;
+  f.warg = arg;
+  f.state = 3;
+  to_sched(false);
+  return g_fibers[g_cur].wres;
+}
+
+unsigned long long wave_ballot(int pred) { return wave_op(1, pred ? 1 : 0, 0); }
+unsigned long long wave_shfl(unsigned long long v, int src) { return wave_op(2, v, src); }
+unsigned lane_id() { return (unsigned)g_cur & 63u; }
 
 void launch_impl(dim3 grid, dim3 block, size_t shmem, std::function<void()> body) {
   if (g_capturing) {

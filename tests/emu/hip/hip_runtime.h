@@ -60,6 +60,10 @@ typedef struct ihipGraphNode_t* hipGraphNode_t;
 namespace emu {
 extern dim3 threadIdx_, blockIdx_, blockDim_, gridDim_;
 void syncthreads();
+// wave-wide collectives: every lane of the wave that has not returned takes part
+unsigned long long wave_ballot(int pred);
+unsigned long long wave_shfl(unsigned long long v, int src);
+unsigned lane_id();
 void* dyn_lds();
 // runs body for every work-item of the grid; while a stream capture is open
 // the launch is recorded instead (the body owns copies of its arguments)
@@ -84,6 +88,16 @@ inline void launch(void (*k)(P...), dim3 g, dim3 b, size_t sh, hipStream_t, A&&.
 inline unsigned atomicAdd(unsigned* p, unsigned v) {
   return __atomic_fetch_add(p, v, __ATOMIC_RELAXED);
 }
+inline unsigned long long __ballot(int pred) { return ::emu::wave_ballot(pred); }
+inline unsigned __lane_id() { return ::emu::lane_id(); }
+inline int __shfl(int v, int src, int = 64) {
+  return (int)(unsigned)::emu::wave_shfl((unsigned)v, src);
+}
+inline unsigned __shfl(unsigned v, int src, int = 64) {
+  return (unsigned)::emu::wave_shfl(v, src);
+}
+inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
+inline unsigned __popcll(unsigned long long x) { return (unsigned)__builtin_popcountll(x); }
 
 hipError_t hipMalloc(void** p, size_t n);
 hipError_t hipFree(void* p);

@@ -303,6 +303,46 @@ def msm_hot_bucket(E):
         assert (out == O.msm_closed_form(s, k)).all(), (c, task)
 
 
+def field_inverse(E):
+    """Inversion in Fr and Fq (binary extended Euclid) against the oracle,
+    including 0, 1, m - 1 and powers of two, then affine conversion."""
+    for f, m in ((O.FR, O.R_MOD), (O.FQ, O.Q_MOD)):
+        vals = [0, 1, 2, m - 1, m - 2, (m + 1) // 2, 1 << 200, (1 << 253) % m]
+        rng = random.Random(5 + f)
+        vals += [rng.randrange(m) for _ in range(56)]
+        a = O.ints_to_array([v % m for v in vals])
+        out = u32(a.size)
+        E.call("impeller_field_op", f, 7, ptr(out), ptr(flat(a)), None, len(vals), None)
+        exp = np.stack([O.inv(a[i], f) for i in range(len(vals))])
+        assert (out.reshape(-1, 8) == exp).all(), f
+
+
+def mlp_grad(E):
+    """Cost-model MLP forward and gradient kernels against the float64
+    restatement, at a batch larger than one work-group's stride."""
+    from oracle import mlp_ref
+    rng = np.random.default_rng(7)
+    F, H = 9, 64
+    for B in (37, 1000):
+        n = int(E.call("impeller_mlp_param_count", F, H))
+        p = rng.normal(0, 0.3, n).astype(np.float32)
+        X = rng.normal(0, 1, (B, F)).astype(np.float32)
+        T = rng.normal(0, 1, B).astype(np.float32)
+        ws = np.zeros(int(E.call("impeller_mlp_workspace_floats", F, H, B)), np.float32)
+        Y = np.zeros(B, np.float32)
+        E.call("impeller_mlp_forward", ptr(p), ptr(X), B, F, H, ptr(Y), ptr(ws), None)
+        Yref, _ = mlp_ref.forward(p.astype(np.float64), X, F, H)
+        assert np.allclose(Y, Yref, rtol=1e-4, atol=1e-5), B
+        g = np.zeros(n, np.float32)
+        loss = np.zeros(1, np.float32)
+        E.call("impeller_mlp_grad", ptr(p), ptr(X), ptr(T), B, F, H, ptr(g), ptr(loss), ptr(ws),
+               None)
+        gref = mlp_ref.grad(p.astype(np.float64), X, T, F, H)
+        assert np.allclose(g, gref, rtol=1e-3, atol=1e-5), B
+        assert np.isclose(loss[0] / B, mlp_ref.loss(p.astype(np.float64), X, T, F, H),
+                          rtol=1e-4), B
+
+
 def gate(E):
     for n in (1, 777, 4096):
         ins = [O.gen_scalars(n, 40 + i) for i in range(6)]
@@ -315,7 +355,7 @@ def gate(E):
 
 ALL = [("single_ntt", single_ntt), ("ntt_plans", ntt_plans), ("batched_ntt_mixed", batched_ntt_mixed),
        ("batched_ntt_plans", batched_ntt_plans), ("ntt_graph_replay", ntt_graph_replay),
-       ("gate", gate), ("single_msm", single_msm), ("msm_hot_bucket", msm_hot_bucket), ("msm_graph_replay", msm_graph_replay),
+       ("gate", gate), ("field_inverse", field_inverse), ("mlp_grad", mlp_grad), ("single_msm", single_msm), ("msm_hot_bucket", msm_hot_bucket), ("msm_graph_replay", msm_graph_replay),
        ("batched_msm_shared_tables", batched_msm_shared_tables)]
 
 
