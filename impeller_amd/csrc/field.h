@@ -649,8 +649,10 @@ __device__ __forceinline__ G1Aff aff_neg(const G1Aff& a) {
   return r;
 }
 
-// z-scaling to affine; identity -> (0, 0)
-__device__ __forceinline__ G1Aff xyzz_to_aff(const G1Xyzz& p) {
+// z-scaling to affine; identity -> (0, 0).  all_lanes: every lane of the wave
+// converts (Fermat's fixed exponent chain avoids the divergence of the binary
+// inversion, which is faster for a single lane)
+__device__ __forceinline__ G1Aff xyzz_to_aff(const G1Xyzz& p, bool all_lanes = false) {
   typedef FqParams Q;
   G1Aff r;
   if (p.is_identity()) {
@@ -658,7 +660,7 @@ __device__ __forceinline__ G1Aff xyzz_to_aff(const G1Xyzz& p) {
     r.y = Fq::zero();
     return r;
   }
-  Fq a = inv<Q>(p.zzz);
+  Fq a = all_lanes ? inv_fermat<Q>(p.zzz) : inv<Q>(p.zzz);
   Fq b = sqr<Q>(mul<Q>(p.zz, a));
   r.x = mul<Q>(p.x, b);
   r.y = mul<Q>(p.y, a);

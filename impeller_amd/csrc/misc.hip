@@ -1,6 +1,7 @@
 // misc.hip -- fused gate kernel, seeded input generators and the small
 // device-arithmetic entry points used to test the field and curve code.
 #include "common.h"
+#include "field29.h"
 #include "timing.h"
 
 namespace imp {
@@ -62,11 +63,11 @@ __global__ void k_gen_bases(u32* out, u64 n, u64 seed, u64 start) {
   g.y = Fq::zero();
   g.y.v[0] = 2;
   g.y = from_number<FqParams>(g.y);
-  store_aff(out + 16 * t, xyzz_to_aff(scalar_mul(g, k.v)));
+  store_aff(out + 16 * t, xyzz_to_aff(scalar_mul(g, k.v), true));
 }
 
 // ------------------------------------------------------- arithmetic probes
-template <class P>
+template <class P, class P29>
 __global__ void k_field_op(int op, u32* r, const u32* a, const u32* b, u64 n) {
   u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -82,7 +83,14 @@ __global__ void k_field_op(int op, u32* r, const u32* a, const u32* b, u64 n) {
     case 4: z = from_number<P>(x); break;
     case 5: z = to_number<P>(x); break;
     case 6: z = pow_big<P>(x, y.v); break;
-    default: z = inv<P>(x); break;
+    case 7: z = inv<P>(x); break;
+    // the 9 x 29-bit forms, entered and left through the R-form
+    case 8: z = from_r29<P29>(mul29<P29>(to_r29<P29>(x), to_r29<P29>(y))); break;
+    case 9: z = from_r29<P29>(sub29<P29, 2>(to_r29<P29>(x), to_r29<P29>(y))); break;
+    default: {
+      F29<P29> ty = to_r29<P29>(y);
+      z = from_r29<P29>(sub3_29<P29, 6>(to_r29<P29>(x), ty, ty, ty));
+    }
   }
   store_fe<F>(r + 8 * i, z);
 }
@@ -95,6 +103,31 @@ __global__ void k_g1_op(int op, u32* r, const u32* a, const u32* b, u64 n) {
     case 1: store_xyzz(r + 32 * i, xyzz_add(load_xyzz(a + 32 * i), load_xyzz(b + 32 * i))); break;
     case 2: store_xyzz(r + 32 * i, xyzz_dbl(load_xyzz(a + 32 * i))); break;
     case 3: store_aff(r + 16 * i, xyzz_to_aff(load_xyzz(a + 32 * i))); break;
+    // the 29-bit forms, entered and left through the R-form XYZZ
+    case 5: {
+      G1Aff q = load_aff(b + 16 * i);
+      G1Aff29 q29;
+      q29.x = to_r29<Q29>(q.x);
+      q29.y = to_r29<Q29>(q.y);
+      store_xyzz(r + 32 * i, xyzz_from_29(xyzz29_madd(xyzz_to_29(load_xyzz(a + 32 * i)), q29)));
+      break;
+    }
+    case 6:
+      store_xyzz(r + 32 * i, xyzz_from_29(xyzz29_add(xyzz_to_29(load_xyzz(a + 32 * i)),
+                                                     xyzz_to_29(load_xyzz(b + 32 * i)))));
+      break;
+    case 7: store_xyzz(r + 32 * i, xyzz_from_29(xyzz29_dbl(xyzz_to_29(load_xyzz(a + 32 * i))))); break;
+    case 8: {
+      // table format round trip, then -P through the table path
+      u32 tab[16];
+      aff_to_table(tab, load_aff(a + 16 * i));
+      G1Aff29 t = aff29_neg(load_aff_table(tab));
+      G1Aff o;
+      o.x = from_r29<Q29>(t.x);
+      o.y = from_r29<Q29>(t.y);
+      store_aff(r + 16 * i, o);
+      break;
+    }
     default: {
       Fr k = load_fe<Fr>(b + 8 * i);
       store_aff(r + 16 * i, xyzz_to_aff(scalar_mul(load_aff(a + 16 * i), k.v)));
@@ -190,16 +223,16 @@ extern "C" hipError_t impeller_field_op(int field, int op, uint32_t* r_d, const 
                                         const uint32_t* b_d, unsigned long long n,
                                         hipStream_t stream) {
   IMP_REQUIRE(r_d && a_d, "null pointer");
-  IMP_REQUIRE(op >= 0 && op <= 7 && (field == 0 || field == 1), "op/field");
+  IMP_REQUIRE(op >= 0 && op <= 10 && (field == 0 || field == 1), "op/field");
   IMP_REQUIRE(b_d || op == 3 || op == 4 || op == 5 || op == 7, "b operand");
   if (n == 0) return hipSuccess;
   dim3 grid((u32)((n + 127) / 128));
   if (field == 0)
-    hipLaunchKernelGGL(k_field_op<FrParams>, grid, dim3(128), 0, stream, op, r_d, a_d, b_d,
-                       (u64)n);
+    hipLaunchKernelGGL((k_field_op<FrParams, Fr29Params>), grid, dim3(128), 0, stream, op, r_d,
+                       a_d, b_d, (u64)n);
   else
-    hipLaunchKernelGGL(k_field_op<FqParams>, grid, dim3(128), 0, stream, op, r_d, a_d, b_d,
-                       (u64)n);
+    hipLaunchKernelGGL((k_field_op<FqParams, Fq29Params>), grid, dim3(128), 0, stream, op, r_d,
+                       a_d, b_d, (u64)n);
   return hipGetLastError();
 }
 
@@ -207,8 +240,8 @@ extern "C" hipError_t impeller_g1_op(int op, uint32_t* r_d, const uint32_t* a_d,
                                      const uint32_t* b_d, unsigned long long n,
                                      hipStream_t stream) {
   IMP_REQUIRE(r_d && a_d, "null pointer");
-  IMP_REQUIRE(op >= 0 && op <= 4, "op");
-  IMP_REQUIRE(b_d || op == 2 || op == 3, "b operand");
+  IMP_REQUIRE(op >= 0 && op <= 8, "op");
+  IMP_REQUIRE(b_d || op == 2 || op == 3 || op == 7 || op == 8, "b operand");
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_g1_op, dim3((u32)((n + 63) / 64)), dim3(64), 0, stream, op, r_d, a_d,
                      b_d, (u64)n);

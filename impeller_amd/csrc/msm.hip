@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "common.h"
+#include "field29.h"
 #include "timing.h"
 
 namespace imp {
@@ -60,6 +61,7 @@ struct MsmShape {
 };
 
 static const u32 kSortTile = 4096;  // 256 threads x 16 keys
+static const u32 kPt = 36;           // u32 per stored XYZZ point (4 x 9 limbs of 29 bits)
 static const u32 kRadixBits = 8;     // digit bits per sort pass
 static const u32 kRadix = 1u << kRadixBits;
 
@@ -98,7 +100,7 @@ static MsmShape msm_shape(u64 N, u32 c, u32 target_windows, u32 task = 64) {
 
 struct MsmLayout {
   size_t keysA, valsA, keysB, valsB, hist, bstart, bend, csum, toff, loff, partA, partB, rparts,
-      sparts, total;
+      sparts, conv, total;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -117,10 +119,12 @@ static MsmLayout msm_layout(const MsmShape& s) {
   l.csum = o; o = align256(o + (size_t)s.n_windows * (s.chunks + 1) * 4);
   l.toff = o; o = align256(o + (size_t)s.n_windows * (s.nb + 2) * 4);
   l.loff = o; o = align256(o + (size_t)s.n_windows * (s.nb + 2) * 4);
-  l.partA = o; o = align256(o + (size_t)s.n_windows * s.task_cap * 128);
-  l.partB = o; o = align256(o + (size_t)s.n_windows * s.task_cap * 128);
-  l.rparts = o; o = align256(o + (size_t)s.n_windows * 2 * s.nseg * 128);
-  l.sparts = o; o = align256(o + (size_t)s.n_windows * (s.lbits + 1) * s.sumg * 128);
+  l.partA = o; o = align256(o + (size_t)s.n_windows * s.task_cap * kPt * 4);
+  l.partB = o; o = align256(o + (size_t)s.n_windows * s.task_cap * kPt * 4);
+  l.rparts = o; o = align256(o + (size_t)s.n_windows * 2 * s.nseg * kPt * 4);
+  l.sparts = o; o = align256(o + (size_t)s.n_windows * (s.lbits + 1) * s.sumg * kPt * 4);
+  // one table (the plain bases, affine R-form) is converted per call
+  l.conv = o; o = align256(o + (s.n_pre == 1 ? (size_t)s.N * 64 : 0));
   l.total = o;
   return l;
 }
@@ -131,7 +135,8 @@ static MsmLayout msm_layout(const MsmShape& s) {
 struct MsmJob {
   MsmShape s;
   u64 tstride;  // points between consecutive tables (>= N)
-  const u32* bases;
+  const u32* bases;  // point tables in the table format
+  const u32* raw;    // one table of affine R-form bases to convert into bases, or null
   const u32* scalars;
   u32* result;
   u32 *kA, *vA, *kB, *vB, *hist, *bstart, *bend, *csum, *toff, *loff, *partA, *partB, *rparts,
@@ -145,6 +150,11 @@ static MsmJob make_msm_job(const MsmShape& s, u64 tstride, const u32* bases, con
   J.s = s;
   J.tstride = tstride;
   J.bases = bases;
+  J.raw = nullptr;
+  if (s.n_pre == 1) {
+    J.raw = bases;
+    J.bases = (const u32*)(ws + l.conv);
+  }
   J.scalars = scalars;
   J.result = result;
   J.kA = (u32*)(ws + l.keysA);
@@ -213,6 +223,12 @@ __device__ __forceinline__ void digits_body(const MsmJob& J, u64 i) {
   }
 }
 
+// one affine R-form base -> the table format
+__device__ __forceinline__ void convert_body(const MsmJob& J, u64 i) {
+  if (i >= J.s.N || J.raw == nullptr) return;
+  aff_to_table(const_cast<u32*>(J.bases) + 16 * i, load_aff(J.raw + 16 * i));
+}
+
 // LSD radix sort, 8-bit digits, stable per pass.  Tile histograms go to
 // hist[(pw * 256 + digit) * tiles + tile]; their exclusive scan in that order
 // is each (digit, tile) run's global start.
@@ -278,13 +294,19 @@ __device__ __forceinline__ u32 block_excl_scan(u32 v, u32* part, u32* total) {
 }
 
 struct ScatterLds {
-  u32 cnt[16 * 256];  // per (nibble, thread) counts, then running ranks
+  u32 cnt[16 * 256];  // per (nibble, thread) counts, then running ranks (swizzled)
   u32 part[256];
   u32 gbase[kRadix];  // global start of each digit's run of this tile
   u32 dbase[kRadix];  // first staged position of each digit
-  u32 sk[kSortTile];  // the tile in (digit, input order): staged for coalesced writes
-  u32 sv[kSortTile];
+  u32 sk[kSortTile];  // the tile in (digit, input order), swizzled: staged for
+  u32 sv[kSortTile];  // coalesced writes
 };
+
+// LDS word swizzle for the 4096-word arrays: thread t's run of 16 words
+// [16t, 16t + 16) lands in 64 distinct banks across a wave (the low four
+// address bits are XORed with bits 6..9), while 64 consecutive words stay in
+// 64 distinct banks
+__device__ __forceinline__ u32 swz(u32 i) { return i ^ ((i >> 6) & 15u); }
 
 // Stable split of the tile by the 4-bit field at bit sh.  Thread t holds tile
 // positions 16t .. 16t + n - 1 in key/val; ranks come from per-thread field
@@ -293,35 +315,35 @@ __device__ __forceinline__ void split_nibble(ScatterLds& L, const u32 key[16], c
                                              u32 n, u32 sh) {
   const u32 t = threadIdx.x;
 #pragma unroll
-  for (u32 d = 0; d < 16; d++) L.cnt[d * 256 + t] = 0;
+  for (u32 d = 0; d < 16; d++) L.cnt[swz(d * 256 + t)] = 0;
 #pragma unroll
   for (u32 j = 0; j < 16; j++)
-    if (j < n) L.cnt[((key[j] >> sh) & 15u) * 256 + t] += 1;
+    if (j < n) L.cnt[swz(((key[j] >> sh) & 15u) * 256 + t)] += 1;
   __syncthreads();
   // thread t scans the flattened counts [16t, 16t + 16)
   u32 loc[16];
   u32 sum = 0;
 #pragma unroll
   for (u32 j = 0; j < 16; j++) {
-    loc[j] = L.cnt[t * 16 + j];
+    loc[j] = L.cnt[swz(t * 16 + j)];
     sum += loc[j];
   }
   u32 total;
   u32 run = block_excl_scan(sum, L.part, &total);
 #pragma unroll
   for (u32 j = 0; j < 16; j++) {
-    L.cnt[t * 16 + j] = run;
+    L.cnt[swz(t * 16 + j)] = run;
     run += loc[j];
   }
   __syncthreads();
 #pragma unroll
   for (u32 j = 0; j < 16; j++) {
     if (j < n) {
-      u32 d = (key[j] >> sh) & 15u;
-      u32 r = L.cnt[d * 256 + t];
-      L.cnt[d * 256 + t] = r + 1;
-      L.sk[r] = key[j];
-      L.sv[r] = val[j];
+      const u32 c = swz(((key[j] >> sh) & 15u) * 256 + t);
+      const u32 r = L.cnt[c];
+      L.cnt[c] = r + 1;
+      L.sk[swz(r)] = key[j];
+      L.sv[swz(r)] = val[j];
     }
   }
   __syncthreads();
@@ -356,8 +378,8 @@ __device__ __forceinline__ void scatter_body(const MsmJob& J, u32 pass, u64 lb, 
 #pragma unroll
   for (u32 j = 0; j < 16; j++) {
     if (j < n) {
-      key[j] = L.sk[16 * t + j];
-      val[j] = L.sv[16 * t + j];
+      key[j] = L.sk[swz(16 * t + j)];
+      val[j] = L.sv[swz(16 * t + j)];
     }
   }
   __syncthreads();
@@ -366,16 +388,16 @@ __device__ __forceinline__ void scatter_body(const MsmJob& J, u32 pass, u64 lb, 
   L.gbase[t] = J.hist[((u64)pw * kRadix + t) * s.tiles + tile];
   for (u32 j = 0; j < n; j++) {
     const u32 i = 16 * t + j;
-    const u32 d = (L.sk[i] >> shift) & (kRadix - 1);
-    if (i == 0 || ((L.sk[i - 1] >> shift) & (kRadix - 1)) != d) L.dbase[d] = i;
+    const u32 d = (L.sk[swz(i)] >> shift) & (kRadix - 1);
+    if (i == 0 || ((L.sk[swz(i - 1)] >> shift) & (kRadix - 1)) != d) L.dbase[d] = i;
   }
   __syncthreads();
   for (u32 i = t; i < n_tile; i += blockDim.x) {
-    const u32 k = L.sk[i];
+    const u32 k = L.sk[swz(i)];
     const u32 d = (k >> shift) & (kRadix - 1);
     const u64 pos = (u64)L.gbase[d] + i - L.dbase[d];
     kdst[seg0 + pos] = k;
-    vdst[seg0 + pos] = L.sv[i];
+    vdst[seg0 + pos] = L.sv[swz(i)];
   }
 }
 
@@ -502,20 +524,20 @@ __device__ __forceinline__ void tasks_body(const MsmJob& J, u64 gi) {
   u32 be = J.bend[(u64)pw * (s.nb + 1) + b];
   u32 hi = lo + s.T < be ? lo + s.T : be;
   const u32* vv = sorted_vals(J) + (u64)pw * s.seg_len;
-  G1Xyzz acc = G1Xyzz::identity();
+  G1Xyzz29 acc = G1Xyzz29::identity();
   u32 e = vv[lo];
-  G1Aff p = load_aff(J.bases + 16 * (u64)(e & 0x7fffffffu));
+  G1Aff29 p = load_aff_table(J.bases + 16 * (u64)(e & 0x7fffffffu));
   for (u32 j = lo; j < hi; j++) {
-    G1Aff cur = p;
+    G1Aff29 cur = p;
     u32 ce = e;
     if (j + 1 < hi) {
       e = vv[j + 1];
-      p = load_aff(J.bases + 16 * (u64)(e & 0x7fffffffu));
+      p = load_aff_table(J.bases + 16 * (u64)(e & 0x7fffffffu));
     }
-    if (ce & 0x80000000u) cur = aff_neg(cur);
-    acc = xyzz_madd(acc, cur);
+    if (ce & 0x80000000u) cur = aff29_neg(cur);
+    acc = xyzz29_madd(acc, cur);
   }
-  store_xyzz(J.partA + 32 * ((u64)pw * s.task_cap + t), acc);
+  store_xyzz29(J.partA + kPt * ((u64)pw * s.task_cap + t), acc);
 }
 
 // level l of the per-bucket partial-sum tree: compact index o of window pw
@@ -545,10 +567,10 @@ __device__ __forceinline__ void combine_body(const MsmJob& J, u32 level, u64 gi)
   u32* dst = (level & 1) ? J.partB : J.partA;
   const u64 base = (u64)pw * s.task_cap + to[b];
   const u32 G = 1u << s.g;
-  G1Xyzz acc = load_xyzz(src + 32 * (base + (u64)k * G));
+  G1Xyzz29 acc = load_xyzz29(src + kPt * (base + (u64)k * G));
   for (u32 j = 1; j < G && k * G + j < n_prev; j++)
-    acc = xyzz_add(acc, load_xyzz(src + 32 * (base + (u64)k * G + j)));
-  store_xyzz(dst + 32 * (base + k), acc);
+    acc = xyzz29_add(acc, load_xyzz29(src + kPt * (base + (u64)k * G + j)));
+  store_xyzz29(dst + kPt * (base + k), acc);
 }
 
 // Window sum S = sum_b b * B_b (b = 1 .. nb) in three shallow stages.
@@ -567,18 +589,18 @@ __device__ __forceinline__ void reduce_body(const MsmJob& J, u64 t) {
   const u32 lo = sg * s.seg;
   const u32* to = J.toff + (u64)pw * (s.nb + 2);
   const u64 pbase = (u64)pw * s.task_cap;
-  G1Xyzz run = G1Xyzz::identity(), tot = G1Xyzz::identity();
+  G1Xyzz29 run = G1Xyzz29::identity(), tot = G1Xyzz29::identity();
   for (int k = (int)s.seg - 1; k >= 0; k--) {
     const u32 b = lo + k + 1;
     const u32 nt = to[b + 1] - to[b];
     if (nt) {
       const u32* pp = (bucket_levels(nt, s.g) & 1) ? J.partB : J.partA;
-      run = xyzz_add(run, load_xyzz(pp + 32 * (pbase + to[b])));
+      run = xyzz29_add(run, load_xyzz29(pp + kPt * (pbase + to[b])));
     }
-    tot = xyzz_add(tot, run);
+    tot = xyzz29_add(tot, run);
   }
-  store_xyzz(J.rparts + 32 * ((u64)(2 * pw) * s.nseg + sg), tot);
-  store_xyzz(J.rparts + 32 * ((u64)(2 * pw + 1) * s.nseg + sg), run);
+  store_xyzz29(J.rparts + kPt * ((u64)(2 * pw) * s.nseg + sg), tot);
+  store_xyzz29(J.rparts + kPt * ((u64)(2 * pw + 1) * s.nseg + sg), run);
 }
 
 // pairwise tree over the 256 XYZZ values red[0..255] with the given stride
@@ -587,8 +609,8 @@ __device__ __forceinline__ void block_tree_sum(u32* red) {
   const u32 t = threadIdx.x;
   for (u32 d = 128; d > 0; d >>= 1) {
     if (t < d) {
-      G1Xyzz a = load_xyzz(red + 32 * t), b = load_xyzz(red + 32 * (t + d));
-      store_xyzz(red + 32 * t, xyzz_add(a, b));
+      G1Xyzz29 a = load_xyzz29(red + kPt * t), b = load_xyzz29(red + kPt * (t + d));
+      store_xyzz29(red + kPt * t, xyzz29_add(a, b));
     }
     __syncthreads();
   }
@@ -602,19 +624,19 @@ __device__ __forceinline__ void sum_body(const MsmJob& J, u64 lb, u32* red) {
   const u32 q = (u32)((lb / s.sumg) % (s.lbits + 1));
   const u32 pw = (u32)(lb / ((u64)s.sumg * (s.lbits + 1)));
   const u32 count = q == 0 ? s.nseg : s.nseg / 2;
-  const u32* src = J.rparts + 32 * ((u64)(2 * pw + (q ? 1 : 0)) * s.nseg);
+  const u32* src = J.rparts + kPt * ((u64)(2 * pw + (q ? 1 : 0)) * s.nseg);
   const u32 l = q ? q - 1 : 0;
-  G1Xyzz acc = G1Xyzz::identity();
+  G1Xyzz29 acc = G1Xyzz29::identity();
   __syncthreads();
   for (u32 i = g * 256 + t; i < count; i += s.sumg * 256) {
     // q > 0: the i-th segment index with bit l set
     const u32 sg = q == 0 ? i : (((i >> l) << (l + 1)) | (1u << l) | (i & ((1u << l) - 1)));
-    acc = xyzz_add(acc, load_xyzz(src + 32 * (u64)sg));
+    acc = xyzz29_add(acc, load_xyzz29(src + kPt * (u64)sg));
   }
-  store_xyzz(red + 32 * t, acc);
+  store_xyzz29(red + kPt * t, acc);
   __syncthreads();
   block_tree_sum(red);
-  if (t < 32) J.sparts[32 * (((u64)pw * (s.lbits + 1) + q) * s.sumg + g) + t] = red[t];
+  if (t < kPt) J.sparts[kPt * (((u64)pw * (s.lbits + 1) + q) * s.sumg + g) + t] = red[t];
 }
 
 // one work-group per job (blockDim.x == 256); the host guarantees
@@ -631,18 +653,18 @@ __device__ __forceinline__ void final_body(const MsmJob& J, u32* red) {
   // value (pw, i) at red[(pw * nv + i) * G], its sum partials in the next G - 1 slots
   if (t < nval * G) {
     const u32 gi = t % G, i = (t / G) % nv, pw = t / (G * nv);
-    G1Xyzz v = G1Xyzz::identity();
+    G1Xyzz29 v = G1Xyzz29::identity();
     if (i != 1) {
       const u32 q = i == 0 ? 0 : i - 1;
-      v = load_xyzz(J.sparts + 32 * (((u64)pw * (s.lbits + 1) + q) * G + gi));
+      v = load_xyzz29(J.sparts + kPt * (((u64)pw * (s.lbits + 1) + q) * G + gi));
     }
-    store_xyzz(red + 32 * t, v);
+    store_xyzz29(red + kPt * t, v);
   }
   __syncthreads();
   for (u32 d = 1; d < G; d <<= 1) {
     if (t < nval * G && (t % (2 * d)) == 0) {
-      G1Xyzz a = load_xyzz(red + 32 * t), b = load_xyzz(red + 32 * (t + d));
-      store_xyzz(red + 32 * t, xyzz_add(a, b));
+      G1Xyzz29 a = load_xyzz29(red + kPt * t), b = load_xyzz29(red + kPt * (t + d));
+      store_xyzz29(red + kPt * t, xyzz29_add(a, b));
     }
     __syncthreads();
   }
@@ -651,20 +673,20 @@ __device__ __forceinline__ void final_body(const MsmJob& J, u32* red) {
     if (t < nval) {
       const u32 i = t % nv;
       if ((i % (2 * h)) == 0 && i + h < nv) {
-        G1Xyzz b = load_xyzz(red + 32 * ((t + h) * G));
-        for (u32 k = 0; k < h; k++) b = xyzz_dbl(b);
-        store_xyzz(red + 32 * (t * G), xyzz_add(load_xyzz(red + 32 * (t * G)), b));
+        G1Xyzz29 b = load_xyzz29(red + kPt * ((t + h) * G));
+        for (u32 k = 0; k < h; k++) b = xyzz29_dbl(b);
+        store_xyzz29(red + kPt * (t * G), xyzz29_add(load_xyzz29(red + kPt * (t * G)), b));
       }
     }
     __syncthreads();
   }
   if (t != 0) return;
-  G1Xyzz acc = load_xyzz(red + 32 * ((s.n_windows - 1) * nv * G));
+  G1Xyzz29 acc = load_xyzz29(red + kPt * ((s.n_windows - 1) * nv * G));
   for (int pw = (int)s.n_windows - 2; pw >= 0; pw--) {
-    for (u32 k = 0; k < s.c; k++) acc = xyzz_dbl(acc);
-    acc = xyzz_add(acc, load_xyzz(red + 32 * ((u32)pw * nv * G)));
+    for (u32 k = 0; k < s.c; k++) acc = xyzz29_dbl(acc);
+    acc = xyzz29_add(acc, load_xyzz29(red + kPt * ((u32)pw * nv * G)));
   }
-  store_aff(J.result, xyzz_to_aff(acc));
+  store_aff(J.result, xyzz_to_aff(xyzz_from_29(acc)));
 }
 
 // ------------------------------------------------------- batched kernels
@@ -683,7 +705,7 @@ __device__ __forceinline__ u32 find_job(const u32* prefix, u32 njobs, u32 t) {
 
 enum MsmStage : u32 {
   MS_DIGITS = 0, MS_HIST, MS_SCAN_A, MS_SCAN_B, MS_SCAN_C, MS_SCATTER, MS_ZERO, MS_BOUNDS,
-  MS_TASKS, MS_COMBINE, MS_REDUCE, MS_SUM, MS_FINAL
+  MS_TASKS, MS_COMBINE, MS_REDUCE, MS_SUM, MS_FINAL, MS_CONVERT
 };
 // scan stage argument: ScanKind | (pass or level) << 8
 static inline u32 scan_arg(u32 kind, u32 x) { return kind | (x << 8); }
@@ -697,6 +719,9 @@ static inline u32 scan_arg(u32 kind, u32 x) { return kind | (x << 8); }
     BODY;                                                           \
   }
 
+__global__ __launch_bounds__(256) void kb_convert(const MsmJob* jobs, u32 njobs, const u32* prefix) {
+  IMP_BATCH_LOOP(convert_body(J, lb * blockDim.x + threadIdx.x))
+}
 __global__ __launch_bounds__(256) void kb_digits(const MsmJob* jobs, u32 njobs, const u32* prefix) {
   IMP_BATCH_LOOP(digits_body(J, lb * blockDim.x + threadIdx.x))
 }
@@ -742,11 +767,11 @@ __global__ __launch_bounds__(256) void kb_reduce(const MsmJob* jobs, u32 njobs, 
   IMP_BATCH_LOOP(reduce_body(J, lb * blockDim.x + threadIdx.x))
 }
 __global__ __launch_bounds__(256) void kb_sum(const MsmJob* jobs, u32 njobs, const u32* prefix) {
-  __shared__ u32 red[256 * 32];
+  __shared__ u32 red[256 * kPt];
   IMP_BATCH_LOOP(sum_body(J, lb, red))
 }
 __global__ __launch_bounds__(256) void kb_final(const MsmJob* jobs, u32 njobs, const u32* prefix) {
-  __shared__ u32 red[256 * 32];
+  __shared__ u32 red[256 * kPt];
   IMP_BATCH_LOOP(final_body(J, red))
 }
 
@@ -758,8 +783,23 @@ __global__ void k_msm_precompute(u32* tables, u64 N, u32 n_pre, u32 step) {
   G1Xyzz acc = aff_to_xyzz(p);
   for (u32 t = 1; t < n_pre; t++) {
     for (u32 k = 0; k < step; k++) acc = xyzz_dbl(acc);
-    store_aff(tables + 16 * ((u64)t * N + i), xyzz_to_aff(acc));
+    store_aff(tables + 16 * ((u64)t * N + i), xyzz_to_aff(acc, true));
   }
+}
+
+// affine R-form points -> table format (in place allowed: each point is read
+// once by the thread that writes it)
+__global__ void k_msm_to_table(u32* dst, const u32* src, u64 count) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  aff_to_table(dst + 16 * i, load_aff(src + 16 * i));
+}
+
+static hipError_t to_table(u32* dst, const u32* src, u64 count, hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_msm_to_table, dim3((u32)((count + 255) / 256)), dim3(256), 0, stream, dst,
+                     src, count);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ plan
@@ -774,6 +814,7 @@ static inline u64 cdiv(u64 a, u64 b) { return (a + b - 1) / b; }
 static u32 stage_blocks(const MsmShape& s, u32 kind, u32 arg) {
   if (s.N == 0) return kind == MS_FINAL ? 1u : 0u;
   switch (kind) {
+    case MS_CONVERT: return s.n_pre == 1 ? (u32)cdiv(s.N, 256) : 0u;
     case MS_DIGITS: return (u32)cdiv(s.N, 256);
     case MS_HIST:
     case MS_SCATTER: return arg < s.passes ? s.tiles * s.n_windows : 0;
@@ -804,6 +845,7 @@ static void plan_stages(const std::vector<MsmShape>& shapes, std::vector<StageLa
     if (s.levels > maxl) maxl = s.levels;
   }
   std::vector<std::pair<u32, u32>> kinds;
+  kinds.push_back({MS_CONVERT, 0});
   kinds.push_back({MS_DIGITS, 0});
   auto scan3 = [&](u32 sk, u32 x) {
     kinds.push_back({MS_SCAN_A, scan_arg(sk, x)});
@@ -843,7 +885,8 @@ static void plan_stages(const std::vector<MsmShape>& shapes, std::vector<StageLa
 // timer of each stage kind (scans count with the sort)
 static const u32 kTimerOf[] = {K_MSM_DIGITS,  K_MSM_SORT,   K_MSM_SORT, K_MSM_SORT, K_MSM_SORT,
                                K_MSM_SORT,    K_MSM_SCAN,   K_MSM_SCAN, K_MSM_BUCKETS,
-                               K_MSM_COMBINE, K_MSM_REDUCE, K_MSM_SUM,  K_MSM_FINAL};
+                               K_MSM_COMBINE, K_MSM_REDUCE, K_MSM_SUM,  K_MSM_FINAL,
+                               K_MSM_DIGITS};
 
 static hipError_t launch_stages(const MsmJob* jobs, u32 njobs, const u32* prefix_dev,
                                 const StageLaunch* st, u32 nst, u32 max_blocks,
@@ -855,6 +898,7 @@ static hipError_t launch_stages(const MsmJob* jobs, u32 njobs, const u32* prefix
     const u32* pf = prefix_dev + (size_t)k * (njobs + 1);
     TimedScope ts(kTimerOf[st[k].kind], stream);
     switch (st[k].kind) {
+      case MS_CONVERT: hipLaunchKernelGGL(kb_convert, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
       case MS_DIGITS: hipLaunchKernelGGL(kb_digits, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
       case MS_HIST: hipLaunchKernelGGL(kb_hist, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf, st[k].arg); break;
       case MS_SCAN_A: hipLaunchKernelGGL(kb_scan_a, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf, st[k].arg); break;
@@ -946,6 +990,7 @@ static hipError_t msm_host(u32 window, u32 target, void* const* buffers,
   for (u32 t = 0; t < s.n_pre; t++)
     IMP_CHECK(hipMemcpyAsync(d_pts + (size_t)16 * t * len, h_points[t], (size_t)len * 64,
                              hipMemcpyHostToDevice, st));
+  if (s.n_pre > 1) IMP_CHECK(to_table(d_pts, d_pts, (u64)s.n_pre * len, st));
   for (u32 b = 0; b < batches; b++) {
     IMP_CHECK(hipMemcpyAsync(d_scl, h_scaler_batch[b], (size_t)len * 32, hipMemcpyHostToDevice,
                              st));
@@ -1052,7 +1097,10 @@ extern "C" hipError_t impeller_msm_precompute(uint32_t* tables_d, unsigned long 
                                               hipStream_t stream) {
   IMP_REQUIRE(valid_window(window), "window");
   IMP_REQUIRE(tables_d || len == 0, "null pointer");
-  return precompute_device(tables_d, len, msm_shape(len, window, target_windows), stream);
+  MsmShape s = msm_shape(len, window, target_windows);
+  if (s.n_pre <= 1) return hipSuccess;  // the plain bases: converted by each MSM call
+  IMP_CHECK(precompute_device(tables_d, len, s, stream));
+  return to_table(tables_d, tables_d, (u64)s.n_pre * len, stream);
 }
 
 extern "C" hipError_t impeller_msm_cfg(const uint32_t* tables_d, const uint32_t* scalars_d,

@@ -298,3 +298,15 @@ def gate(a, b, c, d, q, e):
     out = np.zeros_like(arrs[0])
     lib().orc_gate(_ptr(out), *[_ptr(x) for x in arrs], arrs[0].shape[0])
     return out
+
+
+def to_table_format(points):
+    """Affine points (rows of 16 limbs, Montgomery R = 2^256) in the MSM table
+    format of the HIP library: each coordinate in Montgomery form with
+    R' = 2^261 (x R' = (x R) 2^5 mod q), canonical, as 8 x 32-bit limbs."""
+    out = np.array(points, dtype=np.uint32, copy=True)
+    for row in out:
+        for c in (0, 8):
+            v = limbs_to_int(row[c:c + 8])
+            row[c:c + 8] = int_to_limbs((v << 5) % Q_MOD)
+    return out

@@ -298,7 +298,9 @@ def msm_hot_bucket(E):
         ws_bytes = int(E.call("impeller_msm_cfg_workspace_size", n, c, 0, task))
         ws = garbage_u32((ws_bytes + 3) // 4)
         out = garbage_u32(16)
-        E.call("impeller_msm_cfg", ptr(bases), ptr(flat(s)), n, 0, c, 0, task, ptr(out), ptr(ws),
+        table = bases.copy()                                  # one table: the bases, converted
+        E.call("impeller_msm_precompute", ptr(table), n, c, 0, None)
+        E.call("impeller_msm_cfg", ptr(table), ptr(flat(s)), n, 0, c, 0, task, ptr(out), ptr(ws),
                ws_bytes, None)
         assert (out == O.msm_closed_form(s, k)).all(), (c, task)
 
@@ -315,6 +317,65 @@ def field_inverse(E):
         E.call("impeller_field_op", f, 7, ptr(out), ptr(flat(a)), None, len(vals), None)
         exp = np.stack([O.inv(a[i], f) for i in range(len(vals))])
         assert (out.reshape(-1, 8) == exp).all(), f
+
+
+def _g1(E, op, a, b, cols, n):
+    out = u32(n * cols)
+    E.call("impeller_g1_op", op, ptr(out), ptr(flat(a)), ptr(flat(b)) if b is not None else None,
+           n, None)
+    return out.reshape(n, cols)
+
+
+def _same_xyzz(got, exp):
+    """Bit-identical XYZZ rows; identity rows (zz == 0) only need zz == 0,
+    since the other coordinates of an identity carry no value."""
+    for i in range(len(exp)):
+        if not exp[i][16:24].any():
+            assert not got[i][16:24].any(), i
+        else:
+            assert (got[i] == exp[i]).all(), i
+
+
+def arith29(E):
+    """The 9 x 29-bit Montgomery forms (product, padded subtractions, XYZZ
+    formulas, table format) agree bit for bit with the 8 x 32-bit forms and
+    the oracle, including the doubling and identity branches."""
+    for f, m in ((O.FR, O.R_MOD), (O.FQ, O.Q_MOD)):
+        rng = random.Random(29 + f)
+        vals = [0, 1, m - 1, m - 2, (1 << 253) % m] + [rng.randrange(m) for _ in range(59)]
+        a = O.ints_to_array([O.to_mont_int(v) if f == O.FR else v for v in vals])
+        b = np.roll(a, 3, axis=0).copy()
+        n = len(vals)
+        for op, ref in ((8, lambda x, y: O.mul(x, y, f)), (9, lambda x, y: O.sub(x, y, f)),
+                        (10, lambda x, y: O.sub(O.sub(O.sub(x, y, f), y, f), y, f))):
+            out = u32(n * 8)
+            E.call("impeller_field_op", f, op, ptr(out), ptr(flat(a)), ptr(flat(b)), n, None)
+            exp = np.stack([ref(a[i], b[i]) for i in range(n)])
+            assert (out.reshape(n, 8) == exp).all(), (f, op)
+    n = 48
+    bases = O.gen_bases(n + 1, 77)
+    x = np.stack([O.xyzz_op("orc_affine_to_xyzz", bases[i]) for i in range(n)])
+    x = np.stack([O.xyzz_op("orc_xyzz_dbl", x[i]) for i in range(n)])
+    x[5] = 0
+    x[5][24:] = O.one(O.FQ)                                       # identity
+    q = bases[1:n + 1].copy()
+    q[11] = 0                                                     # affine identity
+    q[12] = O.xyzz_op("orc_xyzz_to_affine", x[12], out_len=16)    # same point: doubling
+    q[13] = O.xyzz_op("orc_xyzz_to_affine", x[13], out_len=16)
+    q[13][8:] = O.neg(q[13][8:], O.FQ)                            # opposite point: identity
+    _same_xyzz(_g1(E, 5, x, q, 32, n), _g1(E, 0, x, q, 32, n))
+    y = np.roll(x, 1, axis=0).copy()
+    y[20] = x[20]                                                 # doubling branch of add
+    y[21] = x[21]
+    y[21][8:16] = O.neg(x[21][8:16], O.FQ)                        # P + (-P)
+    _same_xyzz(_g1(E, 6, x, y, 32, n), _g1(E, 1, x, y, 32, n))
+    _same_xyzz(_g1(E, 7, x, None, 32, n), _g1(E, 2, x, None, 32, n))
+    neg = bases[:n].copy()
+    neg[:, 8:] = np.stack([O.neg(bases[i][8:], O.FQ) for i in range(n)])
+    neg[0] = 0
+    src = bases[:n].copy()
+    src[0] = 0
+    assert (_g1(E, 8, src, None, 16, n) == neg).all()
 
 
 def mlp_grad(E):
@@ -355,7 +416,7 @@ def gate(E):
 
 ALL = [("single_ntt", single_ntt), ("ntt_plans", ntt_plans), ("batched_ntt_mixed", batched_ntt_mixed),
        ("batched_ntt_plans", batched_ntt_plans), ("ntt_graph_replay", ntt_graph_replay),
-       ("gate", gate), ("field_inverse", field_inverse), ("mlp_grad", mlp_grad), ("single_msm", single_msm), ("msm_hot_bucket", msm_hot_bucket), ("msm_graph_replay", msm_graph_replay),
+       ("gate", gate), ("field_inverse", field_inverse), ("arith29", arith29), ("mlp_grad", mlp_grad), ("single_msm", single_msm), ("msm_hot_bucket", msm_hot_bucket), ("msm_graph_replay", msm_graph_replay),
        ("batched_msm_shared_tables", batched_msm_shared_tables)]
 
 
