@@ -2,13 +2,15 @@
 
 Plain single-call path (the metric's first two numbers): forward NTT 2^24,
 coset NTT 2^24 (shift = 7) and one G1 MSM of 2^22 affine bases k_i*G with
-uniform scalars, all device resident; the MSM window c is swept over 12..18.
+uniform scalars, all device resident; the MSM window c is swept over 12..20.
 Batched vs plain (the third number): 64 uneven NTTs (2^18..2^23, forward /
 inverse / coset-extended x8) and 40 commitment MSMs (2^16..2^22, shared 2^22
-base table, 50% uniform / 30% selector / 20% small scalars), each run as
-sequential single calls and as batched launches replayed from a HIP graph;
-outputs compared bit for bit.  With --chunk, also the fused proving chunk at
-k = 22 (plain vs batched+graph).
+base table, 50% uniform / 30% selector / 20% small scalars), run as
+sequential single calls with the untuned default configurations (plain) and
+as batched launches replayed from a HIP graph with the cost model's picks
+(and, for reference, the exhaustive tuner's picks); outputs compared bit for
+bit.  With --chunk, also the fused proving chunk at k = 22 under the same
+three schedules.
 
   python bench.py [--steps K] [--warmup W] [--window C] [--no-cpu] [--no-batch] [--chunk]
 """
@@ -33,11 +35,25 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--log-ntt", type=int, default=24)
     ap.add_argument("--log-msm", type=int, default=22)
-    ap.add_argument("--window", type=int, default=0, help="MSM window c (0: sweep 12..18)")
+    ap.add_argument("--window", type=int, default=0, help="MSM window c (0: sweep 12..20)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched comparisons")
     ap.add_argument("--chunk", action="store_true", help="also run the k=22 proving chunk")
     return ap.parse_args()
+
+
+def dit_traffic(log_n):
+    """HBM bytes per k_ntt_dit launch from the committed counter summary
+    (profiles/ntt<log>_dit_hbm_traffic.md, line "traffic_bytes_per_launch: B"),
+    or None."""
+    path = os.path.join(ROOT, "profiles", f"ntt{log_n}_dit_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    return None
 
 
 def load_picks():
@@ -101,13 +117,13 @@ def main():
     _lib.call("impeller_gen_scalars", scal.data_ptr(), nm, 0, 78, 0, stream.raw())
     stream.sync()
     x_orig = x.clone()
-    ntt_plan = None
-    p = picks.get(f"ntt_fwd:{k}")
-    if p:
-        ntt_plan = p["exhaustive"]
+    # plans / MSM task from the autotuner's exhaustive picks when present
+    ntt_plan = (picks.get(f"ntt_fwd:{k}") or {}).get("exhaustive")
+    coset_plan = (picks.get(f"ntt_coset:{k}") or {}).get("exhaustive")
+    msm_task = ((picks.get(f"msm:{args.log_msm}:0") or {}).get("exhaustive") or [0, 0, 64])[2]
 
     # ---------------------------------------- MSM 2^22: window sweep + tables
-    windows = [args.window] if args.window else list(range(12, 19))
+    windows = [args.window] if args.window else list(range(12, 21))
 
     def time_msm(tables, c, target, task, reps=2):
         ws = device_u32((M.cfg_workspace_size(nm, c, target, task) + 3) // 4)
@@ -127,7 +143,7 @@ def main():
         b.record()
         b.synchronize()
         setup_ms[c] = a.elapsed_time(b)
-        sweep[c] = time_msm(tables, c, 1, 64)
+        sweep[c] = time_msm(tables, c, 1, msm_task)
         del tables
         torch.cuda.empty_cache()
     for c in (windows if args.window else (15, 16, 17)):
@@ -136,17 +152,17 @@ def main():
     tables = device_u32(M.n_tables(best_c, 1) * nm * 16)
     tables[: nm * 16].copy_(bases)
     M.precompute_device(tables, nm, best_c, 1, stream)
-    ws = device_u32((M.cfg_workspace_size(nm, best_c, 1, 64) + 3) // 4)
+    ws = device_u32((M.cfg_workspace_size(nm, best_c, 1, msm_task) + 3) // 4)
     res_buf = device_u32(16)
 
     def run_msm():
-        return M.msm_device_cfg(tables, scal, nm, best_c, 1, stream, task=64, workspace=ws,
-                                out=res_buf, sync=False)
+        return M.msm_device_cfg(tables, scal, nm, best_c, 1, stream, task=msm_task,
+                                workspace=ws, out=res_buf, sync=False)
 
     # ------------------------------------------------------ plain timed steps
     for _ in range(args.warmup):
         N.ntt(x, tw, k, stream, passes=ntt_plan)
-        N.ntt(x, tw, k, stream, coset_shift=shift, passes=ntt_plan)
+        N.ntt(x, tw, k, stream, coset_shift=shift, passes=coset_plan)
         run_msm()
     torch.cuda.synchronize()
     _lib.timing_reset()
@@ -159,7 +175,7 @@ def main():
         e[0].record()
         N.ntt(x, tw, k, stream, passes=ntt_plan)
         e[1].record()
-        N.ntt(x, tw, k, stream, coset_shift=shift, passes=ntt_plan)
+        N.ntt(x, tw, k, stream, coset_shift=shift, passes=coset_plan)
         e[2].record()
         res = run_msm()
         e[3].record()
@@ -179,94 +195,149 @@ def main():
     msm_res_host = to_host(res, 16)[0].copy()
 
     # ---------------------------------------------------- batched vs plain
+    # plain: sequential single calls with the configurations used without a
+    # tuner (default_ntt_plan, default_msm); batched: one launch per stage,
+    # replayed from a HIP graph, with the cost model's picks, and for
+    # reference with the exhaustive tuner's picks.  Outputs compared bit for bit.
+    from impeller_amd import autotune as A
+
+    def picked(prob, key):
+        pk = picks.get(prob)
+        return pk.get(key) if pk else None
+
+    def ntt_plans(key):
+        out = {}
+        for kind, lg in WL.ntt_batch_spec():
+            L = lg + 3 if kind == "ext" else lg
+            op = {"fwd": "ntt_fwd", "inv": "ntt_inv", "ext": "ntt_coset"}[kind]
+            pl = picked(f"{op}:{L}", key)
+            if pl:
+                out[(kind, L)] = pl
+        return out
+
+    def msm_configs(key):
+        out = []
+        for lg, kind in WL.msm_batch_spec():
+            cfg = picked(f"msm:{lg}:{kind}", key) if key else None
+            out.append(tuple(cfg) if cfg else A.default_msm(lg))
+        return out
+
     batch = None
     batch_parity = {}
     if not args.no_batch:
         del ws
         torch.cuda.empty_cache()
-        # batched NTT: 64 uneven transforms
+        # 64 uneven NTTs; every schedule restores its in-place inputs first and
+        # the restore time is measured on its own and subtracted
         spec = WL.ntt_batch_spec()
-        plans = {}
-        for kind, lg in spec:
-            L = lg + 3 if kind == "ext" else lg
-            op = {"fwd": "ntt_fwd", "inv": "ntt_inv", "ext": "ntt_coset"}[kind]
-            pk = picks.get(f"{op}:{L}")
-            if pk:
-                plans[(kind, L)] = pk.get("model") or pk["exhaustive"]
-        ntt_wl = WL.NttWorkload(spec, stream, plans=plans)
-        nb_ = ntt_wl.batch()
-
-        def plain_ntt():
-            ntt_wl.restore()
-            ntt_wl.run_plain()
-
-        def batched_ntt():
-            ntt_wl.restore()
-            nb_.run(stream)
-
-        # the transforms run in place: both schedules restore their inputs
-        # first, and the restore time is measured on its own and subtracted
+        ntt_wl = WL.NttWorkload(spec, stream)
         t_restore = timed(ntt_wl.restore, 2)
-        t_plain_ntt = timed(plain_ntt, 2)
+        t_plain_ntt = timed(lambda: (ntt_wl.restore(), ntt_wl.run_plain()), 2) - t_restore
         plain_out = [it["data"].clone() for it in ntt_wl.items]
-        g_ntt = N.Graph(stream, lambda: nb_.run(stream))
-        t_batch_ntt = timed(lambda: (ntt_wl.restore(), g_ntt.launch(stream)), 2)
-        batch_parity["ntt_batched_equals_plain"] = all(
-            bool(torch.equal(a, it["data"])) for a, it in zip(plain_out, ntt_wl.items))
-        del plain_out, g_ntt, nb_, ntt_wl
+        del ntt_wl
+        t_batch_ntt = {}
+        ntt_equal = True
+        for key in ("model", "exhaustive"):
+            wl = WL.NttWorkload(spec, stream, plans=ntt_plans(key))
+            nb_ = wl.batch()
+            g = N.Graph(stream, lambda: nb_.run(stream))
+            t_batch_ntt[key] = timed(lambda: (wl.restore(), g.launch(stream)), 2) - t_restore
+            ntt_equal &= all(bool(torch.equal(o, it["data"])) for o, it in zip(plain_out, wl.items))
+            del g, nb_, wl
+            torch.cuda.empty_cache()
+        batch_parity["ntt_batched_equals_plain"] = ntt_equal
+        del plain_out
         torch.cuda.empty_cache()
 
-        # batched MSM: 40 commitments on a shared 2^22 base table
+        # 40 commitment MSMs on a shared 2^22 base table
         mspec = WL.msm_batch_spec()
-        configs = []
-        for lg, kind in mspec:
-            pk = picks.get(f"msm:{lg}:{kind}")
-            if pk:
-                configs.append(tuple(pk.get("model") or pk["exhaustive"]))
-            else:
-                configs.append((WL.heuristic_window(lg), 1, 64))
         msm_wl = WL.MsmWorkload(mspec, stream)
-        wss = msm_wl.workspaces(configs)
+        cfg_plain = msm_configs(None)
+        wss = msm_wl.workspaces(cfg_plain)
         plain_res = device_u32(16 * len(mspec))
-        t_plain_msm = timed(lambda: msm_wl.run_plain(configs, plain_res, wss), 2)
+        msm_wl.run_plain(cfg_plain, plain_res, wss)     # builds the table sets
+        t_plain_msm = timed(lambda: msm_wl.run_plain(cfg_plain, plain_res, wss), 2)
         del wss
         torch.cuda.empty_cache()
-        mb = M.MsmBatch(msm_wl.items(configs), stream)
-        g_msm = N.Graph(stream, lambda: mb.run(stream))
-        t_batch_msm = timed(lambda: g_msm.launch(stream), 2)
-        batch_parity["msm_batched_equals_plain"] = bool(torch.equal(plain_res, mb.results))
+        t_batch_msm = {}
+        msm_equal = True
+        for key in ("model", "exhaustive"):
+            mb = M.MsmBatch(msm_wl.items(msm_configs(key)), stream)
+            g = N.Graph(stream, lambda: mb.run(stream))
+            t_batch_msm[key] = timed(lambda: g.launch(stream), 2)
+            msm_equal &= bool(torch.equal(plain_res, mb.results))
+            del g, mb
+            torch.cuda.empty_cache()
+        batch_parity["msm_batched_equals_plain"] = msm_equal
         batch = {
-            "ntt64_plain_ms": round(t_plain_ntt - t_restore, 3),
-            "ntt64_batched_graph_ms": round(t_batch_ntt - t_restore, 3),
+            "ntt64_plain_ms": round(t_plain_ntt, 3),
+            "ntt64_batched_graph_ms": round(t_batch_ntt["model"], 3),
+            "ntt64_batched_exhaustive_ms": round(t_batch_ntt["exhaustive"], 3),
             "msm40_plain_ms": round(t_plain_msm, 3),
-            "msm40_batched_graph_ms": round(t_batch_msm, 3),
-            "ntt_plans": "tuned" if plans else "default",
-            "msm_configs": "tuned" if picks else "heuristic",
+            "msm40_batched_graph_ms": round(t_batch_msm["model"], 3),
+            "msm40_batched_exhaustive_ms": round(t_batch_msm["exhaustive"], 3),
+            "plain_configs": "default (no tuner)",
+            "batched_configs": "cost model picks" if picks else "default (no picks file)",
         }
-        batch["ntt64_speedup"] = round(batch["ntt64_plain_ms"] / batch["ntt64_batched_graph_ms"], 3)
-        batch["msm40_speedup"] = round(t_plain_msm / t_batch_msm, 3)
-        batch["speedup"] = round(
-            (batch["ntt64_plain_ms"] + t_plain_msm)
-            / (batch["ntt64_batched_graph_ms"] + t_batch_msm), 3)
-        msm_batch_res = mb.results_host()
-        del g_msm, mb, msm_wl
+        batch["ntt64_speedup"] = round(t_plain_ntt / t_batch_ntt["model"], 3)
+        batch["msm40_speedup"] = round(t_plain_msm / t_batch_msm["model"], 3)
+        batch["speedup"] = round((t_plain_ntt + t_plain_msm)
+                                 / (t_batch_ntt["model"] + t_batch_msm["model"]), 3)
+        batch["model_vs_exhaustive"] = round(
+            (t_batch_ntt["model"] + t_batch_msm["model"])
+            / (t_batch_ntt["exhaustive"] + t_batch_msm["exhaustive"]), 3)
+        del msm_wl, plain_res
         torch.cuda.empty_cache()
 
     chunk = None
     if args.chunk:
+        # k = 22 proving chunk: plain single calls with the default
+        # configuration vs the batched graph with the cost model's picks
         pc = WL.ProvingChunk(stream)
-        cfg = (16, 0, 64)
-        wsc = device_u32((M.cfg_workspace_size(1 << pc.k, cfg[0], cfg[1], cfg[2]) + 3) // 4)
-        t_plain = timed(lambda: pc.run_plain(cfg, ws=wsc), 1)
+        K = pc.k + pc.EXT
+
+        def chunk_plans(key):
+            out = {}
+            for kind, L, op in (("inv", pc.k, "ntt_inv"), ("ext", K, "ntt_coset"),
+                                ("inv", K, "ntt_inv")):
+                pl = picked(f"{op}:{L}", key)
+                if pl:
+                    out[(kind, L)] = pl
+            return out
+
+        def chunk_tables(cfg):
+            if M.n_tables(cfg[0], cfg[1]) == 1:
+                return pc.bases, 0
+            t = device_u32(M.n_tables(cfg[0], cfg[1]) * (1 << pc.k) * 16)
+            t[: (1 << pc.k) * 16].copy_(pc.bases)
+            M.precompute_device(t, 1 << pc.k, cfg[0], cfg[1], stream)
+            return t, 1 << pc.k
+
+        cfg_d = A.default_msm(pc.k)
+        tab_d, _ = chunk_tables(cfg_d)
+        wsc = device_u32((M.cfg_workspace_size(1 << pc.k, *cfg_d) + 3) // 4)
+        t_plain = timed(lambda: pc.run_plain(cfg_d, msm_tables=tab_d, ws=wsc), 1)
         plain_commit = pc.results.clone()
-        nbs = pc.batches()
-        mbc = pc.msm_batch(cfg)
-        g = N.Graph(stream, lambda: pc.run_batched(nbs, mbc))
-        t_batched = timed(lambda: g.launch(stream), 1)
-        chunk = {"plain_ms": round(t_plain, 2), "batched_graph_ms": round(t_batched, 2),
-                 "speedup": round(t_plain / t_batched, 3),
-                 "commitments_equal": bool(torch.equal(plain_commit, mbc.results))}
-        del pc, g
+        del wsc, tab_d
+        torch.cuda.empty_cache()
+        chunk = {"plain_ms": round(t_plain, 2), "plain_config": list(cfg_d)}
+        equal = True
+        for key in ("model", "exhaustive"):
+            cfg = tuple(picked(f"msm:{pc.k}:0", key) or cfg_d)
+            tab, stride = chunk_tables(cfg)
+            nbs = pc.batches(chunk_plans(key))
+            mbc = pc.msm_batch(cfg, tables=tab, table_stride=stride)
+            g = N.Graph(stream, lambda: pc.run_batched(nbs, mbc))
+            t_b = timed(lambda: g.launch(stream), 1)
+            equal &= bool(torch.equal(plain_commit, mbc.results))
+            chunk[f"batched_{key}_ms"] = round(t_b, 2)
+            chunk[f"{key}_config"] = list(cfg)
+            del g, nbs, mbc, tab
+            torch.cuda.empty_cache()
+        chunk["batched_graph_ms"] = chunk["batched_model_ms"]
+        chunk["speedup"] = round(t_plain / chunk["batched_model_ms"], 3)
+        chunk["commitments_equal"] = equal
+        del pc
 
     # ------------------------------------------------ parity + CPU baseline
     parity = dict(batch_parity)
@@ -325,7 +396,7 @@ def main():
                         "(plain single-call path, device resident)",
             "ntt_log": k, "msm_log": args.log_msm, "msm_window": best_c,
             "ntt_plan": ntt_plan or "default",
-            "msm_layout": "precomputed window tables (one bucket set), task 64",
+            "msm_layout": f"precomputed window tables (one bucket set), task {msm_task}",
             "parallelism": "single GPU",
         },
         "ntt_2^24_ms": round(ntt_ms, 3),
@@ -346,7 +417,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(dit_gbs / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": dit_traffic(k),
             "algorithmic_bytes_per_launch": bytes_ntt,
             "avg_launch_ms": round(dit_avg_ms, 4),
         },

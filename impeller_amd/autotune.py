@@ -20,7 +20,7 @@ import os
 import numpy as np
 
 OPS = ["ntt_fwd", "ntt_inv", "ntt_coset", "msm"]
-N_FEATURES = 16
+N_FEATURES = 24
 
 
 # ------------------------------------------------------------------ spaces
@@ -53,26 +53,47 @@ def msm_candidates(windows=range(10, 21), targets=(0, 1), tasks=(32, 64, 128)):
     return [(c, t, T) for c in windows for t in targets for T in tasks]
 
 
-DEFAULT_MSM = (16, 0, 64)
+DEFAULT_MSM = (16, 0, 64)  # the plain `msm` symbol's configuration
+
+
+def default_msm(log_n):
+    """The configuration the batched workloads use without a tuned pick:
+    heuristic window, precomputed tables, task 64."""
+    from .workloads import heuristic_window
+    return (heuristic_window(log_n), 1, 64)
 
 
 # --------------------------------------------------------------- features
+# Slots 0-3: op one-hot; 4: log2 size; 15: log2 batch.  NTT: 5 fuse / 5,
+# 6 LDS tile log - 10, 7 passes, 8/9 deepest/shallowest pass, 16-19 pass
+# depths (sorted, zero padded), 20 levels after the fused ones, 21 passes
+# deeper than the LDS tile.  MSM: 10 window c, 11 precomputed tables,
+# 12 log2 task, 13/14/7 fraction of zero / one / small scalars, then the
+# derived shape: 16 log2 entries per bucket set, 17 log2 entries per bucket,
+# 18 sort passes, 19 bucket sets, 20 log2 tasks per bucket, 21 log2 buckets.
 def ntt_features(op, L, plan, batch=1):
     f = np.zeros(N_FEATURES, np.float32)
     f[OPS.index(op)] = 1
     f[4] = L
     f[5] = plan[0] / 5.0
     f[6] = plan[1] - 10
-    d = plan[2:]
+    d = list(plan[2:])
     f[7] = len(d)
     f[8] = max(d)
     f[9] = min(d)
+    dd = sorted(d, reverse=True)[:4]
+    f[16:16 + len(dd)] = dd
+    f[20] = L - plan[0]
+    f[21] = sum(1 for x in d if x > plan[1])
     f[15] = np.log2(batch)
     return f
 
 
 def msm_features(logN, cfg, sparsity=(0.0, 0.0, 0.0), batch=1):
     c, target, task = cfg
+    W = 254 // c + 1
+    nw = W if target == 0 else min(target, W)
+    npre = -(-W // nw)
     f = np.zeros(N_FEATURES, np.float32)
     f[3] = 1
     f[4] = logN
@@ -81,8 +102,23 @@ def msm_features(logN, cfg, sparsity=(0.0, 0.0, 0.0), batch=1):
     f[12] = np.log2(task)
     f[13], f[14] = sparsity[0], sparsity[1]
     f[7] = sparsity[2]
+    f[16] = logN + np.log2(npre)
+    f[17] = f[16] - (c - 1)
+    f[18] = -(-c // 8)
+    f[19] = nw
+    f[20] = f[17] - f[12]
+    f[21] = np.log2(nw) + c - 1
     f[15] = np.log2(batch)
     return f
+
+
+def row_features(row):
+    """Features of a timing row, recomputed from its problem and config (so
+    rows recorded with an older feature layout stay usable)."""
+    parts = row["problem"].split(":")
+    if parts[0] == "msm":
+        return msm_features(int(parts[1]), row["config"], scalar_sparsity(int(parts[2])))
+    return ntt_features(parts[0], int(parts[1]), row["config"])
 
 
 def scalar_sparsity(kind):
@@ -216,25 +252,30 @@ def tune_msm(db, logN, bases, stream, kinds=(0,), cands=None, reps=2):
 
 # ------------------------------------------------------------ model picks
 def fit_model(db, stream, rows=None, steps=1500, seed=0):
+    """Cost model on the rows' features; the targets are log-times centred
+    per problem, so the network learns how configurations rank within a
+    problem rather than how problem sizes compare (a pick only needs the
+    ranking)."""
     from .costmodel import CostModel
     rows = rows if rows is not None else db.rows
-    feats = np.array([r["features"] for r in rows], np.float32)
+    feats = np.array([row_features(r) for r in rows], np.float32)
     ms = np.array([r["ms"] for r in rows])
     model = CostModel(N_FEATURES, 64, seed)
-    model.fit(feats, ms, stream, steps=steps)
+    model.fit(feats, ms, stream, steps=steps, groups=[r["problem"] for r in rows])
     return model
 
 
 def pick(model, rows, stream):
     """The row (configuration) the model predicts fastest among candidates."""
-    feats = np.array([r["features"] for r in rows], np.float32)
+    feats = np.array([row_features(r) for r in rows], np.float32)
     pred = model.predict_ms(feats, stream)
     return rows[int(np.argmin(pred))]
 
 
 def regret_report(db, stream, steps=1500, holdout=True, log=None):
     """Per problem: measured ms of the model's pick vs the exhaustive best and
-    vs the library default.  With holdout, the model for a problem is trained
+    vs the configuration used without the model (default_ntt_plan,
+    default_msm).  With holdout, the model for a problem is trained
     without that problem's rows (leave-one-problem-out), i.e. a pick without
     a sweep of that problem."""
     report = {}
@@ -248,7 +289,8 @@ def regret_report(db, stream, steps=1500, holdout=True, log=None):
         choice = pick(model, rows, stream)
         op = p.split(":")[0]
         if op == "msm":
-            default = [r for r in rows if tuple(r["config"]) == DEFAULT_MSM]
+            dcfg = default_msm(int(p.split(":")[1]))
+            default = [r for r in rows if tuple(r["config"]) == dcfg]
         else:
             L = int(p.split(":")[1])
             default = [r for r in rows if r["config"] == default_ntt_plan(L)]

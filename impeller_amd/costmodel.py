@@ -101,9 +101,16 @@ class CostModel:
         self.sd = np.ones(F, np.float32)
         self.ymu, self.ysd = 0.0, 1.0
 
-    def fit(self, feats, ms, stream, steps=1500, lr=2e-3):
+    def fit(self, feats, ms, stream, steps=1500, lr=2e-3, groups=None):
+        """groups: optional problem label per row; log-times are then centred
+        per group (predictions rank configurations within a problem)."""
         X = np.asarray(feats, np.float32)
         y = np.log(np.asarray(ms, np.float64))
+        if groups is not None:
+            g = np.asarray(groups)
+            for label in set(groups):
+                m = g == label
+                y[m] -= y[m].mean()
         self.mu = X.mean(axis=0)
         self.sd = X.std(axis=0) + 1e-6
         self.ymu, self.ysd = float(y.mean()), float(y.std() + 1e-9)

@@ -407,17 +407,39 @@ __device__ __forceinline__ void zero_body(const MsmJob& J, u64 i) {
   else if (i < 2 * nbw) J.bend[i - nbw] = 0;
 }
 
-// bucket run bounds in the sorted keys (bucket 0 = zero digit, skipped)
-__device__ __forceinline__ void bounds_body(const MsmJob& J, u64 gi) {
+// bucket run bounds in the sorted keys (bucket 0 = zero digit, skipped):
+// work-group lb covers 4096 consecutive keys of one physical window, 16 per
+// thread
+static const u32 kBoundsSpan = 4096;
+
+__host__ __device__ inline u64 bounds_blocks_per_window(const MsmShape& s) {
+  return (s.seg_len + kBoundsSpan - 1) / kBoundsSpan;
+}
+
+__device__ __forceinline__ void bounds_body(const MsmJob& J, u64 lb) {
   const MsmShape& s = J.s;
-  if (gi >= (u64)s.n_windows * s.seg_len) return;
-  u32 pw = (u32)(gi / s.seg_len);
-  u64 i = gi - (u64)pw * s.seg_len;
+  const u64 bpw = bounds_blocks_per_window(s);
+  const u32 pw = (u32)(lb / bpw);
+  const u64 i0 = (lb - (u64)pw * bpw) * kBoundsSpan + 16 * (u64)threadIdx.x;
+  if (pw >= s.n_windows || i0 >= s.seg_len) return;
   const u32* kk = sorted_keys(J) + (u64)pw * s.seg_len;
-  u32 k = kk[i];
-  if (k == 0) return;
-  if (i == 0 || kk[i - 1] != k) J.bstart[(u64)pw * (s.nb + 1) + k] = (u32)i;
-  if (i + 1 == s.seg_len || kk[i + 1] != k) J.bend[(u64)pw * (s.nb + 1) + k] = (u32)(i + 1);
+  u32* bs = J.bstart + (u64)pw * (s.nb + 1);
+  u32* be = J.bend + (u64)pw * (s.nb + 1);
+  const u32 n = s.seg_len - i0 < 16 ? (u32)(s.seg_len - i0) : 16u;
+  const u32 prev = i0 ? kk[i0 - 1] : 0xffffffffu;
+  const u32 next = i0 + n < s.seg_len ? kk[i0 + n] : 0xffffffffu;
+  u32 k[16];
+#pragma unroll
+  for (u32 j = 0; j < 16; j++) k[j] = j < n ? kk[i0 + j] : 0u;
+#pragma unroll
+  for (u32 j = 0; j < 16; j++) {
+    const u32 cur = k[j];
+    if (j >= n || cur == 0) continue;
+    const u32 before = j ? k[j - 1] : prev;
+    const u32 after = j + 1 < n ? k[j + 1] : next;
+    if (cur != before) bs[cur] = (u32)(i0 + j);
+    if (cur != after) be[cur] = (u32)(i0 + j + 1);
+  }
 }
 
 // ----------------------------------------------------------- segmented scans
@@ -754,7 +776,7 @@ __global__ __launch_bounds__(256) void kb_zero(const MsmJob* jobs, u32 njobs, co
   IMP_BATCH_LOOP(zero_body(J, lb * blockDim.x + threadIdx.x))
 }
 __global__ __launch_bounds__(256) void kb_bounds(const MsmJob* jobs, u32 njobs, const u32* prefix) {
-  IMP_BATCH_LOOP(bounds_body(J, lb * blockDim.x + threadIdx.x))
+  IMP_BATCH_LOOP(bounds_body(J, lb))
 }
 __global__ __launch_bounds__(256) void kb_tasks(const MsmJob* jobs, u32 njobs, const u32* prefix) {
   IMP_BATCH_LOOP(tasks_body(J, lb * blockDim.x + threadIdx.x))
@@ -827,7 +849,7 @@ static u32 stage_blocks(const MsmShape& s, u32 kind, u32 arg) {
       return kind == MS_SCAN_B ? s.n_windows : s.n_windows * scan_chunks(s, sk);
     }
     case MS_ZERO: return (u32)cdiv(2ull * s.n_windows * (s.nb + 1), 256);
-    case MS_BOUNDS: return (u32)cdiv((u64)s.n_windows * s.seg_len, 256);
+    case MS_BOUNDS: return (u32)(s.n_windows * bounds_blocks_per_window(s));
     case MS_TASKS: return (u32)cdiv((u64)s.n_windows * s.task_cap, 256);
     case MS_COMBINE:
       return arg <= s.levels ? (u32)cdiv((u64)s.n_windows * combine_cap(s, arg), 256) : 0;

@@ -333,8 +333,19 @@ __device__ __forceinline__ void dit_body(const NttJob& J, const NttStage& S, boo
   const bool contig = (s == 0), two = J.two != 0;
   u64 hi = 0, lo0 = 0;
   if (!contig) {
-    hi = wg >> (s - logC);
-    lo0 = (wg & ((1ull << (s - logC)) - 1)) << logC;
+    // strided: the twiddles depend on lo only.  Work-groups are dealt to the
+    // 8 XCDs round-robin, so let each XCD take whole lo groups (wg mod 8
+    // picks the group within a set of 8) and run all hi of a group back to
+    // back: the group's twiddles are then read into that XCD's L2 once.
+    const u32 lg_lo = s - logC, lg_hi = L - s - D;
+    if (lg_lo >= 3) {
+      const u64 k = wg >> 3;
+      hi = k & ((1ull << lg_hi) - 1);
+      lo0 = (((k >> lg_hi) << 3) | (wg & 7)) << logC;
+    } else {
+      hi = wg >> lg_lo;
+      lo0 = (wg & ((1ull << lg_lo) - 1)) << logC;
+    }
   }
   __syncthreads();
   for (u32 idx = threadIdx.x; idx < GC; idx += blockDim.x) {

@@ -2,11 +2,14 @@
 timings and evaluate the model's picks.
 
   python -m impeller_amd.tune_main [--out DIR] [--ntt-logs 18-25] [--msm-logs 16-22]
+                                   [--reuse TIMINGS.json] [--sweep all|ntt|msm|none]
 
 Writes DIR/timings.json (every measured configuration), DIR/picks.json
 (exhaustive best and model pick per problem, used by the batched workloads)
 and DIR/report.json (leave-one-problem-out regret of the model vs the
-exhaustive best and vs the library default).
+exhaustive best and vs the library default).  --reuse starts from the rows
+of an earlier timing file; --sweep re-measures only the named operation
+(rows of that operation in the reused file are dropped).
 """
 import argparse
 import json
@@ -29,6 +32,8 @@ def main():
     ap.add_argument("--msm-logs", default="16-22")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--reuse", default="")
+    ap.add_argument("--sweep", default="all", choices=("all", "ntt", "msm", "none"))
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)
 
@@ -40,18 +45,32 @@ def main():
     db = A.TimingDB()
     t0 = time.time()
     picks = {}
-    for L in parse_range(args.ntt_logs):
+    sweep_ntt = args.sweep in ("all", "ntt")
+    sweep_msm = args.sweep in ("all", "msm")
+    if args.reuse:
+        old = A.TimingDB.load(args.reuse)
+        for r in old.rows:
+            is_msm = r["problem"].startswith("msm")
+            if (is_msm and sweep_msm) or (not is_msm and sweep_ntt):
+                continue
+            db.rows.append(r)
+        for prob in db.problems():
+            best = min(db.for_problem(prob), key=lambda r: r["ms"])
+            picks[prob] = {"exhaustive": best["config"], "exhaustive_ms": best["ms"]}
+        print(f"[tune] reused {len(db.rows)} rows from {args.reuse}", flush=True)
+    for L in (parse_range(args.ntt_logs) if sweep_ntt else []):
         best = A.tune_ntt(db, L, stream, ops=("ntt_fwd", "ntt_inv", "ntt_coset"), reps=args.reps)
         for prob, (cfg, ms) in best.items():
             picks[prob] = {"exhaustive": cfg, "exhaustive_ms": ms}
         print(f"[tune] ntt 2^{L}: {len(A.ntt_plans(L))} plans x 3 ops, {time.time() - t0:.1f}s",
               flush=True)
     msm_logs = parse_range(args.msm_logs)
-    nb = 1 << max(msm_logs)
-    bases = device_u32(nb * 16)
-    _lib.call("impeller_gen_bases", bases.data_ptr(), nb, 77, 0, stream.raw())
-    stream.sync()
-    for lg in msm_logs:
+    if sweep_msm:
+        nb = 1 << max(msm_logs)
+        bases = device_u32(nb * 16)
+        _lib.call("impeller_gen_bases", bases.data_ptr(), nb, 77, 0, stream.raw())
+        stream.sync()
+    for lg in (msm_logs if sweep_msm else []):
         best = A.tune_msm(db, lg, bases, stream, kinds=(0, 1, 2), reps=max(1, args.reps - 1))
         for prob, (cfg, ms) in best.items():
             picks[prob] = {"exhaustive": cfg, "exhaustive_ms": ms}

@@ -215,7 +215,13 @@ hipError_t impeller_msm(const uint32_t *bases_d, const uint32_t *scalars_d,
  * window) * P with n_windows = min(target_windows, floor(254/window) + 1)
  * (target_windows = 0: one bucket set per window, a single table); the
  * reference's precompute layout (core/src/msm/src/msm_impl.cuh:56-67).
- * impeller_msm_precompute fills tables 1.. from table 0 in place;
+ * With more than one table, impeller_msm_precompute fills tables 1.. from
+ * table 0 (affine, Montgomery R = 2^256) in place and leaves every table,
+ * table 0 included, in the table format: each coordinate in Montgomery form
+ * with R' = 2^261, canonical, as a 256-bit integer in 8 u32 (the form the
+ * bucket kernels read); call it once per table buffer.  A single table
+ * (target_windows 0, or at least floor(254/window) + 1) is the plain bases,
+ * left as given and converted by each MSM call into its workspace.
  * table_stride is the number of points between consecutive tables (0 = len,
  * larger when a shorter MSM uses a prefix of a shared table set).  task is
  * the number of bucket entries one thread accumulates before the per-bucket
