@@ -194,6 +194,32 @@ def main():
     dit_gbs = bytes_ntt / (dit_avg_ms * 1e-3) / 1e9
     msm_res_host = to_host(res, 16)[0].copy()
 
+    # VALU side: Montgomery-product throughput measured live (two independent
+    # chains per thread, 4096 x 256 threads) against the products the NTT and
+    # the MSM bucket kernel perform
+    def mul_rate(form, iters=200, blocks=4096):
+        out = device_u32(blocks * 256)
+        launch = lambda: _lib.call("impeller_mont_mul_rate", form, blocks, iters,  # noqa: E731
+                                   out.data_ptr(), stream.raw())
+        return blocks * 256 * 2 * iters / (timed(launch, 3) * 1e-3)
+
+    rate_fr32, rate_fq29 = mul_rate(0), mul_rate(1)
+    ntt_muls = (n // 2) * k
+    tasks_ms, tasks_cnt = kt["msm_buckets"]
+    tasks_avg_ms = tasks_ms / max(tasks_cnt, 1)
+    msm_madds = (254 // best_c + 1) * nm          # one bucket set: W entries per point
+    valu = {
+        "ntt": {"what": f"forward NTT 2^{k}: (n/2) log n Fr Montgomery products",
+                "achieved": round(ntt_muls / (ntt_ms * 1e-3), 1), "peak": round(rate_fr32, 1),
+                "unit": "Fr mont-mul/s (8 x 32-bit form)",
+                "frac": round(ntt_muls / (ntt_ms * 1e-3) / rate_fr32, 4)},
+        "msm": {"what": f"kb_tasks: {msm_madds} mixed additions x 10 Fq products",
+                "achieved": round(10 * msm_madds / (tasks_avg_ms * 1e-3), 1),
+                "peak": round(rate_fq29, 1), "unit": "Fq mont-mul/s (9 x 29-bit form)",
+                "frac": round(10 * msm_madds / (tasks_avg_ms * 1e-3) / rate_fq29, 4),
+                "kernel_avg_ms": round(tasks_avg_ms, 4)},
+    }
+
     # ---------------------------------------------------- batched vs plain
     # plain: sequential single calls with the configurations used without a
     # tuner (default_ntt_plan, default_msm); batched: one launch per stage,
@@ -421,6 +447,7 @@ def main():
             "algorithmic_bytes_per_launch": bytes_ntt,
             "avg_launch_ms": round(dit_avg_ms, 4),
         },
+        "valu_roofline": valu,
         "parity": parity,
         "cpu_baseline": cpu,
     }

@@ -105,6 +105,7 @@ SIGNATURES = {
     "impeller_gen_bases": (_ERR, [_P, _U64, _U64, _U64, _P]),
     "impeller_field_op": (_ERR, [_INT, _INT, _P, _P, _P, _U64, _P]),
     "impeller_g1_op": (_ERR, [_INT, _P, _P, _P, _U64, _P]),
+    "impeller_mont_mul_rate": (_ERR, [_INT, _U32, _U32, _P, _P]),
     "impeller_mlp_param_count": (_U32, [_U32, _U32]),
     "impeller_mlp_workspace_floats": (ctypes.c_size_t, [_U32, _U32, _U32]),
     "impeller_mlp_forward": (_ERR, [_P, _P, _U32, _U32, _U32, _P, _P, _P]),

@@ -87,6 +87,7 @@ __global__ void k_field_op(int op, u32* r, const u32* a, const u32* b, u64 n) {
     // the 9 x 29-bit forms, entered and left through the R-form
     case 8: z = from_r29<P29>(mul29<P29>(to_r29<P29>(x), to_r29<P29>(y))); break;
     case 9: z = from_r29<P29>(sub29<P29, 2>(to_r29<P29>(x), to_r29<P29>(y))); break;
+    case 11: z = from_r29<P29>(sqr29<P29>(to_r29<P29>(x))); break;
     default: {
       F29<P29> ty = to_r29<P29>(y);
       z = from_r29<P29>(sub3_29<P29, 6>(to_r29<P29>(x), ty, ty, ty));
@@ -135,9 +136,47 @@ __global__ void k_g1_op(int op, u32* r, const u32* a, const u32* b, u64 n) {
   }
 }
 
+// two independent product chains per thread (their results are consumed so
+// the loop is not removed)
+template <int FORM>
+__global__ __launch_bounds__(256) void k_mul_rate(u32* out, u32 iters) {
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  u32 acc = 0;
+  if (FORM == 0) {
+    Fr a = Fr::one(), b = Fr::one(), c = Fr::r2();
+    a.v[0] ^= t;
+    b.v[1] ^= t;
+    for (u32 i = 0; i < iters; i++) {
+      a = mul<FrParams>(a, c);
+      b = mul<FrParams>(b, c);
+    }
+    for (int j = 0; j < 8; j++) acc ^= a.v[j] ^ b.v[j];
+  } else {
+    Fq29 a = Fq29::one(), b = Fq29::one(), c = consts29<Fq29Params>(Fq29Params::kin);
+    a.v[0] ^= t & 0xffffu;
+    b.v[1] ^= t & 0xffffu;
+    for (u32 i = 0; i < iters; i++) {
+      a = mul29<Fq29Params>(a, c);
+      b = mul29<Fq29Params>(b, c);
+    }
+    for (int j = 0; j < 9; j++) acc ^= a.v[j] ^ b.v[j];
+  }
+  out[t] = acc;
+}
+
 }  // namespace imp
 
 using namespace imp;
+
+extern "C" hipError_t impeller_mont_mul_rate(int form, unsigned int blocks, unsigned int iters,
+                                             uint32_t* out_d, hipStream_t stream) {
+  IMP_REQUIRE(out_d && blocks > 0 && (form == 0 || form == 1), "mul rate args");
+  if (form == 0)
+    hipLaunchKernelGGL(k_mul_rate<0>, dim3(blocks), dim3(256), 0, stream, out_d, iters);
+  else
+    hipLaunchKernelGGL(k_mul_rate<1>, dim3(blocks), dim3(256), 0, stream, out_d, iters);
+  return hipGetLastError();
+}
 
 extern "C" hipError_t fused_gate_abc_d_qe(const ConstPolyPtr* vars, const PolyPtr* mut_vars,
                                           unsigned long long len, const PolySlice* slice_info,
@@ -223,8 +262,8 @@ extern "C" hipError_t impeller_field_op(int field, int op, uint32_t* r_d, const 
                                         const uint32_t* b_d, unsigned long long n,
                                         hipStream_t stream) {
   IMP_REQUIRE(r_d && a_d, "null pointer");
-  IMP_REQUIRE(op >= 0 && op <= 10 && (field == 0 || field == 1), "op/field");
-  IMP_REQUIRE(b_d || op == 3 || op == 4 || op == 5 || op == 7, "b operand");
+  IMP_REQUIRE(op >= 0 && op <= 11 && (field == 0 || field == 1), "op/field");
+  IMP_REQUIRE(b_d || op == 3 || op == 4 || op == 5 || op == 7 || op == 11, "b operand");
   if (n == 0) return hipSuccess;
   dim3 grid((u32)((n + 127) / 128));
   if (field == 0)

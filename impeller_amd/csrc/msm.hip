@@ -54,10 +54,13 @@ struct MsmShape {
   u32 levels;     // combine levels (worst case over bucket loads)
   u32 chunks;     // scan chunks per window (max over the scans)
   u64 task_cap;   // task slots per physical window
-  u32 seg;        // buckets per reduce segment (4)
-  u32 nseg;       // reduce segments per physical window (nb / 4)
+  u32 seg;        // buckets per reduce segment (4, or 8 for c >= 17)
+  u32 nseg;       // reduce segments per physical window (nb / seg)
   u32 lbits;      // log2(nseg)
-  u32 sumg;       // work-groups per window sum (power of two)
+  u32 sbits;      // log2(seg)
+  u32 g0;         // level-0 work-groups per window sum (power of two, 512 inputs each)
+  u32 sum_levels; // 1: the final reduces the g0 partials; 2: a second level does
+  u32 fin_g;      // partials per window sum that the final reads (g0 or 1)
 };
 
 static const u32 kSortTile = 4096;  // 256 threads x 16 keys
@@ -85,22 +88,26 @@ static MsmShape msm_shape(u64 N, u32 c, u32 target_windows, u32 task = 64) {
   while ((1ull << (s.g * s.levels)) < max_tasks && s.levels < 8) s.levels++;
   u64 hl = (u64)kRadix * s.tiles, bl = (u64)s.nb + 1;
   s.chunks = (u32)(((hl > bl ? hl : bl) + 4095) / 4096);
-  s.seg = 4;
+  // longer segments for wide windows: halves the bit-split sums' work for a
+  // few more dependent additions per segment
+  s.sbits = c >= 17 ? 3 : 2;
+  s.seg = 1u << s.sbits;
   s.nseg = s.nb / s.seg;  // nb >= 8 for c >= 4
-  s.lbits = c - 3;
-  // window sums: lbits + 1 of them (the T_s and one per segment-index bit),
-  // each split over sumg work-groups; the final block holds every partial
-  const u32 nv = s.lbits + 2;
-  s.sumg = 1;
-  while (s.sumg < 8 && (u64)s.sumg * 2 * 256 <= s.nseg &&
-         (u64)s.n_windows * nv * s.sumg * 2 <= 256)
-    s.sumg *= 2;
+  s.lbits = c - 1 - s.sbits;
+  // window sums: lbits + 1 of them (the T_s and one per segment-index bit);
+  // level 0 gives each work-group 512 inputs (two per thread, then a tree),
+  // and when the final block cannot hold all g0 partials of every sum a
+  // second level reduces them to one
+  const u32 nv = s.lbits + s.sbits;  // = c - 1
+  s.g0 = s.nseg > 512 ? s.nseg / 512 : 1;
+  s.sum_levels = (u64)s.n_windows * nv * s.g0 <= 256 ? 1 : 2;
+  s.fin_g = s.sum_levels == 1 ? s.g0 : 1;
   return s;
 }
 
 struct MsmLayout {
   size_t keysA, valsA, keysB, valsB, hist, bstart, bend, csum, toff, loff, partA, partB, rparts,
-      sparts, conv, total;
+      sparts, sparts1, conv, total;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -122,7 +129,8 @@ static MsmLayout msm_layout(const MsmShape& s) {
   l.partA = o; o = align256(o + (size_t)s.n_windows * s.task_cap * kPt * 4);
   l.partB = o; o = align256(o + (size_t)s.n_windows * s.task_cap * kPt * 4);
   l.rparts = o; o = align256(o + (size_t)s.n_windows * 2 * s.nseg * kPt * 4);
-  l.sparts = o; o = align256(o + (size_t)s.n_windows * (s.lbits + 1) * s.sumg * kPt * 4);
+  l.sparts = o; o = align256(o + (size_t)s.n_windows * (s.lbits + 1) * s.g0 * kPt * 4);
+  l.sparts1 = o; o = align256(o + (size_t)s.n_windows * (s.lbits + 1) * kPt * 4);
   // one table (the plain bases, affine R-form) is converted per call
   l.conv = o; o = align256(o + (s.n_pre == 1 ? (size_t)s.N * 64 : 0));
   l.total = o;
@@ -140,7 +148,7 @@ struct MsmJob {
   const u32* scalars;
   u32* result;
   u32 *kA, *vA, *kB, *vB, *hist, *bstart, *bend, *csum, *toff, *loff, *partA, *partB, *rparts,
-      *sparts;
+      *sparts, *sparts1;
 };
 
 static MsmJob make_msm_job(const MsmShape& s, u64 tstride, const u32* bases, const u32* scalars,
@@ -171,6 +179,7 @@ static MsmJob make_msm_job(const MsmShape& s, u64 tstride, const u32* bases, con
   J.partB = (u32*)(ws + l.partB);
   J.rparts = (u32*)(ws + l.rparts);
   J.sparts = (u32*)(ws + l.sparts);
+  J.sparts1 = (u32*)(ws + l.sparts1);
   return J;
 }
 
@@ -375,15 +384,18 @@ __device__ __forceinline__ void scatter_body(const MsmJob& J, u32 pass, u64 lb, 
     }
   }
   split_nibble(L, key, val, n, shift);
+  // keys are below 2^c: a last pass over at most 4 bits needs one split
+  if (shift + 4 < s.c) {
 #pragma unroll
-  for (u32 j = 0; j < 16; j++) {
-    if (j < n) {
-      key[j] = L.sk[swz(16 * t + j)];
-      val[j] = L.sv[swz(16 * t + j)];
+    for (u32 j = 0; j < 16; j++) {
+      if (j < n) {
+        key[j] = L.sk[swz(16 * t + j)];
+        val[j] = L.sv[swz(16 * t + j)];
+      }
     }
+    __syncthreads();
+    split_nibble(L, key, val, n, shift + 4);
   }
-  __syncthreads();
-  split_nibble(L, key, val, n, shift + 4);
   // digit run starts in the staged order, and the runs' global starts
   L.gbase[t] = J.hist[((u64)pw * kRadix + t) * s.tiles + tile];
   for (u32 j = 0; j < n; j++) {
@@ -596,14 +608,15 @@ __device__ __forceinline__ void combine_body(const MsmJob& J, u32 level, u64 gi)
 }
 
 // Window sum S = sum_b b * B_b (b = 1 .. nb) in three shallow stages.
-// Segment s holds buckets 4s + 1 .. 4s + 4:
-//   T_s = sum_k (k + 1) B_(4s+k+1),  R_s = sum_k B_(4s+k+1),
-//   S = sum_s T_s + 4 sum_s s R_s = sum_s T_s + 4 sum_l 2^l V_l,
+// Segment s holds buckets m s + 1 .. m s + m (m = seg = 2^sbits):
+//   T_s = sum_k (k + 1) B_(ms+k+1),  R_s = sum_k B_(ms+k+1),
+//   S = sum_s T_s + m sum_s s R_s = sum_s T_s + m sum_l 2^l V_l,
 //   V_l = sum of R_s over the s with bit l set (l < lbits = log2 nseg).
-// reduce: one thread per segment (running sums over 4 buckets);
-// sum:    lbits + 1 sums (the T_s, then each V_l), sumg work-groups each;
-// final:  partials -> a_0 = sum T, a_1 = 0, a_(l+2) = V_l, S = sum_i 2^i a_i
-//         by a doubling tree, then Horner over physical windows and affine.
+// reduce: one thread per segment (running sums over m buckets);
+// sum:    lbits + 1 sums (the T_s, then each V_l), one or two tree levels;
+// final:  partials -> a_0 = sum T, a_1 .. a_(sbits-1) = 0, a_(sbits+l) = V_l,
+//         S = sum_i 2^i a_i by a doubling tree, then Horner over physical
+//         windows and affine.
 __device__ __forceinline__ void reduce_body(const MsmJob& J, u64 t) {
   const MsmShape& s = J.s;
   if (t >= (u64)s.n_windows * s.nseg) return;
@@ -638,31 +651,47 @@ __device__ __forceinline__ void block_tree_sum(u32* red) {
   }
 }
 
-// work-group lb: window pw, sum q (0: the T_s; q = l + 1: V_l), share g
-__device__ __forceinline__ void sum_body(const MsmJob& J, u64 lb, u32* red) {
+// Level 0, work-group lb: window pw, sum q (0: the T_s; q = l + 1: V_l),
+// inputs [512 g, 512 g + 512) of that sum, two per thread, then a tree.
+// Level 1, work-group lb: window pw, sum q, its g0 level-0 partials.
+__device__ __forceinline__ void sum_body(const MsmJob& J, u32 level, u64 lb, u32* red) {
   const MsmShape& s = J.s;
-  const u32 t = threadIdx.x;
-  const u32 g = (u32)(lb % s.sumg);
-  const u32 q = (u32)((lb / s.sumg) % (s.lbits + 1));
-  const u32 pw = (u32)(lb / ((u64)s.sumg * (s.lbits + 1)));
-  const u32 count = q == 0 ? s.nseg : s.nseg / 2;
-  const u32* src = J.rparts + kPt * ((u64)(2 * pw + (q ? 1 : 0)) * s.nseg);
-  const u32 l = q ? q - 1 : 0;
+  const u32 t = threadIdx.x, nq = s.lbits + 1;
   G1Xyzz29 acc = G1Xyzz29::identity();
   __syncthreads();
-  for (u32 i = g * 256 + t; i < count; i += s.sumg * 256) {
-    // q > 0: the i-th segment index with bit l set
-    const u32 sg = q == 0 ? i : (((i >> l) << (l + 1)) | (1u << l) | (i & ((1u << l) - 1)));
-    acc = xyzz29_add(acc, load_xyzz29(src + kPt * (u64)sg));
+  if (level == 0) {
+    const u32 g = (u32)(lb % s.g0);
+    const u32 q = (u32)((lb / s.g0) % nq);
+    const u32 pw = (u32)(lb / ((u64)s.g0 * nq));
+    const u32 count = q == 0 ? s.nseg : s.nseg / 2;
+    const u32* src = J.rparts + kPt * ((u64)(2 * pw + (q ? 1 : 0)) * s.nseg);
+    const u32 l = q ? q - 1 : 0;
+#pragma unroll
+    for (u32 h = 0; h < 2; h++) {
+      const u32 i = g * 512 + h * 256 + t;
+      if (i < count) {
+        // q > 0: the i-th segment index with bit l set
+        const u32 sg = q == 0 ? i : (((i >> l) << (l + 1)) | (1u << l) | (i & ((1u << l) - 1)));
+        acc = xyzz29_add(acc, load_xyzz29(src + kPt * (u64)sg));
+      }
+    }
+    store_xyzz29(red + kPt * t, acc);
+    __syncthreads();
+    block_tree_sum(red);
+    if (t < kPt) J.sparts[kPt * (((u64)pw * nq + q) * s.g0 + g) + t] = red[t];
+  } else {
+    const u64 v = lb;  // pw * nq + q
+    for (u32 i = t; i < s.g0; i += 256)
+      acc = xyzz29_add(acc, load_xyzz29(J.sparts + kPt * (v * s.g0 + i)));
+    store_xyzz29(red + kPt * t, acc);
+    __syncthreads();
+    block_tree_sum(red);
+    if (t < kPt) J.sparts1[kPt * v + t] = red[t];
   }
-  store_xyzz29(red + kPt * t, acc);
-  __syncthreads();
-  block_tree_sum(red);
-  if (t < kPt) J.sparts[kPt * (((u64)pw * (s.lbits + 1) + q) * s.sumg + g) + t] = red[t];
 }
 
-// one work-group per job (blockDim.x == 256); the host guarantees
-// n_windows * (lbits + 2) * sumg <= 256
+// one work-group per job (blockDim.x == 256); the shape guarantees
+// n_windows * (lbits + sbits) * fin_g <= 256
 __device__ __forceinline__ void final_body(const MsmJob& J, u32* red) {
   const MsmShape& s = J.s;
   const u32 t = threadIdx.x;
@@ -671,14 +700,15 @@ __device__ __forceinline__ void final_body(const MsmJob& J, u32* red) {
     if (t < 16) J.result[t] = 0;
     return;
   }
-  const u32 nv = s.lbits + 2, G = s.sumg, nval = s.n_windows * nv;
+  const u32 nv = s.lbits + s.sbits, G = s.fin_g, nval = s.n_windows * nv;
+  const u32* parts = s.sum_levels == 2 ? J.sparts1 : J.sparts;
   // value (pw, i) at red[(pw * nv + i) * G], its sum partials in the next G - 1 slots
   if (t < nval * G) {
     const u32 gi = t % G, i = (t / G) % nv, pw = t / (G * nv);
     G1Xyzz29 v = G1Xyzz29::identity();
-    if (i != 1) {
-      const u32 q = i == 0 ? 0 : i - 1;
-      v = load_xyzz29(J.sparts + kPt * (((u64)pw * (s.lbits + 1) + q) * G + gi));
+    if (i == 0 || i >= s.sbits) {
+      const u32 q = i == 0 ? 0 : i - s.sbits + 1;
+      v = load_xyzz29(parts + kPt * (((u64)pw * (s.lbits + 1) + q) * G + gi));
     }
     store_xyzz29(red + kPt * t, v);
   }
@@ -788,9 +818,10 @@ __global__ __launch_bounds__(256) void kb_combine(const MsmJob* jobs, u32 njobs,
 __global__ __launch_bounds__(256) void kb_reduce(const MsmJob* jobs, u32 njobs, const u32* prefix) {
   IMP_BATCH_LOOP(reduce_body(J, lb * blockDim.x + threadIdx.x))
 }
-__global__ __launch_bounds__(256) void kb_sum(const MsmJob* jobs, u32 njobs, const u32* prefix) {
+__global__ __launch_bounds__(256) void kb_sum(const MsmJob* jobs, u32 njobs, const u32* prefix,
+                                              u32 level) {
   __shared__ u32 red[256 * kPt];
-  IMP_BATCH_LOOP(sum_body(J, lb, red))
+  IMP_BATCH_LOOP(sum_body(J, level, lb, red))
 }
 __global__ __launch_bounds__(256) void kb_final(const MsmJob* jobs, u32 njobs, const u32* prefix) {
   __shared__ u32 red[256 * kPt];
@@ -854,7 +885,9 @@ static u32 stage_blocks(const MsmShape& s, u32 kind, u32 arg) {
     case MS_COMBINE:
       return arg <= s.levels ? (u32)cdiv((u64)s.n_windows * combine_cap(s, arg), 256) : 0;
     case MS_REDUCE: return (u32)cdiv((u64)s.n_windows * s.nseg, 256);
-    case MS_SUM: return s.n_windows * (s.lbits + 1) * s.sumg;
+    case MS_SUM:
+      if (arg == 0) return s.n_windows * (s.lbits + 1) * s.g0;
+      return s.sum_levels == 2 ? s.n_windows * (s.lbits + 1) : 0u;
     default: return 1;
   }
 }
@@ -889,6 +922,7 @@ static void plan_stages(const std::vector<MsmShape>& shapes, std::vector<StageLa
   }
   kinds.push_back({MS_REDUCE, 0});
   kinds.push_back({MS_SUM, 0});
+  kinds.push_back({MS_SUM, 1});
   kinds.push_back({MS_FINAL, 0});
   const u32 n = (u32)shapes.size();
   out.clear();
@@ -932,7 +966,7 @@ static hipError_t launch_stages(const MsmJob* jobs, u32 njobs, const u32* prefix
       case MS_TASKS: hipLaunchKernelGGL(kb_tasks, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
       case MS_COMBINE: hipLaunchKernelGGL(kb_combine, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf, st[k].arg); break;
       case MS_REDUCE: hipLaunchKernelGGL(kb_reduce, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
-      case MS_SUM: hipLaunchKernelGGL(kb_sum, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
+      case MS_SUM: hipLaunchKernelGGL(kb_sum, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf, st[k].arg); break;
       default: hipLaunchKernelGGL(kb_final, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
     }
     IMP_CHECK(hipGetLastError());
@@ -967,7 +1001,7 @@ static hipError_t msm_run(const u32* bases, u64 tstride, const u32* scalars, con
 
 static size_t msm_ws_total(const MsmShape& s) { return desc_bytes(1) + msm_layout(s).total; }
 
-// c in [4, 20]; the final work-group holds lbits + 2 = c - 1 values for each of
+// c in [4, 20]; the final work-group holds lbits + sbits = c - 1 values for each of
 // up to W = 254 / c + 1 windows, at most 255 over that range
 static bool valid_window(u32 c) { return c >= 4 && c <= 20 && (254 / c + 1) * (c - 1) <= 256; }
 

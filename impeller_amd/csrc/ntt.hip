@@ -413,12 +413,12 @@ __device__ __forceinline__ void dit_body(const NttJob& J, const NttStage& S, boo
 }
 
 // --------------------------------------------------------------- kernels
-__global__ __launch_bounds__(256) void k_ntt_br(NttJob J) {
+__global__ __launch_bounds__(256, 4) void k_ntt_br(NttJob J) {
   IMP_DYNAMIC_LDS(lds);
   br_body(J, blockIdx.x, lds);
 }
 
-__global__ __launch_bounds__(256) void k_ntt_dit(NttJob J, u32 stage) {
+__global__ __launch_bounds__(256, 4) void k_ntt_dit(NttJob J, u32 stage) {
   IMP_DYNAMIC_LDS(lds);
   dit_body(J, J.st[stage], stage + 1 == J.nst, blockIdx.x, lds);
 }
@@ -436,7 +436,7 @@ __device__ __forceinline__ u32 find_job(const u32* prefix, u32 njobs, u32 t) {
 // Batched stages: work-groups stride over the concatenated block space of all
 // jobs (prefix[j] = first block of job j in this stage); jobs without work in
 // the stage contribute zero blocks.
-__global__ __launch_bounds__(256) void k_ntt_br_batch(const NttJob* jobs, u32 njobs,
+__global__ __launch_bounds__(256, 4) void k_ntt_br_batch(const NttJob* jobs, u32 njobs,
                                                       const u32* prefix) {
   IMP_DYNAMIC_LDS(lds);
   const u32 total = prefix[njobs];
@@ -446,7 +446,7 @@ __global__ __launch_bounds__(256) void k_ntt_br_batch(const NttJob* jobs, u32 nj
   }
 }
 
-__global__ __launch_bounds__(256) void k_ntt_dit_batch(const NttJob* jobs, u32 njobs,
+__global__ __launch_bounds__(256, 4) void k_ntt_dit_batch(const NttJob* jobs, u32 njobs,
                                                        const u32* prefix, u32 stage) {
   IMP_DYNAMIC_LDS(lds);
   const u32 total = prefix[njobs];

@@ -278,15 +278,26 @@ hipError_t impeller_gen_scalars(uint32_t *out_d, unsigned long long n, int kind,
 hipError_t impeller_gen_bases(uint32_t *out_d, unsigned long long n, unsigned long long seed,
                               unsigned long long start, hipStream_t stream);
 
-/* Device field helpers used by tests of the arithmetic itself:
- * op 0 mul, 1 add, 2 sub, 3 neg(a), 4 from_number(a), 5 to_number(a),
- * 6 pow(a, b as integer), 7 inv(a); field 0 = Fr, 1 = Fq; n elements. */
+/* Device field helpers used by tests of the arithmetic itself (no reference
+ * analogue): op 0 mul, 1 add, 2 sub, 3 neg(a), 4 from_number(a),
+ * 5 to_number(a), 6 pow(a, b as integer), 7 inv(a), and through the
+ * 9 x 29-bit form (R-form in and out): 8 mul, 9 a - b, 10 a - 3b, 11 a^2;
+ * field 0 = Fr, 1 = Fq; n elements. */
 hipError_t impeller_field_op(int field, int op, uint32_t *r_d, const uint32_t *a_d,
                              const uint32_t *b_d, unsigned long long n, hipStream_t stream);
 /* G1 helpers: op 0 madd(xyzz, aff), 1 add(xyzz, xyzz), 2 dbl(xyzz),
- * 3 to_affine(xyzz) -> aff, 4 scalar_mul(aff, integer) -> aff; n items. */
+ * 3 to_affine(xyzz) -> aff, 4 scalar_mul(aff, integer) -> aff, and the
+ * 29-bit forms 5 madd, 6 add, 7 dbl, 8 -aff through the table format;
+ * n items. */
 hipError_t impeller_g1_op(int op, uint32_t *r_d, const uint32_t *a_d, const uint32_t *b_d,
                           unsigned long long n, hipStream_t stream);
+/* Montgomery-multiply throughput probe (no reference analogue): blocks x 256
+ * threads, each running two independent chains of iters products (form 0:
+ * the 8 x 32-bit Fr product; form 1: the 9 x 29-bit Fq product used by the
+ * MSM bucket kernels); out_d receives blocks * 256 words.  Products per
+ * launch = blocks * 256 * 2 * iters. */
+hipError_t impeller_mont_mul_rate(int form, unsigned int blocks, unsigned int iters,
+                                  uint32_t *out_d, hipStream_t stream);
 
 /* FP32 cost model of the autotuner (no reference analogue): a 3-layer MLP
  * F -> H -> H -> 1 with ReLU, parameters W1[H][F] b1[H] W2[H][H] b2[H]

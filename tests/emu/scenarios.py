@@ -266,7 +266,8 @@ def msm_graph_replay(E):
 
 def single_msm(E):
     for n, c, target, task, kind in ((1, 8, 0, 64, 0), (7, 13, 0, 64, 0), (1000, 10, 1, 16, 1),
-                                     (4096, 16, 0, 64, 2), (4096, 12, 2, 32, 0)):
+                                     (4096, 16, 0, 64, 2), (4096, 12, 2, 32, 0),
+                                     (1024, 17, 1, 32, 0), (512, 20, 1, 32, 2)):
         bases = u32(n * 16)
         E.call("impeller_gen_bases", ptr(bases), n, 77, 0, None)
         nt = int(E.call("impeller_msm_tables", c, target))
@@ -347,7 +348,8 @@ def arith29(E):
         b = np.roll(a, 3, axis=0).copy()
         n = len(vals)
         for op, ref in ((8, lambda x, y: O.mul(x, y, f)), (9, lambda x, y: O.sub(x, y, f)),
-                        (10, lambda x, y: O.sub(O.sub(O.sub(x, y, f), y, f), y, f))):
+                        (10, lambda x, y: O.sub(O.sub(O.sub(x, y, f), y, f), y, f)),
+                        (11, lambda x, y: O.mul(x, x, f))):
             out = u32(n * 8)
             E.call("impeller_field_op", f, op, ptr(out), ptr(flat(a)), ptr(flat(b)), n, None)
             exp = np.stack([ref(a[i], b[i]) for i in range(n)])
