@@ -543,32 +543,39 @@ __device__ __forceinline__ u32 find_bucket(const u32* toff, u32 nb, u32 t) {
   return lo;
 }
 
+// task t of window pw: entries [lo, hi) of the sorted values
+__device__ __forceinline__ bool task_range(const MsmJob& J, u32 pw, u32 t, u32& lo, u32& hi) {
+  const MsmShape& s = J.s;
+  const u32* to = J.toff + (u64)pw * (s.nb + 2);
+  if (t >= to[s.nb + 1]) return false;
+  const u32 b = find_bucket(to, s.nb, t);
+  const u32 k = t - to[b];
+  lo = J.bstart[(u64)pw * (s.nb + 1) + b] + k * s.T;
+  const u32 be = J.bend[(u64)pw * (s.nb + 1) + b];
+  hi = lo + s.T < be ? lo + s.T : be;
+  return true;
+}
+
+__device__ __forceinline__ G1Aff29 entry_point(const MsmJob& J, u32 e) {
+  G1Aff29 p = load_aff_table(J.bases + 16 * (u64)(e & 0x7fffffffu));
+  return (e & 0x80000000u) ? aff29_neg(p) : p;
+}
+
 // one task = up to T consecutive entries of one bucket, mixed XYZZ + affine
 // additions; the next point is loaded while the current one is added
 __device__ __forceinline__ void tasks_body(const MsmJob& J, u64 gi) {
   const MsmShape& s = J.s;
-  u32 pw = (u32)(gi / s.task_cap);
+  const u32 pw = (u32)(gi / s.task_cap);
   if (pw >= s.n_windows) return;
-  u32 t = (u32)(gi - (u64)pw * s.task_cap);
-  const u32* to = J.toff + (u64)pw * (s.nb + 2);
-  if (t >= to[s.nb + 1]) return;
-  u32 b = find_bucket(to, s.nb, t);
-  u32 k = t - to[b];
-  u32 lo = J.bstart[(u64)pw * (s.nb + 1) + b] + k * s.T;
-  u32 be = J.bend[(u64)pw * (s.nb + 1) + b];
-  u32 hi = lo + s.T < be ? lo + s.T : be;
+  const u32 t = (u32)(gi - (u64)pw * s.task_cap);
+  u32 lo, hi;
+  if (!task_range(J, pw, t, lo, hi)) return;
   const u32* vv = sorted_vals(J) + (u64)pw * s.seg_len;
   G1Xyzz29 acc = G1Xyzz29::identity();
-  u32 e = vv[lo];
-  G1Aff29 p = load_aff_table(J.bases + 16 * (u64)(e & 0x7fffffffu));
+  G1Aff29 p = entry_point(J, vv[lo]);
   for (u32 j = lo; j < hi; j++) {
-    G1Aff29 cur = p;
-    u32 ce = e;
-    if (j + 1 < hi) {
-      e = vv[j + 1];
-      p = load_aff_table(J.bases + 16 * (u64)(e & 0x7fffffffu));
-    }
-    if (ce & 0x80000000u) cur = aff29_neg(cur);
+    const G1Aff29 cur = p;
+    if (j + 1 < hi) p = entry_point(J, vv[j + 1]);
     acc = xyzz29_madd(acc, cur);
   }
   store_xyzz29(J.partA + kPt * ((u64)pw * s.task_cap + t), acc);

@@ -165,3 +165,19 @@ def test_msm_selectors_2_18(stream):
         stream.sync()
         expect = O.msm_closed_form(to_host(scal), k)
         assert (M.msm_device(bases, scal, n, 15, stream) == expect).all()
+
+
+@pytest.mark.parametrize("c,target,task", [(20, 1, 32), (16, 0, 64)])
+def test_msm_2_24_closed_form(stream, c, target, task):
+    # the largest size the benchmarks do not reach: 2^24 bases k_i * G,
+    # uniform scalars, checked through (sum s_i k_i) * G
+    n = 1 << 24
+    nt = M.n_tables(c, target)
+    tables = device_u32(nt * n * 16)
+    _lib.call("impeller_gen_bases", tables.data_ptr(), n, 41, 0, stream.raw())
+    M.precompute_device(tables, n, c, target, stream)
+    scal = device_u32(n * 8)
+    _lib.call("impeller_gen_scalars", scal.data_ptr(), n, 0, 42, 0, stream.raw())
+    stream.sync()
+    got = M.msm_device_cfg(tables, scal, n, c, target, stream, task=task)
+    assert (got == O.msm_closed_form(to_host(scal), O.gen_scalars(n, 41))).all()
