@@ -120,13 +120,6 @@ __device__ __forceinline__ void bfly2_v(Fr& a, Fr& b, u64 e, const Twiddle& tw, 
   a = s;
 }
 
-__device__ __forceinline__ void bfly2(u32* pa, u32* pb, u64 e, const Twiddle& tw, bool two) {
-  Fr a = load_fe<Fr>(pa), b = load_fe<Fr>(pb);
-  bfly2_v(a, b, e, tw, two);
-  store_fe<Fr>(pa, a);
-  store_fe<Fr>(pb, b);
-}
-
 // Two consecutive DIT levels on positions g0 < g1 = g0 + 2^j < g2 = g0 +
 // 2^(j+1) < g3: level j pairs (g0,g1),(g2,g3) with exponent e1; level j+1
 // pairs (g0,g2) with e2 and (g1,g3) with e3.  The same butterflies, in the
@@ -146,16 +139,6 @@ __device__ __forceinline__ void bfly4_v(Fr& a0, Fr& a1, Fr& a2, Fr& a3, u64 e1, 
   a2 = sub<FrParams>(b0, b2);
   a1 = add<FrParams>(b1, b3);
   a3 = sub<FrParams>(b1, b3);
-}
-
-__device__ __forceinline__ void bfly4(u32* p0, u32* p1, u32* p2, u32* p3, u64 e1, u64 e2, u64 e3,
-                                      const Twiddle& tw, bool two) {
-  Fr a0 = load_fe<Fr>(p0), a1 = load_fe<Fr>(p1), a2 = load_fe<Fr>(p2), a3 = load_fe<Fr>(p3);
-  bfly4_v(a0, a1, a2, a3, e1, e2, e3, tw, two);
-  store_fe<Fr>(p0, a0);
-  store_fe<Fr>(p1, a1);
-  store_fe<Fr>(p2, a2);
-  store_fe<Fr>(p3, a3);
 }
 
 // Planar LDS tiles: slot s of a buffer of ns slots keeps its low 16 bytes at
@@ -323,6 +306,12 @@ __device__ __forceinline__ void br_body(const NttJob& J, u64 lb, u32* lds) {
   }
 }
 
+// 16-byte slot swizzle of the DIT tiles (planar storage): the low four slot
+// bits are XORed with the next eight, so the power-of-two slot strides of the
+// butterfly levels and of the load / store phases spread over the 16 slot
+// groups of a bank row; a bijection on every power-of-two range >= 16.
+__device__ __forceinline__ u32 dslot(u32 s) { return s ^ (((s >> 4) ^ (s >> 8)) & 15u); }
+
 // LDS pass over levels s .. s+D-1 on positions p = hi*2^(s+D) + g*2^s + lo.
 // contiguous (s == 0): work-group = C consecutive groups of G positions.
 // strided (2^s >= C): work-group = fixed hi, C consecutive lo.  The last pass
@@ -347,6 +336,7 @@ __device__ __forceinline__ void dit_body(const NttJob& J, const NttStage& S, boo
       lo0 = (wg & ((1ull << lg_lo) - 1)) << logC;
     }
   }
+  const Planar T{t, GC};
   __syncthreads();
   for (u32 idx = threadIdx.x; idx < GC; idx += blockDim.x) {
     u32 g, c;
@@ -360,7 +350,7 @@ __device__ __forceinline__ void dit_body(const NttJob& J, const NttStage& S, boo
       c = idx & (C - 1);
       p = (hi << (s + D)) | ((u64)g << s) | (lo0 + c);
     }
-    store_fe<Fr>(t + 8 * (g * C + c), load_fe<Fr>(J.x.base + 8 * view_phys(J.x, p)));
+    pl_store(T, dslot(g * C + c), load_fe<Fr>(J.x.base + 8 * view_phys(J.x, p)));
   }
   __syncthreads();
   u32 j = 0;
@@ -374,9 +364,14 @@ __device__ __forceinline__ void dit_body(const NttJob& J, const NttStage& S, boo
       u64 lo = contig ? 0 : lo0 + c;
       u64 base = (((u64)glo) << s) + lo;
       u64 e1 = base << (sh1 + 1), e2 = base << sh1, e3 = (base + (1ull << l)) << sh1;
-      bfly4(t + 8 * (g0 * C + c), t + 8 * ((g0 | (1u << j)) * C + c),
-            t + 8 * ((g0 | (2u << j)) * C + c), t + 8 * ((g0 | (3u << j)) * C + c), e1, e2, e3,
-            J.tw, two);
+      const u32 s0 = dslot(g0 * C + c), s1 = dslot((g0 | (1u << j)) * C + c);
+      const u32 s2 = dslot((g0 | (2u << j)) * C + c), s3 = dslot((g0 | (3u << j)) * C + c);
+      Fr a0 = pl_load(T, s0), a1 = pl_load(T, s1), a2 = pl_load(T, s2), a3 = pl_load(T, s3);
+      bfly4_v(a0, a1, a2, a3, e1, e2, e3, J.tw, two);
+      pl_store(T, s0, a0);
+      pl_store(T, s1, a1);
+      pl_store(T, s2, a2);
+      pl_store(T, s3, a3);
     }
     __syncthreads();
   }
@@ -389,7 +384,11 @@ __device__ __forceinline__ void dit_body(const NttJob& J, const NttStage& S, boo
       u32 g0 = ((k >> j) << (j + 1)) | glo, g1 = g0 | (1u << j);
       u64 lo = contig ? 0 : lo0 + c;
       u64 e = ((((u64)glo) << s) + lo) << sh;
-      bfly2(t + 8 * (g0 * C + c), t + 8 * (g1 * C + c), e, J.tw, two);
+      const u32 s0 = dslot(g0 * C + c), s1 = dslot(g1 * C + c);
+      Fr a0 = pl_load(T, s0), a1 = pl_load(T, s1);
+      bfly2_v(a0, a1, e, J.tw, two);
+      pl_store(T, s0, a0);
+      pl_store(T, s1, a1);
     }
     __syncthreads();
   }
@@ -406,7 +405,7 @@ __device__ __forceinline__ void dit_body(const NttJob& J, const NttStage& S, boo
       c = idx & (C - 1);
       p = (hi << (s + D)) | ((u64)g << s) | (lo0 + c);
     }
-    Fr v = load_fe<Fr>(t + 8 * (g * C + c));
+    Fr v = pl_load(T, dslot(g * C + c));
     if (epi) v = mul<FrParams>(v, factor_at(J.epi, p));
     store_fe<Fr>(J.x.base + 8 * view_phys(J.x, p), v);
   }
