@@ -55,6 +55,31 @@ inline bool make_view(const PolyPtr& p, View* v) {
   return true;
 }
 
+// Global-memory access through a pointer whose address space the compiler
+// cannot see (one read from a job descriptor in memory): a plain access would
+// be a flat one, which also counts against the LDS wait counter and so gets
+// ordered against the kernel's LDS operations.  Every buffer the kernels
+// address through such pointers is global memory.
+#ifdef IMP_EMU
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {
+  return *p;
+}
+template <class T>
+__device__ __forceinline__ void gst(T* p, T v) {
+  *p = v;
+}
+#else
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ void gst(T* p, T v) {
+  *(__attribute__((address_space(1))) T*)p = v;
+}
+#endif
+
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // The launch's dynamic LDS as a u32 pointer (one definition per kernel).

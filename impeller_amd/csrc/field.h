@@ -109,17 +109,24 @@ __device__ __forceinline__ F load_fe_nt(const u32* p) {
 }
 
 // --------------------------------------------------------------- arithmetic
+// 32-bit add (subtract) with carry (borrow) in and out, carries 0 or 1.  A
+// chain of these compiles to one v_add_co / v_addc_co (v_sub_co / v_subb_co)
+// per limb; the same chain written with 64-bit intermediates costs 64-bit
+// adds, shifts and moves per limb.
+__host__ __device__ __forceinline__ u32 adc(u32 a, u32 b, u32 cin, u32* cout) {
+  return __builtin_addc(a, b, cin, cout);
+}
+__host__ __device__ __forceinline__ u32 sbb(u32 a, u32 b, u32 bin, u32* bout) {
+  return __builtin_subc(a, b, bin, bout);
+}
+
 // r = a - m if a >= m else a   (a < 2m)
 template <class P>
 __device__ __forceinline__ Fp<P> reduce_once(const Fp<P>& a) {
   Fp<P> d;
   u32 borrow = 0;
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    u64 t = (u64)a.v[i] - P::m[i] - borrow;
-    d.v[i] = (u32)t;
-    borrow = (u32)(t >> 63);
-  }
+  for (int i = 0; i < 8; i++) d.v[i] = sbb(a.v[i], P::m[i], borrow, &borrow);
   Fp<P> r;
 #pragma unroll
   for (int i = 0; i < 8; i++) r.v[i] = borrow ? a.v[i] : d.v[i];
@@ -132,11 +139,7 @@ __device__ __forceinline__ Fp<P> add(const Fp<P>& a, const Fp<P>& b) {
   Fp<P> s;
   u32 c = 0;
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    u64 t = (u64)a.v[i] + b.v[i] + c;
-    s.v[i] = (u32)t;
-    c = (u32)(t >> 32);
-  }
+  for (int i = 0; i < 8; i++) s.v[i] = adc(a.v[i], b.v[i], c, &c);
   return reduce_once<P>(s);
 }
 
@@ -145,20 +148,13 @@ __device__ __forceinline__ Fp<P> sub(const Fp<P>& a, const Fp<P>& b) {
   Fp<P> d;
   u32 borrow = 0;
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    u64 t = (u64)a.v[i] - b.v[i] - borrow;
-    d.v[i] = (u32)t;
-    borrow = (u32)(t >> 63);
-  }
+  for (int i = 0; i < 8; i++) d.v[i] = sbb(a.v[i], b.v[i], borrow, &borrow);
   // add back m when the subtraction wrapped
+  const u32 mask = 0u - borrow;
   Fp<P> r;
   u32 c = 0;
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    u64 t = (u64)d.v[i] + (borrow ? P::m[i] : 0u) + c;
-    r.v[i] = (u32)t;
-    c = (u32)(t >> 32);
-  }
+  for (int i = 0; i < 8; i++) r.v[i] = adc(d.v[i], P::m[i] & mask, c, &c);
   return r;
 }
 
@@ -168,11 +164,7 @@ __device__ __forceinline__ Fp<P> neg(const Fp<P>& a) {
   Fp<P> r;
   u32 borrow = 0;
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    u64 t = (u64)P::m[i] - a.v[i] - borrow;
-    r.v[i] = (u32)t;
-    borrow = (u32)(t >> 63);
-  }
+  for (int i = 0; i < 8; i++) r.v[i] = sbb(P::m[i], a.v[i], borrow, &borrow);
   return r;
 }
 
@@ -379,11 +371,7 @@ __device__ __forceinline__ Fp<P> inv_fermat(const Fp<P>& a) {
   u32 e[8];
   u32 borrow = 0;
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    u64 t = (u64)P::m[i] - (i == 0 ? 2u : 0u) - borrow;
-    e[i] = (u32)t;
-    borrow = (u32)(t >> 63);
-  }
+  for (int i = 0; i < 8; i++) e[i] = sbb(P::m[i], i == 0 ? 2u : 0u, borrow, &borrow);
   return pow_big<P>(a, e);
 }
 
@@ -394,11 +382,7 @@ __device__ __forceinline__ void half_mod(u32 x[8]) {
   u32 c = 0;
   u32 s[8];
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    u64 t = (u64)x[i] + (odd ? P::m[i] : 0u) + c;
-    s[i] = (u32)t;
-    c = (u32)(t >> 32);
-  }
+  for (int i = 0; i < 8; i++) s[i] = adc(x[i], odd ? P::m[i] : 0u, c, &c);
 #pragma unroll
   for (int i = 0; i < 7; i++) x[i] = (s[i] >> 1) | (s[i + 1] << 31);
   x[7] = s[7] >> 1;
@@ -409,18 +393,11 @@ template <class P>
 __device__ __forceinline__ void sub_mod_limbs(u32 x[8], const u32 y[8]) {
   u32 borrow = 0;
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    u64 t = (u64)x[i] - y[i] - borrow;
-    x[i] = (u32)t;
-    borrow = (u32)(t >> 63);
-  }
+  for (int i = 0; i < 8; i++) x[i] = sbb(x[i], y[i], borrow, &borrow);
+  const u32 mask = 0u - borrow;
   u32 c = 0;
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    u64 t = (u64)x[i] + (borrow ? P::m[i] : 0u) + c;
-    x[i] = (u32)t;
-    c = (u32)(t >> 32);
-  }
+  for (int i = 0; i < 8; i++) x[i] = adc(x[i], P::m[i] & mask, c, &c);
 }
 
 // Inversion by the binary extended Euclidean algorithm (Hankerson, Menezes
@@ -465,11 +442,7 @@ __device__ __forceinline__ Fp<P> inv(const Fp<P>& a_in) {
       // both odd: subtract the smaller from the larger (the difference is even)
       u32 d[8], borrow = 0;
 #pragma unroll
-      for (int i = 0; i < 8; i++) {
-        u64 t = (u64)u[i] - v[i] - borrow;
-        d[i] = (u32)t;
-        borrow = (u32)(t >> 63);
-      }
+      for (int i = 0; i < 8; i++) d[i] = sbb(u[i], v[i], borrow, &borrow);
       if (!borrow) {
 #pragma unroll
         for (int i = 0; i < 7; i++) u[i] = (d[i] >> 1) | (d[i + 1] << 31);
@@ -479,11 +452,7 @@ __device__ __forceinline__ Fp<P> inv(const Fp<P>& a_in) {
       } else {
         borrow = 0;
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-          u64 t = (u64)v[i] - u[i] - borrow;
-          d[i] = (u32)t;
-          borrow = (u32)(t >> 63);
-        }
+        for (int i = 0; i < 8; i++) d[i] = sbb(v[i], u[i], borrow, &borrow);
 #pragma unroll
         for (int i = 0; i < 7; i++) v[i] = (d[i] >> 1) | (d[i + 1] << 31);
         v[7] = d[7] >> 1;
@@ -716,10 +685,7 @@ __device__ __forceinline__ Fr gen_uniform_fr(u64 seed, u64 i) {
     // accept if r < m
     u32 borrow = 0;
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-      u64 t = (u64)r.v[j] - FrParams::m[j] - borrow;
-      borrow = (u32)(t >> 63);
-    }
+    for (int j = 0; j < 8; j++) (void)sbb(r.v[j], FrParams::m[j], borrow, &borrow);
     if (borrow) return r;
   }
 }

@@ -251,12 +251,20 @@ __device__ __forceinline__ void hist_body(const MsmJob& J, u32 pass, u64 lb, u32
   __syncthreads();
   const u32* kk = keys + (u64)pw * s.seg_len;
   const u64 tile0 = (u64)tile * kSortTile;
+  // all loads first (one wait), then the LDS counts
+  u32 k[kSortTile / 256];
+#pragma unroll
   for (u32 j = 0; j < kSortTile / 256; j++) {
-    u64 idx = tile0 + (u64)j * 256 + t;
-    if (idx < s.seg_len) atomicAdd(&cnt[(kk[idx] >> shift) & (kRadix - 1)], 1u);
+    const u64 idx = tile0 + (u64)j * 256 + t;
+    k[j] = idx < s.seg_len ? gld(kk + idx) : 0u;
+  }
+#pragma unroll
+  for (u32 j = 0; j < kSortTile / 256; j++) {
+    const u64 idx = tile0 + (u64)j * 256 + t;
+    if (idx < s.seg_len) atomicAdd(&cnt[(k[j] >> shift) & (kRadix - 1)], 1u);
   }
   __syncthreads();
-  J.hist[((u64)pw * kRadix + t) * s.tiles + tile] = cnt[t];
+  gst(J.hist + ((u64)pw * kRadix + t) * s.tiles + tile, cnt[t]);
 }
 
 // block-wide exclusive scan of len entries (in place allowed); with
@@ -379,8 +387,8 @@ __device__ __forceinline__ void scatter_body(const MsmJob& J, u32 pass, u64 lb, 
 #pragma unroll
   for (u32 j = 0; j < 16; j++) {
     if (j < n) {
-      key[j] = ksrc[seg0 + tile0 + 16 * t + j];
-      val[j] = vsrc[seg0 + tile0 + 16 * t + j];
+      key[j] = gld(ksrc + seg0 + tile0 + 16 * t + j);
+      val[j] = gld(vsrc + seg0 + tile0 + 16 * t + j);
     }
   }
   split_nibble(L, key, val, n, shift);
@@ -397,7 +405,7 @@ __device__ __forceinline__ void scatter_body(const MsmJob& J, u32 pass, u64 lb, 
     split_nibble(L, key, val, n, shift + 4);
   }
   // digit run starts in the staged order, and the runs' global starts
-  L.gbase[t] = J.hist[((u64)pw * kRadix + t) * s.tiles + tile];
+  L.gbase[t] = gld(J.hist + ((u64)pw * kRadix + t) * s.tiles + tile);
   for (u32 j = 0; j < n; j++) {
     const u32 i = 16 * t + j;
     const u32 d = (L.sk[swz(i)] >> shift) & (kRadix - 1);
@@ -408,8 +416,8 @@ __device__ __forceinline__ void scatter_body(const MsmJob& J, u32 pass, u64 lb, 
     const u32 k = L.sk[swz(i)];
     const u32 d = (k >> shift) & (kRadix - 1);
     const u64 pos = (u64)L.gbase[d] + i - L.dbase[d];
-    kdst[seg0 + pos] = k;
-    vdst[seg0 + pos] = L.sv[swz(i)];
+    gst(kdst + seg0 + pos, k);
+    gst(vdst + seg0 + pos, L.sv[swz(i)]);
   }
 }
 
@@ -778,6 +786,27 @@ static inline u32 scan_arg(u32 kind, u32 x) { return kind | (x << 8); }
     BODY;                                                           \
   }
 
+// Work-groups are dealt to the 8 XCDs round-robin (block b runs on XCD
+// b mod 8).  xcd_order(b, n) renumbers the blocks of a one-block-per-item
+// grid so that every XCD takes one contiguous range of [0, n): consecutive
+// sort tiles then share one L2, where the adjacent partial-line runs they
+// write for one digit merge before going to HBM.  A bijection on [0, n).
+__device__ __forceinline__ u32 xcd_order(u32 b, u32 n) {
+  const u32 x = b & 7, k = b >> 3, q = n >> 3, r = n & 7;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+#define IMP_BATCH_LOOP_XCD(BODY)                                    \
+  const u32 total = prefix[njobs];                                  \
+  const bool remap = gridDim.x >= total;                            \
+  for (u32 t0 = blockIdx.x; t0 < total; t0 += gridDim.x) {          \
+    const u32 t = remap ? xcd_order(t0, total) : t0;                \
+    u32 j = find_job(prefix, njobs, t);                             \
+    const MsmJob& J = jobs[j];                                      \
+    const u64 lb = t - prefix[j];                                   \
+    BODY;                                                           \
+  }
+
 __global__ __launch_bounds__(256) void kb_convert(const MsmJob* jobs, u32 njobs, const u32* prefix) {
   IMP_BATCH_LOOP(convert_body(J, lb * blockDim.x + threadIdx.x))
 }
@@ -807,7 +836,7 @@ __global__ __launch_bounds__(256) void kb_scan_c(const MsmJob* jobs, u32 njobs,
 __global__ __launch_bounds__(256) void kb_scatter(const MsmJob* jobs, u32 njobs,
                                                   const u32* prefix, u32 pass) {
   __shared__ ScatterLds lds;
-  IMP_BATCH_LOOP(scatter_body(J, pass, lb, lds))
+  IMP_BATCH_LOOP_XCD(scatter_body(J, pass, lb, lds))
 }
 __global__ __launch_bounds__(256) void kb_zero(const MsmJob* jobs, u32 njobs, const u32* prefix) {
   IMP_BATCH_LOOP(zero_body(J, lb * blockDim.x + threadIdx.x))

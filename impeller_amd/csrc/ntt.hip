@@ -113,6 +113,31 @@ __device__ __forceinline__ Fr tw_load(const Twiddle& tw, bool two, u64 e) {
   return two ? twiddle_at<true>(tw, e) : twiddle_at<false>(tw, e);
 }
 
+// Butterflies with the twiddles already in registers (w1 unused when !m1,
+// w2 when !m2: those twiddles are 1).
+__device__ __forceinline__ void bfly2_w(Fr& a, Fr& b, bool m, const Fr& w) {
+  if (m) b = mul<FrParams>(b, w);
+  Fr s = add<FrParams>(a, b);
+  b = sub<FrParams>(a, b);
+  a = s;
+}
+
+__device__ __forceinline__ void bfly4_w(Fr& a0, Fr& a1, Fr& a2, Fr& a3, bool m1, bool m2,
+                                        const Fr& w1, const Fr& w2, const Fr& w3) {
+  if (m1) {
+    a1 = mul<FrParams>(a1, w1);
+    a3 = mul<FrParams>(a3, w1);
+  }
+  Fr b0 = add<FrParams>(a0, a1), b1 = sub<FrParams>(a0, a1);
+  Fr b2 = add<FrParams>(a2, a3), b3 = sub<FrParams>(a2, a3);
+  if (m2) b2 = mul<FrParams>(b2, w2);
+  b3 = mul<FrParams>(b3, w3);
+  a0 = add<FrParams>(b0, b2);
+  a2 = sub<FrParams>(b0, b2);
+  a1 = add<FrParams>(b1, b3);
+  a3 = sub<FrParams>(b1, b3);
+}
+
 __device__ __forceinline__ void bfly2_v(Fr& a, Fr& b, u64 e, const Twiddle& tw, bool two) {
   if (e != 0) b = mul<FrParams>(b, tw_load(tw, two, e));
   Fr s = add<FrParams>(a, b);
@@ -316,10 +341,13 @@ __device__ __forceinline__ u32 dslot(u32 s) { return s ^ (((s >> 4) ^ (s >> 8)) 
 // contiguous (s == 0): work-group = C consecutive groups of G positions.
 // strided (2^s >= C): work-group = fixed hi, C consecutive lo.  The last pass
 // multiplies each output by the epilogue factor (n^-1, coset un-shift).
+// TWO: two-level twiddle table (J.two).  Each butterfly issues its twiddle
+// loads before its LDS reads, so one wait covers both.
+template <bool TWO>
 __device__ __forceinline__ void dit_body(const NttJob& J, const NttStage& S, bool last, u64 wg, u32* t) {
   const u32 L = J.L, s = S.s, D = S.D, logC = S.logC;
   const u32 C = 1u << logC, G = 1u << D, GC = G * C;
-  const bool contig = (s == 0), two = J.two != 0;
+  const bool contig = (s == 0);
   u64 hi = 0, lo0 = 0;
   if (!contig) {
     // strided: the twiddles depend on lo only.  Work-groups are dealt to the
@@ -366,8 +394,10 @@ __device__ __forceinline__ void dit_body(const NttJob& J, const NttStage& S, boo
       u64 e1 = base << (sh1 + 1), e2 = base << sh1, e3 = (base + (1ull << l)) << sh1;
       const u32 s0 = dslot(g0 * C + c), s1 = dslot((g0 | (1u << j)) * C + c);
       const u32 s2 = dslot((g0 | (2u << j)) * C + c), s3 = dslot((g0 | (3u << j)) * C + c);
+      const Fr w1 = twiddle_at<TWO>(J.tw, e1), w2 = twiddle_at<TWO>(J.tw, e2);
+      const Fr w3 = twiddle_at<TWO>(J.tw, e3);
       Fr a0 = pl_load(T, s0), a1 = pl_load(T, s1), a2 = pl_load(T, s2), a3 = pl_load(T, s3);
-      bfly4_v(a0, a1, a2, a3, e1, e2, e3, J.tw, two);
+      bfly4_w(a0, a1, a2, a3, e1 != 0, e2 != 0, w1, w2, w3);
       pl_store(T, s0, a0);
       pl_store(T, s1, a1);
       pl_store(T, s2, a2);
@@ -385,8 +415,9 @@ __device__ __forceinline__ void dit_body(const NttJob& J, const NttStage& S, boo
       u64 lo = contig ? 0 : lo0 + c;
       u64 e = ((((u64)glo) << s) + lo) << sh;
       const u32 s0 = dslot(g0 * C + c), s1 = dslot(g1 * C + c);
+      const Fr w = twiddle_at<TWO>(J.tw, e);
       Fr a0 = pl_load(T, s0), a1 = pl_load(T, s1);
-      bfly2_v(a0, a1, e, J.tw, two);
+      bfly2_w(a0, a1, e != 0, w);
       pl_store(T, s0, a0);
       pl_store(T, s1, a1);
     }
@@ -417,9 +448,10 @@ __global__ __launch_bounds__(256, 4) void k_ntt_br(NttJob J) {
   br_body(J, blockIdx.x, lds);
 }
 
+template <bool TWO>
 __global__ __launch_bounds__(256, 4) void k_ntt_dit(NttJob J, u32 stage) {
   IMP_DYNAMIC_LDS(lds);
-  dit_body(J, J.st[stage], stage + 1 == J.nst, blockIdx.x, lds);
+  dit_body<TWO>(J, J.st[stage], stage + 1 == J.nst, blockIdx.x, lds);
 }
 
 __device__ __forceinline__ u32 find_job(const u32* prefix, u32 njobs, u32 t) {
@@ -452,7 +484,8 @@ __global__ __launch_bounds__(256, 4) void k_ntt_dit_batch(const NttJob* jobs, u3
   for (u32 t = blockIdx.x; t < total; t += gridDim.x) {
     u32 j = find_job(prefix, njobs, t);
     const NttJob& J = jobs[j];
-    dit_body(J, J.st[stage], stage + 1 == J.nst, t - prefix[j], lds);
+    if (J.two) dit_body<true>(J, J.st[stage], stage + 1 == J.nst, t - prefix[j], lds);
+    else dit_body<false>(J, J.st[stage], stage + 1 == J.nst, t - prefix[j], lds);
   }
 }
 
@@ -607,8 +640,12 @@ static hipError_t run_job(const NttJob& J, hipStream_t stream) {
   IMP_CHECK(hipGetLastError());
   for (u32 st = 0; st < J.nst; st++) {
     TimedScope ts(K_NTT_DIT, stream);
-    hipLaunchKernelGGL(k_ntt_dit, dim3(J.st[st].blocks), dim3(256), stage_lds(J, st), stream, J,
-                       st);
+    if (J.two)
+      hipLaunchKernelGGL(k_ntt_dit<true>, dim3(J.st[st].blocks), dim3(256), stage_lds(J, st),
+                         stream, J, st);
+    else
+      hipLaunchKernelGGL(k_ntt_dit<false>, dim3(J.st[st].blocks), dim3(256), stage_lds(J, st),
+                         stream, J, st);
     IMP_CHECK(hipGetLastError());
   }
   return hipSuccess;
