@@ -44,9 +44,9 @@ def parse():
 
 def dit_traffic(log_n):
     """HBM bytes per k_ntt_dit launch from the committed counter summary
-    (profiles/ntt<log>_dit_hbm_traffic.md, line "traffic_bytes_per_launch: B"),
+    (profiles/ntt<log>_dit_hbm_traffic.txt, line "traffic_bytes_per_launch: B"),
     or None."""
-    path = os.path.join(ROOT, "profiles", f"ntt{log_n}_dit_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", f"ntt{log_n}_dit_hbm_traffic.txt")
     if not os.path.exists(path):
         return None
     with open(path) as f:

@@ -8,8 +8,9 @@ problem without a sweep; the exhaustive result stays the fallback and the
 judge of the pick's regret.
 
 Configuration spaces:
-  NTT  plan = [fuse, lds_log, d0, d1, ...]  (fuse in {0, 4, 5}, lds_log in
-       {10, 11}, passes of the remaining levels)
+  NTT  plan = [fuse, lds_log, d0, d1, ...]  (fuse in {0, 4, 5}: DIT levels
+       done in the bit-reversal kernel, or 1: bit reversal fused into the
+       first pass; lds_log in {10, 11}; passes of the remaining levels)
   MSM  (window c, target windows (0 = per-window buckets, 1 = precomputed
        window tables), task size T)
 """
@@ -27,22 +28,38 @@ N_FEATURES = 24
 def ntt_plans(L, max_passes=4):
     """Candidate plans for a 2^L transform."""
     out = []
-    for fuse in ((0, 4, 5) if L >= 10 else ((0, 4) if L >= 8 else (0,))):
-        rest = L - fuse
+    for fuse in ((0, 1, 4, 5) if L >= 10 else ((0, 1, 4) if L >= 8 else (0, 1))):
+        rest = L - fused_levels(fuse)
         for lds_log in (10, 11):
-            for k in range(1, max_passes + 1):
+            for k in range(2 if fuse == 1 else 1, max_passes + 1):
                 lo = max(1, rest // k - 2)
                 for parts in itertools.product(range(lo, lds_log + 1), repeat=k):
                     if sum(parts) != rest:
                         continue
-                    # keep non-increasing splits only (order matters little)
-                    if list(parts) != sorted(parts, reverse=True):
+                    # keep non-increasing splits only (order matters little),
+                    # except the first pass of a fused reversal: it reads runs
+                    # of 2^(lds_log - d0) elements, so it is kept at least 4
+                    # columns wide and may be shallower than the rest
+                    tail = parts[1:] if fuse == 1 else parts
+                    if list(tail) != sorted(tail, reverse=True):
+                        continue
+                    if fuse == 1 and lds_log - parts[0] < min(2, rest - parts[0]):
                         continue
                     out.append([fuse, lds_log, *parts])
     return out
 
 
+def fused_levels(fuse):
+    """DIT levels the plan's first entry takes out of the LDS passes."""
+    return fuse if fuse in (4, 5) else 0
+
+
 def default_ntt_plan(L):
+    """The library's default plan (default_plan in csrc/ntt.hip)."""
+    if L >= 20:
+        rest = L - 8
+        np_ = (rest + 8) // 9
+        return [1, 10, 8] + [rest // np_ + (1 if i < rest % np_ else 0) for i in range(np_)]
     fuse = 4 if L >= 8 else 0
     rest = L - fuse
     np_ = max(1, (rest + 7) // 8)
@@ -64,10 +81,11 @@ def default_msm(log_n):
 
 
 # --------------------------------------------------------------- features
-# Slots 0-3: op one-hot; 4: log2 size; 15: log2 batch.  NTT: 5 fuse / 5,
-# 6 LDS tile log - 10, 7 passes, 8/9 deepest/shallowest pass, 16-19 pass
-# depths (sorted, zero padded), 20 levels after the fused ones, 21 passes
-# deeper than the LDS tile.  MSM: 10 window c, 11 precomputed tables,
+# Slots 0-3: op one-hot; 4: log2 size; 15: log2 batch.  NTT: 5 fused levels
+# / 5, 6 LDS tile log - 10, 7 passes, 8/9 deepest/shallowest pass, 16-19
+# pass depths (sorted, zero padded), 20 levels after the fused ones, 21
+# passes deeper than the LDS tile, 22 bit reversal fused into the first
+# pass, 23 log2 column run of that pass.  MSM: 10 window c, 11 precomputed tables,
 # 12 log2 task, 13/14/7 fraction of zero / one / small scalars, then the
 # derived shape: 16 log2 entries per bucket set, 17 log2 entries per bucket,
 # 18 sort passes, 19 bucket sets, 20 log2 tasks per bucket, 21 log2 buckets.
@@ -75,7 +93,8 @@ def ntt_features(op, L, plan, batch=1):
     f = np.zeros(N_FEATURES, np.float32)
     f[OPS.index(op)] = 1
     f[4] = L
-    f[5] = plan[0] / 5.0
+    fl = fused_levels(plan[0])
+    f[5] = fl / 5.0
     f[6] = plan[1] - 10
     d = list(plan[2:])
     f[7] = len(d)
@@ -83,8 +102,11 @@ def ntt_features(op, L, plan, batch=1):
     f[9] = min(d)
     dd = sorted(d, reverse=True)[:4]
     f[16:16 + len(dd)] = dd
-    f[20] = L - plan[0]
+    f[20] = L - fl
     f[21] = sum(1 for x in d if x > plan[1])
+    if plan[0] == 1:
+        f[22] = 1
+        f[23] = min(plan[1] - d[0], L - d[0])
     f[15] = np.log2(batch)
     return f
 

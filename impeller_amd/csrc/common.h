@@ -59,8 +59,9 @@ inline bool make_view(const PolyPtr& p, View* v) {
 // cannot see (one read from a job descriptor in memory): a plain access would
 // be a flat one, which also counts against the LDS wait counter and so gets
 // ordered against the kernel's LDS operations.  Every buffer the kernels
-// address through such pointers is global memory.
-#ifdef IMP_EMU
+// address through such pointers is global memory.  (The host pass and the
+// CPU emulation build see plain accesses.)
+#if defined(IMP_EMU) || !defined(__HIP_DEVICE_COMPILE__)
 template <class T>
 __device__ __forceinline__ T gld(const T* p) {
   return *p;

@@ -569,8 +569,34 @@ __device__ __forceinline__ G1Aff29 entry_point(const MsmJob& J, u32 e) {
   return (e & 0x80000000u) ? aff29_neg(p) : p;
 }
 
+// A sorted entry's table point as stored (x then y, 8 words each) and its
+// sign: loaded one entry ahead, unpacked to limbs only when it is added.
+struct RawEntry {
+  uint4 q[4];
+  u32 neg;
+};
+
+__device__ __forceinline__ RawEntry load_entry(const MsmJob& J, u32 e) {
+  const uint4* q = reinterpret_cast<const uint4*>(J.bases + 16 * (u64)(e & 0x7fffffffu));
+  RawEntry r;
+#pragma unroll
+  for (int k = 0; k < 4; k++) r.q[k] = gld(q + k);
+  r.neg = e >> 31;
+  return r;
+}
+
+__device__ __forceinline__ G1Aff29 unpack_entry(const RawEntry& r) {
+  const u32 x[8] = {r.q[0].x, r.q[0].y, r.q[0].z, r.q[0].w, r.q[1].x, r.q[1].y, r.q[1].z, r.q[1].w};
+  const u32 y[8] = {r.q[2].x, r.q[2].y, r.q[2].z, r.q[2].w, r.q[3].x, r.q[3].y, r.q[3].z, r.q[3].w};
+  G1Aff29 p;
+  p.x = unpack29<Q29>(x);
+  p.y = unpack29<Q29>(y);
+  return r.neg ? aff29_neg(p) : p;
+}
+
 // one task = up to T consecutive entries of one bucket, mixed XYZZ + affine
-// additions; the next point is loaded while the current one is added
+// additions.  The next entry's point and the index after it are loaded before
+// the current addition, so their latency hides behind it.
 __device__ __forceinline__ void tasks_body(const MsmJob& J, u64 gi) {
   const MsmShape& s = J.s;
   const u32 pw = (u32)(gi / s.task_cap);
@@ -580,11 +606,14 @@ __device__ __forceinline__ void tasks_body(const MsmJob& J, u64 gi) {
   if (!task_range(J, pw, t, lo, hi)) return;
   const u32* vv = sorted_vals(J) + (u64)pw * s.seg_len;
   G1Xyzz29 acc = G1Xyzz29::identity();
-  G1Aff29 p = entry_point(J, vv[lo]);
+  RawEntry cur = load_entry(J, gld(vv + lo));
+  u32 next_e = lo + 1 < hi ? gld(vv + lo + 1) : 0u;
   for (u32 j = lo; j < hi; j++) {
-    const G1Aff29 cur = p;
-    if (j + 1 < hi) p = entry_point(J, vv[j + 1]);
-    acc = xyzz29_madd(acc, cur);
+    RawEntry nxt = cur;
+    if (j + 1 < hi) nxt = load_entry(J, next_e);
+    if (j + 2 < hi) next_e = gld(vv + j + 2);
+    acc = xyzz29_madd(acc, unpack_entry(cur));
+    cur = nxt;
   }
   store_xyzz29(J.partA + kPt * ((u64)pw * s.task_cap + t), acc);
 }
@@ -630,8 +659,8 @@ __device__ __forceinline__ void combine_body(const MsmJob& J, u32 level, u64 gi)
 // reduce: one thread per segment (running sums over m buckets);
 // sum:    lbits + 1 sums (the T_s, then each V_l), one or two tree levels;
 // final:  partials -> a_0 = sum T, a_1 .. a_(sbits-1) = 0, a_(sbits+l) = V_l,
-//         S = sum_i 2^i a_i by a doubling tree, then Horner over physical
-//         windows and affine.
+//         S = sum_i 2^i a_i (each a_i doubled by its own lane, then a
+//         tree of additions), then Horner over physical windows and affine.
 __device__ __forceinline__ void reduce_body(const MsmJob& J, u64 t) {
   const MsmShape& s = J.s;
   if (t >= (u64)s.n_windows * s.nseg) return;
@@ -735,15 +764,24 @@ __device__ __forceinline__ void final_body(const MsmJob& J, u32* red) {
     }
     __syncthreads();
   }
-  // doubling tree per window: a_i += 2^h a_(i+h) for i = 0 mod 2h
+  // S = sum_i 2^i a_i per window: lane (pw, i) doubles a_i i times (the
+  // lanes run side by side, so the chain is nv - 1 doublings deep), then a
+  // tree adds the nv terms
+  if (t < nval) {
+    const u32 i = t % nv;
+    if (i >= s.sbits) {  // a_1 .. a_(sbits-1) are the identity
+      G1Xyzz29 b = load_xyzz29(red + kPt * (t * G));
+      for (u32 k = 0; k < i; k++) b = xyzz29_dbl(b);
+      store_xyzz29(red + kPt * (t * G), b);
+    }
+  }
+  __syncthreads();
   for (u32 h = 1; h < nv; h <<= 1) {
     if (t < nval) {
       const u32 i = t % nv;
-      if ((i % (2 * h)) == 0 && i + h < nv) {
-        G1Xyzz29 b = load_xyzz29(red + kPt * ((t + h) * G));
-        for (u32 k = 0; k < h; k++) b = xyzz29_dbl(b);
-        store_xyzz29(red + kPt * (t * G), xyzz29_add(load_xyzz29(red + kPt * (t * G)), b));
-      }
+      if ((i % (2 * h)) == 0 && i + h < nv)
+        store_xyzz29(red + kPt * (t * G), xyzz29_add(load_xyzz29(red + kPt * (t * G)),
+                                                     load_xyzz29(red + kPt * ((t + h) * G))));
     }
     __syncthreads();
   }

@@ -9,10 +9,12 @@
 // omega^((p mod 2^l) * 2^(L-l-1)), a table entry below n/2, and computes
 // (a + w*b, a - w*b).  Only the grouping of levels into HBM passes differs.
 //
-// Passes (all in place, no workspace):
+// Passes (in place, no workspace, unless the bit reversal is fused):
 //   1. bit-reversal permutation: tile pairs (b, rev b) of 16 x 16 (or 32 x 32)
 //      elements swapped through LDS, optionally multiplying x[q] by shift^q
-//      (coset) and running the first 4 (or 5) DIT levels;
+//      (coset) and running the first 4 (or 5) DIT levels; or, fused into the
+//      first LDS pass, reads of the source at bit-reversed positions (that
+//      pass then writes a second buffer, which the next pass reads back);
 //   2. ceil(L / 8) DIT passes; each work-group stages a tile of G = 2^D
 //      positions x C columns in LDS and runs D consecutive levels, columns
 //      chosen so that every global access is a run of >= 4 consecutive
@@ -213,6 +215,8 @@ struct NttJob {
   Factor pro, epi;  // tab == nullptr: none
   u32 nst;
   u32 br_blocks;
+  u32 fbr;   // bit reversal fused into stage 0 (out of place through tmp)
+  u32* tmp;  // 2^L elements: stage 0 writes, stage 1 reads (fbr only)
   NttStage st[8];
 };
 
@@ -343,11 +347,21 @@ __device__ __forceinline__ u32 dslot(u32 s) { return s ^ (((s >> 4) ^ (s >> 8)) 
 // multiplies each output by the epilogue factor (n^-1, coset un-shift).
 // TWO: two-level twiddle table (J.two).  Each butterfly issues its twiddle
 // loads before its LDS reads, so one wait covers both.
+//
+// With the bit reversal fused (J.fbr), stage 0 takes for work-group m the
+// source positions q = (g' << (L-D)) | (m << logC) | c (runs of C elements),
+// i.e. the transform positions p = rev(q) = (rev c << (L-logC)) |
+// (rev m << D) | rev g': tile row g = rev g' of column c, the same tile as a
+// contiguous pass over those columns.  It writes p to J.tmp (runs of G
+// elements); stage 1 reads J.tmp and writes x.
 template <bool TWO>
-__device__ __forceinline__ void dit_body(const NttJob& J, const NttStage& S, bool last, u64 wg, u32* t) {
+__device__ __forceinline__ void dit_body(const NttJob& J, u32 stage, u64 wg, u32* t) {
+  const NttStage& S = J.st[stage];
+  const bool last = stage + 1 == J.nst;
   const u32 L = J.L, s = S.s, D = S.D, logC = S.logC;
   const u32 C = 1u << logC, G = 1u << D, GC = G * C;
   const bool contig = (s == 0);
+  const bool rd_rev = J.fbr && stage == 0, rd_tmp = J.fbr && stage == 1;
   u64 hi = 0, lo0 = 0;
   if (!contig) {
     // strided: the twiddles depend on lo only.  Work-groups are dealt to the
@@ -366,19 +380,28 @@ __device__ __forceinline__ void dit_body(const NttJob& J, const NttStage& S, boo
   }
   const Planar T{t, GC};
   __syncthreads();
-  for (u32 idx = threadIdx.x; idx < GC; idx += blockDim.x) {
-    u32 g, c;
-    u64 p;
-    if (contig) {
-      c = idx >> D;
-      g = idx & (G - 1);
-      p = (((wg << logC) + c) << D) | g;
-    } else {
-      g = idx >> logC;
-      c = idx & (C - 1);
-      p = (hi << (s + D)) | ((u64)g << s) | (lo0 + c);
+  if (rd_rev) {
+    for (u32 idx = threadIdx.x; idx < GC; idx += blockDim.x) {
+      const u32 c = idx & (C - 1), gq = idx >> logC;
+      const u64 q = ((u64)gq << (L - D)) | (wg << logC) | c;
+      pl_store(T, dslot(brev_bits(gq, D) * C + c), src_load(J, q));
     }
-    pl_store(T, dslot(g * C + c), load_fe<Fr>(J.x.base + 8 * view_phys(J.x, p)));
+  } else {
+    for (u32 idx = threadIdx.x; idx < GC; idx += blockDim.x) {
+      u32 g, c;
+      u64 p;
+      if (contig) {
+        c = idx >> D;
+        g = idx & (G - 1);
+        p = (((wg << logC) + c) << D) | g;
+      } else {
+        g = idx >> logC;
+        c = idx & (C - 1);
+        p = (hi << (s + D)) | ((u64)g << s) | (lo0 + c);
+      }
+      const u32* src = rd_tmp ? J.tmp + 8 * p : J.x.base + 8 * view_phys(J.x, p);
+      pl_store(T, dslot(g * C + c), load_fe<Fr>(src));
+    }
   }
   __syncthreads();
   u32 j = 0;
@@ -424,10 +447,17 @@ __device__ __forceinline__ void dit_body(const NttJob& J, const NttStage& S, boo
     __syncthreads();
   }
   const bool epi = last && J.epi.tab != nullptr;
+  // fused reversal: column c of work-group m holds p = (rev c << (L-logC)) |
+  // (rev m << D) | g
+  const u64 rev_m = rd_rev ? (u64)brev_bits((u32)wg, L - D - logC) << D : 0;
   for (u32 idx = threadIdx.x; idx < GC; idx += blockDim.x) {
     u32 g, c;
     u64 p;
-    if (contig) {
+    if (rd_rev) {
+      c = idx >> D;
+      g = idx & (G - 1);
+      p = ((u64)brev_bits(c, logC) << (L - logC)) | rev_m | g;
+    } else if (contig) {
       c = idx >> D;
       g = idx & (G - 1);
       p = (((wg << logC) + c) << D) | g;
@@ -438,7 +468,7 @@ __device__ __forceinline__ void dit_body(const NttJob& J, const NttStage& S, boo
     }
     Fr v = pl_load(T, dslot(g * C + c));
     if (epi) v = mul<FrParams>(v, factor_at(J.epi, p));
-    store_fe<Fr>(J.x.base + 8 * view_phys(J.x, p), v);
+    store_fe<Fr>(rd_rev ? J.tmp + 8 * p : J.x.base + 8 * view_phys(J.x, p), v);
   }
 }
 
@@ -451,7 +481,7 @@ __global__ __launch_bounds__(256, 4) void k_ntt_br(NttJob J) {
 template <bool TWO>
 __global__ __launch_bounds__(256, 4) void k_ntt_dit(NttJob J, u32 stage) {
   IMP_DYNAMIC_LDS(lds);
-  dit_body<TWO>(J, J.st[stage], stage + 1 == J.nst, blockIdx.x, lds);
+  dit_body<TWO>(J, stage, blockIdx.x, lds);
 }
 
 __device__ __forceinline__ u32 find_job(const u32* prefix, u32 njobs, u32 t) {
@@ -484,8 +514,8 @@ __global__ __launch_bounds__(256, 4) void k_ntt_dit_batch(const NttJob* jobs, u3
   for (u32 t = blockIdx.x; t < total; t += gridDim.x) {
     u32 j = find_job(prefix, njobs, t);
     const NttJob& J = jobs[j];
-    if (J.two) dit_body<true>(J, J.st[stage], stage + 1 == J.nst, t - prefix[j], lds);
-    else dit_body<false>(J, J.st[stage], stage + 1 == J.nst, t - prefix[j], lds);
+    if (J.two) dit_body<true>(J, stage, t - prefix[j], lds);
+    else dit_body<false>(J, stage, t - prefix[j], lds);
   }
 }
 
@@ -550,7 +580,8 @@ __global__ void k_ninv(u32* out, u32 L) {
 
 // ----------------------------------------------------------------- planner
 // fuse: DIT levels done inside the bit-reversal kernel (0; 4 with 16 x 16 tiles,
-// needs L >= 8; 5 with 32 x 32 tiles, needs L >= 10);
+// needs L >= 8; 5 with 32 x 32 tiles, needs L >= 10), or 1: no bit-reversal
+// kernel, the first LDS pass reads the source bit-reversed (needs >= 2 passes);
 // lds_log: log2 of the elements one work-group stages (10: 32 KiB, 11: 64 KiB);
 // d[]: level counts of the following LDS passes.
 struct PassPlan {
@@ -560,9 +591,21 @@ struct PassPlan {
   u32 d[8];
 };
 
+// Default: from 2^20 up, the bit reversal fused into a first pass of 8
+// levels (runs of 4 elements) and the rest in passes of at most 9 levels
+// (measured best or within a few percent of it for 2^20 .. 2^26); below,
+// 4 levels in the bit-reversal kernel and passes of at most 8.
 static void default_plan(u32 L, PassPlan* p) {
-  p->fuse = L >= 8 ? 4 : 0;
   p->lds_log = 10;
+  if (L >= 20) {
+    p->fuse = 1;
+    const u32 rest = L - 8, np = (rest + 8) / 9;
+    p->n = 1 + np;
+    p->d[0] = 8;
+    for (u32 i = 0; i < np; i++) p->d[1 + i] = rest / np + (i < rest % np ? 1 : 0);
+    return;
+  }
+  p->fuse = L >= 8 ? 4 : 0;
   u32 rest = L - p->fuse;
   u32 np = (rest + 7) / 8;
   if (np == 0) np = 1;
@@ -579,9 +622,11 @@ static bool plan_from(const unsigned int* plan, unsigned int n, u32 L, PassPlan*
   if (n < 3 || n > 10) return false;
   p->fuse = plan[0];
   p->lds_log = plan[1];
-  if (!(p->fuse == 0 || (p->fuse == 4 && L >= 8) || (p->fuse == 5 && L >= 10))) return false;
+  if (!(p->fuse == 0 || (p->fuse == 1 && n >= 4) || (p->fuse == 4 && L >= 8) ||
+        (p->fuse == 5 && L >= 10)))
+    return false;
   if (p->lds_log != 10 && p->lds_log != 11) return false;
-  u32 sum = p->fuse;
+  u32 sum = p->fuse == 1 ? 0 : p->fuse;
   p->n = n - 2;
   for (u32 i = 0; i < p->n; i++) {
     u32 d = plan[2 + i];
@@ -609,6 +654,8 @@ static NttJob make_job(View x, View src, u64 src_len, u32 L, const Twiddle& tw, 
   J.L = L;
   J.tb = plan.fuse == 5 ? 5 : 4;
   J.fuse = (plan.fuse == J.tb && L >= 2 * J.tb) ? J.tb : 0;
+  J.fbr = plan.fuse == 1 ? 1 : 0;
+  J.tmp = nullptr;
   J.two = two ? 1 : 0;
   J.tw = tw;
   J.pro = pro;
@@ -632,12 +679,32 @@ static NttJob make_job(View x, View src, u64 src_len, u32 L, const Twiddle& tw, 
   return J;
 }
 
-static hipError_t run_job(const NttJob& J, hipStream_t stream) {
-  {
+// A fused-reversal transform takes its 2^L-element second buffer from the
+// device's default stream-ordered pool.  The pool's release threshold is
+// raised once per device so that freed blocks stay in the pool: repeated
+// transforms then reuse the memory instead of mapping it on every call.
+static hipError_t keep_pool_memory() {
+  static std::mutex mu;
+  static bool done[64] = {};
+  int dev = 0;
+  IMP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  if (dev >= 0 && dev < 64 && !done[dev]) {
+    hipMemPool_t pool;
+    IMP_CHECK(hipDeviceGetDefaultMemPool(&pool, dev));
+    uint64_t keep = ~0ull;
+    IMP_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+    done[dev] = true;
+  }
+  return hipSuccess;
+}
+
+static hipError_t launch_stages(const NttJob& J, hipStream_t stream) {
+  if (!J.fbr) {
     TimedScope ts(K_NTT_BITREV, stream);
     hipLaunchKernelGGL(k_ntt_br, dim3(J.br_blocks), dim3(256), br_lds(J), stream, J);
+    IMP_CHECK(hipGetLastError());
   }
-  IMP_CHECK(hipGetLastError());
   for (u32 st = 0; st < J.nst; st++) {
     TimedScope ts(K_NTT_DIT, stream);
     if (J.two)
@@ -649,6 +716,23 @@ static hipError_t run_job(const NttJob& J, hipStream_t stream) {
     IMP_CHECK(hipGetLastError());
   }
   return hipSuccess;
+}
+
+static hipError_t run_job(const NttJob& J_in, hipStream_t stream) {
+  if (!J_in.fbr) return launch_stages(J_in, stream);
+  NttJob J = J_in;
+  IMP_CHECK(keep_pool_memory());
+  if (hipMallocAsync((void**)&J.tmp, (size_t)32 << J.L, stream) != hipSuccess) {
+    // no room for the second buffer: the same passes in place, after the
+    // bit-reversal kernel (same result)
+    (void)hipGetLastError();
+    J.fbr = 0;
+    J.tmp = nullptr;
+    return launch_stages(J, stream);
+  }
+  const hipError_t e = launch_stages(J, stream);
+  const hipError_t f = hipFreeAsync(J.tmp, stream);
+  return e != hipSuccess ? e : f;
 }
 
 // factor tables in stream-ordered temporary memory
@@ -837,6 +921,22 @@ extern "C" size_t impeller_ntt_batch_bytes(unsigned int count) {
   return batch_factor_off(count) + (size_t)count * kJobArea;
 }
 
+namespace {
+// second buffer of an item whose plan fuses the bit reversal (0 otherwise)
+size_t item_tmp_bytes(const ImpNttItem& it) {
+  PassPlan plan;
+  if (it.log_n > 31 || !plan_from(it.plan_len ? it.plan : nullptr, it.plan_len, it.log_n, &plan))
+    return 0;
+  return plan.fuse == 1 ? align_up((size_t)32 << it.log_n, 256) : 0;
+}
+}  // namespace
+
+extern "C" size_t impeller_ntt_batch_bytes_items(const ImpNttItem* items, unsigned int count) {
+  size_t b = align_up(impeller_ntt_batch_bytes(count), 256);
+  for (u32 i = 0; items && i < count; i++) b += item_tmp_bytes(items[i]);
+  return b;
+}
+
 extern "C" hipError_t impeller_ntt_batch_prepare(const ImpNttItem* items, unsigned int count,
                                                  void* dev, size_t bytes, ImpNttBatch* out,
                                                  hipStream_t stream) {
@@ -846,6 +946,7 @@ extern "C" hipError_t impeller_ntt_batch_prepare(const ImpNttItem* items, unsign
   std::vector<NttJob> jobs(count);
   std::vector<u32> prefix((size_t)9 * (count + 1), 0);
   u32 nst_max = 0;
+  size_t tmp_off = align_up(impeller_ntt_batch_bytes(count), 256);
   for (u32 i = 0; i < count; i++) {
     const ImpNttItem& it = items[i];
     IMP_REQUIRE(it.data && it.log_n <= 31 && (it.log_n == 0 || it.twiddle), "batch item");
@@ -853,6 +954,19 @@ extern "C" hipError_t impeller_ntt_batch_prepare(const ImpNttItem* items, unsign
     PassPlan plan;
     IMP_REQUIRE(plan_from(it.plan_len ? it.plan : nullptr, it.plan_len, it.log_n, &plan),
                 "batch item plan");
+    // a fused-reversal plan takes its second buffer from the batch buffer
+    // past the fixed area; without room it runs in place after the
+    // bit-reversal stage (same passes, same result)
+    u32* tmp = nullptr;
+    if (plan.fuse == 1) {
+      const size_t need = align_up((size_t)32 << it.log_n, 256);
+      if (tmp_off + need <= bytes) {
+        tmp = (u32*)(base + tmp_off);
+        tmp_off += need;
+      } else {
+        plan.fuse = 0;
+      }
+    }
     View x = {it.data, 0, 1ull << it.log_n};
     View src = it.src ? View{const_cast<u32*>(it.src), 0, 1ull << it.log_src} : x;
     u64 src_len = it.src ? (1ull << it.log_src) : x.wl;
@@ -875,6 +989,7 @@ extern "C" hipError_t impeller_ntt_batch_prepare(const ImpNttItem* items, unsign
     }
     IMP_CHECK(hipGetLastError());
     jobs[i] = make_job(x, src, src_len, it.log_n, tw, false, pro, epi, plan);
+    jobs[i].tmp = tmp;
     if (jobs[i].nst > nst_max) nst_max = jobs[i].nst;
   }
   out->dev = dev;
@@ -887,9 +1002,11 @@ extern "C" hipError_t impeller_ntt_batch_prepare(const ImpNttItem* items, unsign
       prefix[(size_t)st * (count + 1) + i] = run;
       u32 blocks = 0;
       if (st == 0) {
-        blocks = jobs[i].br_blocks;
-        size_t l = br_lds(jobs[i]);
-        if (l > lds) lds = l;
+        if (!jobs[i].fbr) {
+          blocks = jobs[i].br_blocks;
+          size_t l = br_lds(jobs[i]);
+          if (l > lds) lds = l;
+        }
       } else if (st - 1 < jobs[i].nst) {
         blocks = jobs[i].st[st - 1].blocks;
         size_t l = stage_lds(jobs[i], st - 1);

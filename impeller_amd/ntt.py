@@ -113,8 +113,10 @@ def ntt(x_tensor, twiddle_tensor, log_len, stream: Stream, inverse=False, coset_
     """In-place NTT of a contiguous device tensor (impeller_ntt).
 
     passes: optional plan [fuse, lds_log, d0, d1, ...] -- levels fused into the
-    bit-reversal kernel (0, 4 or 5), log2 elements staged per work-group (10 or
-    11) and the level count of each following pass (fuse + sum(d) == log_len).
+    bit-reversal kernel (0, 4 or 5; or 1: no bit-reversal kernel, the first
+    pass reads the input bit-reversed into a second buffer, at least two
+    passes), log2 elements staged per work-group (10 or 11) and the level
+    count of each following pass (levels of fuse + sum(d) == log_len).
     """
     plan = None
     n_passes = 0
@@ -156,7 +158,8 @@ class NttBatch:
             for k, v in enumerate(plan):
                 a.plan[k] = v
             self._keep.append(it)
-        nbytes = int(_lib.call("impeller_ntt_batch_bytes", n))
+        # room for the second buffer of every fused-reversal item
+        nbytes = int(_lib.call("impeller_ntt_batch_bytes_items", arr, n))
         self.dev = device_u32((nbytes + 3) // 4)
         self.info = _lib.ImpNttBatch()
         _lib.call("impeller_ntt_batch_prepare", arr, n, self.dev.data_ptr(), nbytes,

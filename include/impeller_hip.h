@@ -148,9 +148,13 @@ hipError_t impeller_gate(const uint32_t *const *inputs, uint32_t *out, unsigned 
  * plan_passes (n_passes entries, or NULL for the library default):
  * {fuse, lds_log, d0, d1, ...}: fuse = DIT levels done inside the
  * bit-reversal kernel (0; 4 when log_len >= 8, 16 x 16 tiles; 5 when
- * log_len >= 10, 32 x 32 tiles), lds_log = log2 of the
- * elements one work-group stages in LDS (10 or 11), d_i = levels of each
- * following pass; fuse + sum(d_i) == log_len. */
+ * log_len >= 10, 32 x 32 tiles), or 1: no bit-reversal kernel, the first
+ * pass reads x bit-reversed and writes a second 2^log_len-element buffer
+ * taken from the device's default stream-ordered pool (at least two passes;
+ * the pool's release threshold is raised so the memory stays pooled);
+ * lds_log = log2 of the elements one work-group stages in LDS (10 or 11),
+ * d_i = levels of each following pass; fused levels + sum(d_i) == log_len.
+ * The batched entry points run a fuse-1 plan with the bit-reversal kernel. */
 hipError_t impeller_ntt(uint32_t *x, const uint32_t *twiddle, unsigned int log_len,
                         int inverse, const uint32_t *coset_shift,
                         const unsigned int *plan_passes, unsigned int n_passes,
@@ -162,10 +166,14 @@ hipError_t impeller_ntt(uint32_t *x, const uint32_t *twiddle, unsigned int log_l
  * (2^log_n elements, in place) receives the result; src, when non-NULL, is
  * the 2^log_src-element input, zero-extended to 2^log_n (coset extension);
  * twiddle/inverse/coset_shift/plan as for impeller_ntt.  prepare fills the
- * device buffer dev (impeller_ntt_batch_bytes(count) bytes) and the host
- * descriptor *batch, synchronously; run only launches kernels (it can be
+ * device buffer dev (at least impeller_ntt_batch_bytes(count) bytes) and the
+ * host descriptor *batch, synchronously; run only launches kernels (it can be
  * captured into a HIP graph); max_blocks (0 = no cap) bounds the grid of
- * each stage launch. */
+ * each stage launch.  Items whose plan (given or default) fuses the bit
+ * reversal into the first pass need a second 2^log_n-element buffer: a dev
+ * buffer of impeller_ntt_batch_bytes_items(items, count) bytes holds them
+ * all; items without room run the same passes in place after the
+ * bit-reversal stage (same result). */
 typedef struct ImpNttItem {
     uint32_t *data;
     const uint32_t *src;
@@ -185,6 +193,7 @@ typedef struct ImpNttBatch {
 } ImpNttBatch;
 
 size_t impeller_ntt_batch_bytes(unsigned int count);
+size_t impeller_ntt_batch_bytes_items(const ImpNttItem *items, unsigned int count);
 hipError_t impeller_ntt_batch_prepare(const ImpNttItem *items, unsigned int count, void *dev,
                                       size_t bytes, ImpNttBatch *batch, hipStream_t stream);
 hipError_t impeller_ntt_batch_run(const ImpNttBatch *batch, unsigned int max_blocks,

@@ -274,6 +274,12 @@ hipError_t hipGetDevice(int* d) {
   return hipSuccess;
 }
 hipError_t hipSetDevice(int) { return hipSuccess; }
+// the emulated allocator has no pool: the attribute is accepted and ignored
+hipError_t hipDeviceGetDefaultMemPool(hipMemPool_t* pool, int) {
+  *pool = nullptr;
+  return hipSuccess;
+}
+hipError_t hipMemPoolSetAttribute(hipMemPool_t, hipMemPoolAttr, void*) { return hipSuccess; }
 hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) {
   *v = 8;  // a small "chip": persistent grids stride more often
   return hipSuccess;

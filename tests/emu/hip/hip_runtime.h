@@ -112,6 +112,10 @@ hipError_t hipGetLastError();
 const char* hipGetErrorString(hipError_t e);
 hipError_t hipGetDevice(int* d);
 hipError_t hipSetDevice(int d);
+typedef struct EmuMemPool* hipMemPool_t;
+enum hipMemPoolAttr { hipMemPoolAttrReleaseThreshold = 4 };
+hipError_t hipDeviceGetDefaultMemPool(hipMemPool_t* pool, int dev);
+hipError_t hipMemPoolSetAttribute(hipMemPool_t pool, hipMemPoolAttr attr, void* value);
 hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t a, int dev);
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned flags);
 hipError_t hipStreamDestroy(hipStream_t s);

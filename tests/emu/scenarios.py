@@ -63,7 +63,7 @@ def ntt_items(specs, seed):
 
 
 class NttBatch:
-    def __init__(self, E, items):
+    def __init__(self, E, items, second_buffers=True):
         self.E = E
         n = len(items)
         arr = (_lib.ImpNttItem * n)()
@@ -80,7 +80,10 @@ class NttBatch:
             a.plan_len = len(plan)
             for k, v in enumerate(plan):
                 a.plan[k] = v
-        nbytes = int(E.call("impeller_ntt_batch_bytes", n))
+        # second_buffers=False: the minimal buffer, fused-reversal items then
+        # run in place after the bit-reversal stage
+        nbytes = int(E.call("impeller_ntt_batch_bytes_items", arr, n) if second_buffers
+                     else E.call("impeller_ntt_batch_bytes", n))
         self.dev = garbage_u32((nbytes + 3) // 4)
         self.info = _lib.ImpNttBatch()
         E.call("impeller_ntt_batch_prepare", arr, n, ptr(self.dev), nbytes,
@@ -122,13 +125,17 @@ def batched_ntt_mixed(E):
 
 
 def batched_ntt_plans(E):
-    items, expects = ntt_items([("fwd_random", 14), ("inv", 12), ("ext", 14), ("fwd", 9)], 5)
-    items[0]["plan"] = [5, 11, 9]
-    items[1]["plan"] = [0, 10, 2, 3, 7]
-    items[2]["plan"] = [5, 10, 4, 5]
-    items[3]["plan"] = [4, 10, 5]
-    NttBatch(E, items).run()
-    check_items(items, expects, "plans")
+    for second in (True, False):
+        items, expects = ntt_items([("fwd_random", 14), ("inv", 12), ("ext", 14), ("fwd", 9),
+                                    ("ext", 13), ("inv", 11)], 5)
+        items[0]["plan"] = [5, 11, 9]
+        items[1]["plan"] = [0, 10, 2, 3, 7]
+        items[2]["plan"] = [5, 10, 4, 5]
+        items[3]["plan"] = [1, 10, 4, 5]
+        items[4]["plan"] = [1, 10, 6, 7]  # coset extension with the reversal fused
+        items[5]["plan"] = [1, 11, 2, 9]
+        NttBatch(E, items, second_buffers=second).run(7 if second else 0)
+        check_items(items, expects, f"plans second_buffers={second}")
 
 
 def ntt_graph_replay(E):
@@ -163,9 +170,37 @@ def single_ntt(E):
             assert (d.reshape(-1, 8) == e).all(), (k, inverse, shift is not None)
 
 
+def ntt_fused_reversal(E):
+    """Forward, coset, inverse and inverse-coset transforms with plans whose
+    first pass reads the source bit-reversed, against best_fft."""
+    for k, plan in ((2, [1, 10, 1, 1]), (5, [1, 10, 3, 2]), (9, [1, 11, 5, 4]),
+                    (13, [1, 10, 7, 6]), (14, [1, 10, 4, 5, 5])):
+        n = 1 << k
+        x = O.gen_scalars(n, 400 + k)
+        w = mont(omega(k))
+        tw = flat(O.powers(w, max(1, n // 2)))
+        arr = (ctypes.c_uint * len(plan))(*plan)
+        for inverse, shift in ((0, None), (0, flat(mont(7))), (1, None), (1, flat(mont(5)))):
+            d = flat(x).copy()
+            E.call("impeller_ntt", ptr(d), ptr(tw), k, inverse, ptr(shift),
+                   ctypes.cast(arr, ctypes.c_void_p), len(plan), None)
+            if inverse:
+                e = O.best_fft(x, w, k)
+                e = O.vec_scale(e, mont(pow(n, -1, r)))
+                if shift is not None:
+                    e = O.vec_mul(e, O.powers(mont(5), n))
+            else:
+                xin = x if shift is None else O.vec_mul(x, O.powers(mont(7), n))
+                e = O.best_fft(xin, w, k)
+            assert (d.reshape(-1, 8) == e).all(), (k, plan, inverse, shift is not None)
+
+
 PLANS = [[0, 10, 4, 4, 4, 4], [0, 10, 8, 8], [5, 10, 3, 8], [0, 11, 10, 6], [0, 10] + [2] * 8,
          [5, 11, 11], [5, 10, 1, 1, 9], [5, 11, 5, 6], [5, 10] + [1] * 3 + [2] * 4,
-         [4, 10, 6, 6], [4, 11, 11, 1], [4, 10, 2, 5, 5]]
+         [4, 10, 6, 6], [4, 11, 11, 1], [4, 10, 2, 5, 5],
+         # bit reversal fused into the first pass (column runs of 2^(lds_log - d0))
+         [1, 10, 8, 8], [1, 10, 7, 9], [1, 11, 6, 10], [1, 10, 3, 3, 10], [1, 10, 10, 6],
+         [1, 11, 11, 5], [1, 10, 1, 5, 10], [1, 10, 2, 2, 2, 2, 2, 2, 2, 2]]
 
 
 def ntt_plans(E):
@@ -416,7 +451,8 @@ def gate(E):
         assert (out.reshape(-1, 8) == O.gate(*ins)).all(), n
 
 
-ALL = [("single_ntt", single_ntt), ("ntt_plans", ntt_plans), ("batched_ntt_mixed", batched_ntt_mixed),
+ALL = [("single_ntt", single_ntt), ("ntt_plans", ntt_plans),
+       ("ntt_fused_reversal", ntt_fused_reversal), ("batched_ntt_mixed", batched_ntt_mixed),
        ("batched_ntt_plans", batched_ntt_plans), ("ntt_graph_replay", ntt_graph_replay),
        ("gate", gate), ("field_inverse", field_inverse), ("arith29", arith29), ("mlp_grad", mlp_grad), ("single_msm", single_msm), ("msm_hot_bucket", msm_hot_bucket), ("msm_graph_replay", msm_graph_replay),
        ("batched_msm_shared_tables", batched_msm_shared_tables)]

@@ -83,6 +83,34 @@ def test_batched_ntt_plans(stream):
         assert (to_host(it["data"]) == e).all()
 
 
+def test_batched_ntt_fused_reversal():
+    # fused-reversal plans in a batch (second buffers in the batch buffer),
+    # including the library default at 2^20 and 2^21, eager and replayed
+    import torch
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        stream = Stream.current(0)
+        specs = [("fwd", 20), ("ext", 21), ("inv", 14), ("fwd_random", 13), ("fwd", 9)]
+        items, expects = build_items(specs, 21)
+        items[2]["plan"] = [1, 10, 6, 8]
+        items[3]["plan"] = [1, 11, 3, 10]
+        items[4]["plan"] = [4, 10, 5]
+        restore = [(it["data"], it["data"].clone()) for it in items if it.get("src") is None]
+        b = N.NttBatch(items, stream)
+        g = N.Graph(stream, lambda: b.run(stream))
+        for rep in range(3):
+            for dst, src in restore:
+                dst.copy_(src)
+            torch.cuda.synchronize()
+            if rep == 0:
+                b.run(stream)
+            else:
+                g.launch(stream)
+            stream.sync()
+            for it, e in zip(items, expects):
+                assert (to_host(it["data"]) == e).all(), (rep, it["log_n"])
+
+
 def test_graph_replay_matches_eager():
     import torch
     s = torch.cuda.Stream()

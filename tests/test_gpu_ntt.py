@@ -171,7 +171,10 @@ def test_impeller_ntt_inverse_and_coset(stream, k):
 @pytest.mark.parametrize("passes", [[0, 10, 4, 4, 4, 4], [0, 10, 8, 8], [5, 10, 3, 8],
                                     [0, 11, 10, 6], [0, 10] + [2] * 8, [5, 11, 11],
                                     [5, 10, 1, 1, 9], [5, 11, 5, 6], [5, 10] + [1] * 3 + [2] * 4,
-                                    [4, 10, 6, 6], [4, 11, 11, 1], [4, 10, 2, 5, 5]])
+                                    [4, 10, 6, 6], [4, 11, 11, 1], [4, 10, 2, 5, 5],
+                                    # bit reversal fused into the first pass
+                                    [1, 10, 8, 8], [1, 10, 7, 9], [1, 11, 6, 10],
+                                    [1, 10, 3, 3, 10], [1, 10, 10, 6], [1, 10, 1, 5, 10]])
 def test_pass_splits_agree(stream, passes):
     k = 16
     x = O.gen_scalars(1 << k, 7000)
@@ -191,6 +194,50 @@ def test_invalid_plans_rejected(stream, passes):
     d = to_device(O.gen_scalars(1 << k, 7001))
     with pytest.raises(_lib.HipError):
         N.ntt(d, to_device(host_twiddles(random_omega(71), k)), k, stream, passes=passes)
+
+
+def test_fused_reversal_needs_two_passes(stream):
+    k = 8
+    d = to_device(O.gen_scalars(1 << k, 7002))
+    with pytest.raises(_lib.HipError):
+        N.ntt(d, to_device(host_twiddles(random_omega(72), k)), k, stream, passes=[1, 10, 8])
+
+
+@pytest.mark.parametrize("k,passes", [(17, [1, 10, 7, 10]), (12, [1, 11, 5, 7])])
+def test_fused_reversal_inverse_and_coset(stream, k, passes):
+    n = 1 << k
+    x = O.gen_scalars(n, 6100 + k)
+    w = mont(omega(k))
+    winv = mont(pow(omega(k), -1, r))
+    g = mont(7)
+    ginv = mont(pow(7, -1, r))
+    tw = to_device(host_twiddles(w, k))
+    twi = to_device(host_twiddles(winv, k))
+    d = to_device(x)
+    N.ntt(d, tw, k, stream, coset_shift=to_device(g), passes=passes)
+    stream.sync()
+    assert (to_host(d) == O.best_fft(O.distribute_powers(x, O.powers(g, n)), w, k)).all()
+    N.ntt(d, twi, k, stream, inverse=True, coset_shift=to_device(ginv), passes=passes)
+    stream.sync()
+    assert (to_host(d) == x).all()
+
+
+def test_fused_reversal_2_24_matches_in_place(stream):
+    # the fused-reversal plan of the benchmark (also the library default at
+    # this size) against an in-place plan with the bit-reversal kernel, bit
+    # for bit, forward and coset at 2^24
+    k = 24
+    n = 1 << k
+    x = device_u32(n * 8)
+    _lib.call("impeller_gen_scalars", x.data_ptr(), n, 0, 9100, 0, stream.raw())
+    tw = N.twiddles(mont(omega(k)), n // 2, stream)
+    g = to_device(mont(7))
+    for shift in (None, g):
+        a, b = x.clone(), x.clone()
+        N.ntt(a, tw, k, stream, coset_shift=shift, passes=[4, 10, 7, 7, 6])
+        N.ntt(b, tw, k, stream, coset_shift=shift, passes=[1, 10, 8, 8, 8])
+        stream.sync()
+        assert bool((a == b).all()), shift is not None
 
 
 def test_full_size_2_20_vs_oracle(stream):
