@@ -9,10 +9,10 @@ base table, 50% uniform / 30% selector / 20% small scalars), run as
 sequential single calls with the untuned default configurations (plain) and
 as batched launches replayed from a HIP graph with the cost model's picks
 (and, for reference, the exhaustive tuner's picks); outputs compared bit for
-bit.  With --chunk, also the fused proving chunk at k = 22 under the same
-three schedules.
+bit.  Then the fused proving chunk at k = 22 under the same three schedules
+(--no-chunk skips it).
 
-  python bench.py [--steps K] [--warmup W] [--window C] [--no-cpu] [--no-batch] [--chunk]
+  python bench.py [--steps K] [--warmup W] [--window C] [--no-cpu] [--no-batch] [--no-chunk]
 """
 import argparse
 import json
@@ -38,7 +38,7 @@ def parse():
     ap.add_argument("--window", type=int, default=0, help="MSM window c (0: sweep 12..20)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched comparisons")
-    ap.add_argument("--chunk", action="store_true", help="also run the k=22 proving chunk")
+    ap.add_argument("--no-chunk", action="store_true", help="skip the k=22 proving chunk")
     return ap.parse_args()
 
 
@@ -316,7 +316,7 @@ def main():
         torch.cuda.empty_cache()
 
     chunk = None
-    if args.chunk:
+    if not args.no_chunk:
         # k = 22 proving chunk: plain single calls with the default
         # configuration vs the batched graph with the cost model's picks
         pc = WL.ProvingChunk(stream)

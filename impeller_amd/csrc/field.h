@@ -261,8 +261,9 @@ __device__ __forceinline__ void mac96_quad(u64& acc, u32& hi, u32 x0, u32 y0, u3
 
 // Product-scanning Montgomery multiplication (FIPS, Koc-Acar-Kaliski 1996):
 // column k accumulates a_j*b_(k-j) and m_j*q_(k-j) into a 96-bit accumulator;
-// m_k = acc * q' mod 2^32 for k < 8.  For a, b < 2m and 4m < 2^256 the result
-// is < 2m and fits 8 limbs.  The (a_j b, m_j q) pairs of a column go through
+// m_k = acc * q' mod 2^32 for k < 8.  The result is (a b + q m) / 2^256 <
+// a b / 2^256 + m: below 2m for a, b < 2m and 4m < 2^256, and also for a < 4m,
+// b < m (the NTT's lazy butterflies); it fits 8 limbs.  The (a_j b, m_j q) pairs of a column go through
 // the quad and pair blocks above; the column's last products (a_k b_0, then
 // m_k q_0 once m_k is known) through the single-product forms.
 template <class P>

@@ -247,7 +247,7 @@ __device__ __forceinline__ void hist_body(const MsmJob& J, u32 pass, u64 lb, u32
   const u32 shift = kRadixBits * pass;
   const u32* keys = (pass & 1) ? J.kB : J.kA;
   __syncthreads();
-  cnt[t] = 0;  // blockDim.x == kRadix
+  for (u32 b = t; b < kRadix; b += 256) cnt[b] = 0;
   __syncthreads();
   const u32* kk = keys + (u64)pw * s.seg_len;
   const u64 tile0 = (u64)tile * kSortTile;
@@ -264,7 +264,7 @@ __device__ __forceinline__ void hist_body(const MsmJob& J, u32 pass, u64 lb, u32
     if (idx < s.seg_len) atomicAdd(&cnt[(k[j] >> shift) & (kRadix - 1)], 1u);
   }
   __syncthreads();
-  gst(J.hist + ((u64)pw * kRadix + t) * s.tiles + tile, cnt[t]);
+  for (u32 b = t; b < kRadix; b += 256) gst(J.hist + ((u64)pw * kRadix + b) * s.tiles + tile, cnt[b]);
 }
 
 // block-wide exclusive scan of len entries (in place allowed); with
@@ -311,10 +311,14 @@ __device__ __forceinline__ u32 block_excl_scan(u32 v, u32* part, u32* total) {
 }
 
 struct ScatterLds {
-  u32 cnt[16 * 256];  // per (nibble, thread) counts, then running ranks (swizzled)
+  union {
+    u32 cnt[16 * 256];  // splits: per (field value, thread) counts, then running ranks (swizzled)
+    struct {
+      u32 gbase[kRadix];  // scatter: global start of each digit's run of this tile
+      u32 dbase[kRadix];  // first staged position of each digit
+    } run;
+  };
   u32 part[256];
-  u32 gbase[kRadix];  // global start of each digit's run of this tile
-  u32 dbase[kRadix];  // first staged position of each digit
   u32 sk[kSortTile];  // the tile in (digit, input order), swizzled: staged for
   u32 sv[kSortTile];  // coalesced writes
 };
@@ -325,17 +329,19 @@ struct ScatterLds {
 // 64 distinct banks
 __device__ __forceinline__ u32 swz(u32 i) { return i ^ ((i >> 6) & 15u); }
 
-// Stable split of the tile by the 4-bit field at bit sh.  Thread t holds tile
-// positions 16t .. 16t + n - 1 in key/val; ranks come from per-thread field
-// counts scanned in (field, thread) order, and the entries land in sk/sv.
-__device__ __forceinline__ void split_nibble(ScatterLds& L, const u32 key[16], const u32 val[16],
-                                             u32 n, u32 sh) {
+// Stable split of the tile by the field of `bits` (<= 4) bits at bit sh.
+// Thread t holds tile positions 16t .. 16t + n - 1 in key/val; ranks come
+// from per-thread field counts scanned in (field, thread) order, and the
+// entries land in sk/sv.
+__device__ __forceinline__ void split_field(ScatterLds& L, const u32 key[16], const u32 val[16],
+                                            u32 n, u32 sh, u32 bits) {
+  const u32 mask = (1u << bits) - 1;
   const u32 t = threadIdx.x;
 #pragma unroll
   for (u32 d = 0; d < 16; d++) L.cnt[swz(d * 256 + t)] = 0;
 #pragma unroll
   for (u32 j = 0; j < 16; j++)
-    if (j < n) L.cnt[swz(((key[j] >> sh) & 15u) * 256 + t)] += 1;
+    if (j < n) L.cnt[swz(((key[j] >> sh) & mask) * 256 + t)] += 1;
   __syncthreads();
   // thread t scans the flattened counts [16t, 16t + 16)
   u32 loc[16];
@@ -356,7 +362,7 @@ __device__ __forceinline__ void split_nibble(ScatterLds& L, const u32 key[16], c
 #pragma unroll
   for (u32 j = 0; j < 16; j++) {
     if (j < n) {
-      const u32 c = swz(((key[j] >> sh) & 15u) * 256 + t);
+      const u32 c = swz(((key[j] >> sh) & mask) * 256 + t);
       const u32 r = L.cnt[c];
       L.cnt[c] = r + 1;
       L.sk[swz(r)] = key[j];
@@ -366,10 +372,12 @@ __device__ __forceinline__ void split_nibble(ScatterLds& L, const u32 key[16], c
   __syncthreads();
 }
 
-// One tile of 4096 entries: two stable nibble splits order it by the 8-bit
-// digit (input order kept within a digit), staged in LDS; then consecutive
-// threads write consecutive positions of each digit's run: staged entry i of
-// digit d goes to gbase[d] + (i - dbase[d]).
+// One tile of 4096 entries: stable 4-bit splits order it by the digit
+// (input order kept within a digit; one split for a top digit of at most 4
+// bits), staged in LDS; then consecutive threads write consecutive positions
+// of each digit's run: staged entry i of digit d goes to
+// gbase[d] + (i - dbase[d]).  (10-bit digits, two passes instead of three at
+// c = 20, measured no faster: runs of 4 entries per digit and tile.)
 __device__ __forceinline__ void scatter_body(const MsmJob& J, u32 pass, u64 lb, ScatterLds& L) {
   const MsmShape& s = J.s;
   const u32 tile = (u32)(lb % s.tiles), pw = (u32)(lb / s.tiles), t = threadIdx.x;
@@ -391,31 +399,35 @@ __device__ __forceinline__ void scatter_body(const MsmJob& J, u32 pass, u64 lb, 
       val[j] = gld(vsrc + seg0 + tile0 + 16 * t + j);
     }
   }
-  split_nibble(L, key, val, n, shift);
-  // keys are below 2^c: a last pass over at most 4 bits needs one split
-  if (shift + 4 < s.c) {
+  // keys are below 2^c: the top digit may be narrower
+  const u32 wbits = s.c - shift < kRadixBits ? s.c - shift : kRadixBits;
+  for (u32 b = 0; b < wbits; b += 4) {
+    if (b) {
 #pragma unroll
-    for (u32 j = 0; j < 16; j++) {
-      if (j < n) {
-        key[j] = L.sk[swz(16 * t + j)];
-        val[j] = L.sv[swz(16 * t + j)];
+      for (u32 j = 0; j < 16; j++) {
+        if (j < n) {
+          key[j] = L.sk[swz(16 * t + j)];
+          val[j] = L.sv[swz(16 * t + j)];
+        }
       }
+      __syncthreads();
     }
-    __syncthreads();
-    split_nibble(L, key, val, n, shift + 4);
+    split_field(L, key, val, n, shift + b, wbits - b < 4 ? wbits - b : 4);
   }
-  // digit run starts in the staged order, and the runs' global starts
-  L.gbase[t] = gld(J.hist + ((u64)pw * kRadix + t) * s.tiles + tile);
+  // digit run starts in the staged order, and the runs' global starts (the
+  // split counters are dead: gbase / dbase reuse their LDS)
+  for (u32 b = t; b < kRadix; b += 256)
+    L.run.gbase[b] = gld(J.hist + ((u64)pw * kRadix + b) * s.tiles + tile);
   for (u32 j = 0; j < n; j++) {
     const u32 i = 16 * t + j;
     const u32 d = (L.sk[swz(i)] >> shift) & (kRadix - 1);
-    if (i == 0 || ((L.sk[swz(i - 1)] >> shift) & (kRadix - 1)) != d) L.dbase[d] = i;
+    if (i == 0 || ((L.sk[swz(i - 1)] >> shift) & (kRadix - 1)) != d) L.run.dbase[d] = i;
   }
   __syncthreads();
   for (u32 i = t; i < n_tile; i += blockDim.x) {
     const u32 k = L.sk[swz(i)];
     const u32 d = (k >> shift) & (kRadix - 1);
-    const u64 pos = (u64)L.gbase[d] + i - L.dbase[d];
+    const u64 pos = (u64)L.run.gbase[d] + i - L.run.dbase[d];
     gst(kdst + seg0 + pos, k);
     gst(vdst + seg0 + pos, L.sv[swz(i)]);
   }
@@ -562,11 +574,6 @@ __device__ __forceinline__ bool task_range(const MsmJob& J, u32 pw, u32 t, u32& 
   const u32 be = J.bend[(u64)pw * (s.nb + 1) + b];
   hi = lo + s.T < be ? lo + s.T : be;
   return true;
-}
-
-__device__ __forceinline__ G1Aff29 entry_point(const MsmJob& J, u32 e) {
-  G1Aff29 p = load_aff_table(J.bases + 16 * (u64)(e & 0x7fffffffu));
-  return (e & 0x80000000u) ? aff29_neg(p) : p;
 }
 
 // A sorted entry's table point as stored (x then y, 8 words each) and its
@@ -854,7 +861,7 @@ __global__ __launch_bounds__(256) void kb_digits(const MsmJob* jobs, u32 njobs, 
 __global__ __launch_bounds__(256) void kb_hist(const MsmJob* jobs, u32 njobs, const u32* prefix,
                                                u32 pass) {
   __shared__ u32 cnt[kRadix];
-  IMP_BATCH_LOOP(hist_body(J, pass, lb, cnt))
+  IMP_BATCH_LOOP_XCD(hist_body(J, pass, lb, cnt))
 }
 __global__ __launch_bounds__(256) void kb_scan_a(const MsmJob* jobs, u32 njobs,
                                                  const u32* prefix, u32 arg) {

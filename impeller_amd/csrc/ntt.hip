@@ -115,30 +115,45 @@ __device__ __forceinline__ Fr tw_load(const Twiddle& tw, bool two, u64 e) {
   return two ? twiddle_at<true>(tw, e) : twiddle_at<false>(tw, e);
 }
 
-// Butterflies with the twiddles already in registers (w1 unused when !m1,
-// w2 when !m2: those twiddles are 1).
-__device__ __forceinline__ void bfly2_w(Fr& a, Fr& b, bool m, const Fr& w) {
-  if (m) b = mul<FrParams>(b, w);
-  Fr s = add<FrParams>(a, b);
-  b = sub<FrParams>(a, b);
-  a = s;
+// Lazy butterflies of the LDS passes (Harvey, "Faster arithmetic for
+// number-theoretic transforms", 2014): values live in [0, 4m) between levels
+// (4m < 2^256 for Fr).  (a, b) -> (a' + t, a' + 2m - t) with a' = a mod 2m
+// folded into [0, 2m) and t = b*w by the Montgomery product without its final
+// subtraction (b < 4m, w < m give t < 2m), or b folded into [0, 2m) when w = 1.
+__device__ __forceinline__ Fr fold2m(const Fr& a) {  // [0, 4m) -> [0, 2m)
+  Fr d;
+  u32 br = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const u32 m2 = (FrParams::m[i] << 1) | (i ? FrParams::m[i - 1] >> 31 : 0u);
+    d.v[i] = sbb(a.v[i], m2, br, &br);
+  }
+  Fr r;
+#pragma unroll
+  for (int i = 0; i < 8; i++) r.v[i] = br ? a.v[i] : d.v[i];
+  return r;
 }
 
-__device__ __forceinline__ void bfly4_w(Fr& a0, Fr& a1, Fr& a2, Fr& a3, bool m1, bool m2,
-                                        const Fr& w1, const Fr& w2, const Fr& w3) {
-  if (m1) {
-    a1 = mul<FrParams>(a1, w1);
-    a3 = mul<FrParams>(a3, w1);
+__device__ __forceinline__ void bfly2_lazy(Fr& a, Fr& b, bool m, const Fr& w) {
+  const Fr x = fold2m(a);
+  const Fr t = m ? mul_lazy<FrParams>(b, w) : fold2m(b);
+  Fr u;  // 2m - t, t < 2m
+  u32 br = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const u32 m2 = (FrParams::m[i] << 1) | (i ? FrParams::m[i - 1] >> 31 : 0u);
+    u.v[i] = sbb(m2, t.v[i], br, &br);
   }
-  Fr b0 = add<FrParams>(a0, a1), b1 = sub<FrParams>(a0, a1);
-  Fr b2 = add<FrParams>(a2, a3), b3 = sub<FrParams>(a2, a3);
-  if (m2) b2 = mul<FrParams>(b2, w2);
-  b3 = mul<FrParams>(b3, w3);
-  a0 = add<FrParams>(b0, b2);
-  a2 = sub<FrParams>(b0, b2);
-  a1 = add<FrParams>(b1, b3);
-  a3 = sub<FrParams>(b1, b3);
+  u32 c0 = 0, c1 = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    a.v[i] = adc(x.v[i], t.v[i], c0, &c0);
+    b.v[i] = adc(x.v[i], u.v[i], c1, &c1);
+  }
 }
+
+// [0, 4m) -> [0, m)
+__device__ __forceinline__ Fr canon4m(const Fr& a) { return reduce_once<FrParams>(fold2m(a)); }
 
 __device__ __forceinline__ void bfly2_v(Fr& a, Fr& b, u64 e, const Twiddle& tw, bool two) {
   if (e != 0) b = mul<FrParams>(b, tw_load(tw, two, e));
@@ -420,7 +435,12 @@ __device__ __forceinline__ void dit_body(const NttJob& J, u32 stage, u64 wg, u32
       const Fr w1 = twiddle_at<TWO>(J.tw, e1), w2 = twiddle_at<TWO>(J.tw, e2);
       const Fr w3 = twiddle_at<TWO>(J.tw, e3);
       Fr a0 = pl_load(T, s0), a1 = pl_load(T, s1), a2 = pl_load(T, s2), a3 = pl_load(T, s3);
-      bfly4_w(a0, a1, a2, a3, e1 != 0, e2 != 0, w1, w2, w3);
+      // two levels: (a0, a1), (a2, a3) with w1, then (a0, a2) with w2 and
+      // (a1, a3) with w3 -- best_fft's butterflies, in its order per element
+      bfly2_lazy(a0, a1, e1 != 0, w1);
+      bfly2_lazy(a2, a3, e1 != 0, w1);
+      bfly2_lazy(a0, a2, e2 != 0, w2);
+      bfly2_lazy(a1, a3, true, w3);
       pl_store(T, s0, a0);
       pl_store(T, s1, a1);
       pl_store(T, s2, a2);
@@ -440,7 +460,7 @@ __device__ __forceinline__ void dit_body(const NttJob& J, u32 stage, u64 wg, u32
       const u32 s0 = dslot(g0 * C + c), s1 = dslot(g1 * C + c);
       const Fr w = twiddle_at<TWO>(J.tw, e);
       Fr a0 = pl_load(T, s0), a1 = pl_load(T, s1);
-      bfly2_w(a0, a1, e != 0, w);
+      bfly2_lazy(a0, a1, e != 0, w);
       pl_store(T, s0, a0);
       pl_store(T, s1, a1);
     }
@@ -467,7 +487,7 @@ __device__ __forceinline__ void dit_body(const NttJob& J, u32 stage, u64 wg, u32
       p = (hi << (s + D)) | ((u64)g << s) | (lo0 + c);
     }
     Fr v = pl_load(T, dslot(g * C + c));
-    if (epi) v = mul<FrParams>(v, factor_at(J.epi, p));
+    v = epi ? mul<FrParams>(v, factor_at(J.epi, p)) : canon4m(v);
     store_fe<Fr>(rd_rev ? J.tmp + 8 * p : J.x.base + 8 * view_phys(J.x, p), v);
   }
 }
