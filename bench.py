@@ -148,6 +148,35 @@ def main():
         torch.cuda.empty_cache()
     for c in (windows if args.window else (15, 16, 17)):
         sweep_plain[c] = time_msm(bases, c, 0, 64)
+
+    # end to end through the reference's host-buffer entry point `msm`
+    # (msm.cu:9-41 contract; c = 16, one bucket set per window): the points
+    # (256 MiB) and scalars (128 MiB) cross PCIe from pinned host memory and
+    # the affine result comes back, each call
+    def msm_end_to_end(reps=3):
+        import ctypes
+        pts_h = torch.empty(nm * 16, dtype=torch.int32, pin_memory=True)
+        scal_h = torch.empty(nm * 8, dtype=torch.int32, pin_memory=True)
+        res_h = torch.zeros(16, dtype=torch.int32, pin_memory=True)
+        pts_h.copy_(bases)
+        scal_h.copy_(scal)
+        sizes = (ctypes.c_ulong * 1)()
+        cards = (ctypes.c_uint * 1)(0)
+        _lib.call("msm", None, sizes, nm, 1, 1, 1, 1, 1, cards, None, 0, None, None)
+        buf = device_u32((sizes[0] + 3) // 4)
+        arr = lambda *p: (ctypes.c_void_p * len(p))(*p)
+        times = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            _lib.call("msm", arr(buf.data_ptr()), None, nm, 1, 1, 1, 1, 1, cards,
+                      arr(pts_h.data_ptr()), 1, arr(scal_h.data_ptr()), arr(res_h.data_ptr()))
+            times.append((time.perf_counter() - t) * 1e3)
+        del buf
+        torch.cuda.empty_cache()
+        return sorted(times)[reps // 2], res_h.numpy().view(np.uint32).copy()
+
+    e2e_ms, e2e_res = msm_end_to_end()
     best_c = min(sweep, key=sweep.get)
     tables = device_u32(M.n_tables(best_c, 1) * nm * 16)
     tables[: nm * 16].copy_(bases)
@@ -371,7 +400,9 @@ def main():
     if not args.no_cpu:
         from oracle import oracle as O
         kk = O.gen_scalars(nm, 77)
-        parity["msm_closed_form"] = bool((msm_res_host == O.msm_closed_form(to_host(scal), kk)).all())
+        expect = O.msm_closed_form(to_host(scal), kk)
+        parity["msm_closed_form"] = bool((msm_res_host == expect).all())
+        parity["msm_end_to_end_closed_form"] = bool((e2e_res == expect.reshape(-1)).all())
         x.copy_(x_orig)
         N.ntt(x, tw, k, stream, passes=ntt_plan)
         stream.sync()
@@ -431,6 +462,9 @@ def main():
         "msm_window_sweep_ms": {str(c): round(v, 3) for c, v in sweep.items()},
         "msm_table_setup_ms": {str(c): round(v, 2) for c, v in setup_ms.items()},
         "msm_no_tables_ms": {str(c): round(v, 3) for c, v in sweep_plain.items()},
+        "msm_2^22_end_to_end_ms": {"value": round(e2e_ms, 3), "what": "reference host-buffer "
+                                   "entry point msm (c = 16, per-window buckets): pinned host "
+                                   "points and scalars in, affine result out, PCIe included"},
         "batched_vs_plain_speedup": batch["speedup"] if batch else None,
         "batched": batch,
         "proving_chunk": chunk,

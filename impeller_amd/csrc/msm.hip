@@ -191,6 +191,18 @@ __device__ __forceinline__ const u32* sorted_vals(const MsmJob& J) {
 }
 // combine levels bucket b needs: smallest l with 2^(g l) >= nt (0 for nt <= 1);
 // its total ends in partA for even l, partB for odd l
+// The accumulation chunks of a window: T consecutive sorted entries each,
+// counted from z, the number of zero digits (bucket 0, sorted first and never
+// added).  The tasks of a bucket b >= 1 whose entries are [bs, be) are the
+// chunks that meet it (0 for an empty bucket); over a window they number at
+// most ceil(seg_len / T) + nb - 1 < task_cap.
+__host__ __device__ __forceinline__ u32 bucket_tasks(u32 bs, u32 be, u32 z, u32 T) {
+  return be > bs ? (be - 1 - z) / T - (bs - z) / T + 1 : 0u;
+}
+__host__ __device__ inline u64 chunks_per_window(const MsmShape& s) {
+  return (s.seg_len + s.T - 1) / s.T;
+}
+
 __device__ __forceinline__ u32 bucket_levels(u32 nt, u32 g) {
   u32 l = 0;
   while ((1ull << (g * l)) < nt) l++;
@@ -439,9 +451,9 @@ __device__ __forceinline__ void zero_body(const MsmJob& J, u64 i) {
   else if (i < 2 * nbw) J.bend[i - nbw] = 0;
 }
 
-// bucket run bounds in the sorted keys (bucket 0 = zero digit, skipped):
-// work-group lb covers 4096 consecutive keys of one physical window, 16 per
-// thread
+// bucket run bounds in the sorted keys (bucket 0, the zero digits, first:
+// its end is their count): work-group lb covers 4096 consecutive keys of one
+// physical window, 16 per thread
 static const u32 kBoundsSpan = 4096;
 
 __host__ __device__ inline u64 bounds_blocks_per_window(const MsmShape& s) {
@@ -466,7 +478,7 @@ __device__ __forceinline__ void bounds_body(const MsmJob& J, u64 lb) {
 #pragma unroll
   for (u32 j = 0; j < 16; j++) {
     const u32 cur = k[j];
-    if (j >= n || cur == 0) continue;
+    if (j >= n) continue;
     const u32 before = j ? k[j - 1] : prev;
     const u32 after = j + 1 < n ? k[j + 1] : next;
     if (cur != before) bs[cur] = (u32)(i0 + j);
@@ -477,7 +489,7 @@ __device__ __forceinline__ void bounds_body(const MsmJob& J, u64 lb) {
 // ----------------------------------------------------------- segmented scans
 // One exclusive scan per physical window over one of three inputs:
 //   SC_HIST   the radix histograms hist[(pw * 256 + digit) * tiles + tile], in place;
-//   SC_TASKS  tasks per bucket ceil(count_b / T) (0 for bucket 0) -> toff, total at [nb+1];
+//   SC_TASKS  tasks per bucket, the chunks meeting it (0 for bucket 0) -> toff, total at [nb+1];
 //   SC_LEVEL  groups of combine level x per bucket (0 for a bucket already reduced
 //             to one entry) -> loff, total at [nb+1].
 // Phase A sums chunks of 4096 inputs, phase B scans the chunk sums of a window
@@ -497,7 +509,7 @@ __device__ __forceinline__ u32 scan_in(const MsmJob& J, u32 kind, u32 x, u32 pw,
   if (kind == SC_HIST) return J.hist[(u64)pw * kRadix * s.tiles + i];
   if (i == 0) return 0;
   const u64 bo = (u64)pw * (s.nb + 1) + i;
-  const u32 nt = (J.bend[bo] - J.bstart[bo] + s.T - 1) / s.T;
+  const u32 nt = bucket_tasks(J.bstart[bo], J.bend[bo], J.bend[(u64)pw * (s.nb + 1)], s.T);
   if (kind == SC_TASKS) return nt;
   const u32 shp = s.g * (x - 1), sh = s.g * x;
   const u32 nprev = (u32)(((u64)nt + (1ull << shp) - 1) >> shp);
@@ -563,19 +575,6 @@ __device__ __forceinline__ u32 find_bucket(const u32* toff, u32 nb, u32 t) {
   return lo;
 }
 
-// task t of window pw: entries [lo, hi) of the sorted values
-__device__ __forceinline__ bool task_range(const MsmJob& J, u32 pw, u32 t, u32& lo, u32& hi) {
-  const MsmShape& s = J.s;
-  const u32* to = J.toff + (u64)pw * (s.nb + 2);
-  if (t >= to[s.nb + 1]) return false;
-  const u32 b = find_bucket(to, s.nb, t);
-  const u32 k = t - to[b];
-  lo = J.bstart[(u64)pw * (s.nb + 1) + b] + k * s.T;
-  const u32 be = J.bend[(u64)pw * (s.nb + 1) + b];
-  hi = lo + s.T < be ? lo + s.T : be;
-  return true;
-}
-
 // A sorted entry's table point as stored (x then y, 8 words each) and its
 // sign: loaded one entry ahead, unpacked to limbs only when it is added.
 struct RawEntry {
@@ -601,28 +600,53 @@ __device__ __forceinline__ G1Aff29 unpack_entry(const RawEntry& r) {
   return r.neg ? aff29_neg(p) : p;
 }
 
-// one task = up to T consecutive entries of one bucket, mixed XYZZ + affine
-// additions.  The next entry's point and the index after it are loaded before
-// the current addition, so their latency hides behind it.
+// One thread per chunk of T consecutive sorted entries of a physical window
+// past its z zero digits (the last chunk may be shorter), so every lane of a
+// wave adds about the same number of points whatever the bucket sizes and
+// the share of zero scalars.  The chunk's run inside bucket b is task
+// k = chunk - (bstart[b] - z) / T of that bucket; its partial sum, mixed
+// XYZZ + affine additions, goes to the task's slot.  The next entry's point and key and the index
+// after it are loaded before each addition, so their latency hides behind it.
 __device__ __forceinline__ void tasks_body(const MsmJob& J, u64 gi) {
   const MsmShape& s = J.s;
-  const u32 pw = (u32)(gi / s.task_cap);
+  const u64 cpw = chunks_per_window(s);
+  const u32 pw = (u32)(gi / cpw);
   if (pw >= s.n_windows) return;
-  const u32 t = (u32)(gi - (u64)pw * s.task_cap);
-  u32 lo, hi;
-  if (!task_range(J, pw, t, lo, hi)) return;
+  const u64 c = gi - (u64)pw * cpw;
+  const u32 z = gld(J.bend + (u64)pw * (s.nb + 1));
+  if (z + c * s.T >= s.seg_len) return;
+  const u32 lo = (u32)(z + c * s.T);
+  const u32 hi = (u32)(lo + s.T < s.seg_len ? lo + s.T : s.seg_len);
+  const u32* kk = sorted_keys(J) + (u64)pw * s.seg_len;
   const u32* vv = sorted_vals(J) + (u64)pw * s.seg_len;
+  const u32* to = J.toff + (u64)pw * (s.nb + 2);
+  const u32* bs = J.bstart + (u64)pw * (s.nb + 1);
+  const u64 pbase = (u64)pw * s.task_cap;
   G1Xyzz29 acc = G1Xyzz29::identity();
+  u32 b = gld(kk + lo);
   RawEntry cur = load_entry(J, gld(vv + lo));
-  u32 next_e = lo + 1 < hi ? gld(vv + lo + 1) : 0u;
+  u32 key1 = lo + 1 < hi ? gld(kk + lo + 1) : 0u;
+  u32 val1 = lo + 1 < hi ? gld(vv + lo + 1) : 0u;
   for (u32 j = lo; j < hi; j++) {
     RawEntry nxt = cur;
-    if (j + 1 < hi) nxt = load_entry(J, next_e);
-    if (j + 2 < hi) next_e = gld(vv + j + 2);
-    acc = xyzz29_madd(acc, unpack_entry(cur));
+    u32 bn = b;
+    if (j + 1 < hi) {
+      nxt = load_entry(J, val1);
+      bn = key1;
+    }
+    if (j + 2 < hi) {
+      key1 = gld(kk + j + 2);
+      val1 = gld(vv + j + 2);
+    }
+    if (b != 0) acc = xyzz29_madd(acc, unpack_entry(cur));
+    if (j + 1 == hi || bn != b) {
+      if (b != 0)
+        store_xyzz29(J.partA + kPt * (pbase + to[b] + (u32)(c - (bs[b] - z) / s.T)), acc);
+      acc = G1Xyzz29::identity();
+    }
     cur = nxt;
+    b = bn;
   }
-  store_xyzz29(J.partA + kPt * ((u64)pw * s.task_cap + t), acc);
 }
 
 // level l of the per-bucket partial-sum tree: compact index o of window pw
@@ -962,7 +986,7 @@ static u32 stage_blocks(const MsmShape& s, u32 kind, u32 arg) {
     }
     case MS_ZERO: return (u32)cdiv(2ull * s.n_windows * (s.nb + 1), 256);
     case MS_BOUNDS: return (u32)(s.n_windows * bounds_blocks_per_window(s));
-    case MS_TASKS: return (u32)cdiv((u64)s.n_windows * s.task_cap, 256);
+    case MS_TASKS: return (u32)cdiv((u64)s.n_windows * chunks_per_window(s), 256);
     case MS_COMBINE:
       return arg <= s.levels ? (u32)cdiv((u64)s.n_windows * combine_cap(s, arg), 256) : 0;
     case MS_REDUCE: return (u32)cdiv((u64)s.n_windows * s.nseg, 256);
