@@ -239,8 +239,8 @@ __device__ __forceinline__ void digits_body(const MsmJob& J, u64 i) {
         carry = 0;
       }
     }
-    J.kA[idx] = key;
-    J.vA[idx] = (u32)((u64)grp * J.tstride + i) | sign;
+    gst(J.kA + idx, key);
+    gst(J.vA + idx, (u32)((u64)grp * J.tstride + i) | sign);
   }
 }
 
@@ -470,19 +470,19 @@ __device__ __forceinline__ void bounds_body(const MsmJob& J, u64 lb) {
   u32* bs = J.bstart + (u64)pw * (s.nb + 1);
   u32* be = J.bend + (u64)pw * (s.nb + 1);
   const u32 n = s.seg_len - i0 < 16 ? (u32)(s.seg_len - i0) : 16u;
-  const u32 prev = i0 ? kk[i0 - 1] : 0xffffffffu;
-  const u32 next = i0 + n < s.seg_len ? kk[i0 + n] : 0xffffffffu;
+  const u32 prev = i0 ? gld(kk + i0 - 1) : 0xffffffffu;
+  const u32 next = i0 + n < s.seg_len ? gld(kk + i0 + n) : 0xffffffffu;
   u32 k[16];
 #pragma unroll
-  for (u32 j = 0; j < 16; j++) k[j] = j < n ? kk[i0 + j] : 0u;
+  for (u32 j = 0; j < 16; j++) k[j] = j < n ? gld(kk + i0 + j) : 0u;
 #pragma unroll
   for (u32 j = 0; j < 16; j++) {
     const u32 cur = k[j];
     if (j >= n) continue;
     const u32 before = j ? k[j - 1] : prev;
     const u32 after = j + 1 < n ? k[j + 1] : next;
-    if (cur != before) bs[cur] = (u32)(i0 + j);
-    if (cur != after) be[cur] = (u32)(i0 + j + 1);
+    if (cur != before) gst(bs + cur, (u32)(i0 + j));
+    if (cur != after) gst(be + cur, (u32)(i0 + j + 1));
   }
 }
 
