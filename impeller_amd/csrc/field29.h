@@ -456,6 +456,42 @@ __device__ __forceinline__ void store_xyzz29(u32* p, const G1Xyzz29& a) {
     p[27 + i] = a.zzz.v[i];
   }
 }
+// the same layout in global memory (p 16-byte aligned): nine 16-byte
+// accesses through a global-space pointer
+__device__ __forceinline__ void store_xyzz29_g(u32* p, const G1Xyzz29& a) {
+  u32 w[36];
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    w[i] = a.x.v[i];
+    w[9 + i] = a.y.v[i];
+    w[18 + i] = a.zz.v[i];
+    w[27 + i] = a.zzz.v[i];
+  }
+  uint4* q = reinterpret_cast<uint4*>(p);
+#pragma unroll
+  for (int k = 0; k < 9; k++) gst(q + k, make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]));
+}
+__device__ __forceinline__ G1Xyzz29 load_xyzz29_g(const u32* p) {
+  const uint4* q = reinterpret_cast<const uint4*>(p);
+  u32 w[36];
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const uint4 v = gld(q + k);
+    w[4 * k] = v.x;
+    w[4 * k + 1] = v.y;
+    w[4 * k + 2] = v.z;
+    w[4 * k + 3] = v.w;
+  }
+  G1Xyzz29 a;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    a.x.v[i] = w[i];
+    a.y.v[i] = w[9 + i];
+    a.zz.v[i] = w[18 + i];
+    a.zzz.v[i] = w[27 + i];
+  }
+  return a;
+}
 __device__ __forceinline__ G1Xyzz29 load_xyzz29(const u32* p) {
   G1Xyzz29 a;
 #pragma unroll

@@ -250,6 +250,22 @@ __device__ __forceinline__ void convert_body(const MsmJob& J, u64 i) {
   aff_to_table(const_cast<u32*>(J.bases) + 16 * i, load_aff(J.raw + 16 * i));
 }
 
+// 16 consecutive words from p (16-byte aligned) in four 16-byte loads: a
+// thread's run of 16 keys or counts, read with a quarter of the instructions
+// that word loads would take (lanes 64 bytes apart).
+__device__ __forceinline__ bool aligned16(const u32* p) { return ((uintptr_t)p & 15) == 0; }
+__device__ __forceinline__ void gld16(const u32* p, u32 v[16]) {
+  const uint4* q = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint4 w = gld(q + k);
+    v[4 * k] = w.x;
+    v[4 * k + 1] = w.y;
+    v[4 * k + 2] = w.z;
+    v[4 * k + 3] = w.w;
+  }
+}
+
 // LSD radix sort, 8-bit digits, stable per pass.  Tile histograms go to
 // hist[(pw * 256 + digit) * tiles + tile]; their exclusive scan in that order
 // is each (digit, tile) run's global start.
@@ -404,11 +420,16 @@ __device__ __forceinline__ void scatter_body(const MsmJob& J, u32 pass, u64 lb, 
   const u32 n = n_tile > 16 * t ? (n_tile - 16 * t < 16 ? n_tile - 16 * t : 16) : 0;
   u32 key[16], val[16];
   __syncthreads();
+  if (n == 16 && aligned16(ksrc + seg0) && aligned16(vsrc + seg0)) {
+    gld16(ksrc + seg0 + tile0 + 16 * t, key);
+    gld16(vsrc + seg0 + tile0 + 16 * t, val);
+  } else {
 #pragma unroll
-  for (u32 j = 0; j < 16; j++) {
-    if (j < n) {
-      key[j] = gld(ksrc + seg0 + tile0 + 16 * t + j);
-      val[j] = gld(vsrc + seg0 + tile0 + 16 * t + j);
+    for (u32 j = 0; j < 16; j++) {
+      if (j < n) {
+        key[j] = gld(ksrc + seg0 + tile0 + 16 * t + j);
+        val[j] = gld(vsrc + seg0 + tile0 + 16 * t + j);
+      }
     }
   }
   // keys are below 2^c: the top digit may be narrower
@@ -453,7 +474,8 @@ __device__ __forceinline__ void zero_body(const MsmJob& J, u64 i) {
 
 // bucket run bounds in the sorted keys (bucket 0, the zero digits, first:
 // its end is their count): work-group lb covers 4096 consecutive keys of one
-// physical window, 16 per thread
+// physical window; lanes read consecutive keys (and their neighbours, which
+// the cache already holds), so every load instruction is one coalesced run
 static const u32 kBoundsSpan = 4096;
 
 __host__ __device__ inline u64 bounds_blocks_per_window(const MsmShape& s) {
@@ -464,25 +486,20 @@ __device__ __forceinline__ void bounds_body(const MsmJob& J, u64 lb) {
   const MsmShape& s = J.s;
   const u64 bpw = bounds_blocks_per_window(s);
   const u32 pw = (u32)(lb / bpw);
-  const u64 i0 = (lb - (u64)pw * bpw) * kBoundsSpan + 16 * (u64)threadIdx.x;
+  const u64 i0 = (lb - (u64)pw * bpw) * kBoundsSpan;
   if (pw >= s.n_windows || i0 >= s.seg_len) return;
   const u32* kk = sorted_keys(J) + (u64)pw * s.seg_len;
   u32* bs = J.bstart + (u64)pw * (s.nb + 1);
   u32* be = J.bend + (u64)pw * (s.nb + 1);
-  const u32 n = s.seg_len - i0 < 16 ? (u32)(s.seg_len - i0) : 16u;
-  const u32 prev = i0 ? gld(kk + i0 - 1) : 0xffffffffu;
-  const u32 next = i0 + n < s.seg_len ? gld(kk + i0 + n) : 0xffffffffu;
-  u32 k[16];
-#pragma unroll
-  for (u32 j = 0; j < 16; j++) k[j] = j < n ? gld(kk + i0 + j) : 0u;
-#pragma unroll
-  for (u32 j = 0; j < 16; j++) {
-    const u32 cur = k[j];
-    if (j >= n) continue;
-    const u32 before = j ? k[j - 1] : prev;
-    const u32 after = j + 1 < n ? k[j + 1] : next;
-    if (cur != before) gst(bs + cur, (u32)(i0 + j));
-    if (cur != after) gst(be + cur, (u32)(i0 + j + 1));
+#pragma unroll 4
+  for (u32 j = 0; j < kBoundsSpan / 256; j++) {
+    const u64 i = i0 + (u64)j * 256 + threadIdx.x;
+    if (i >= s.seg_len) break;
+    const u32 cur = gld(kk + i);
+    const u32 before = i ? gld(kk + i - 1) : 0xffffffffu;
+    const u32 after = i + 1 < s.seg_len ? gld(kk + i + 1) : 0xffffffffu;
+    if (cur != before) gst(bs + cur, (u32)i);
+    if (cur != after) gst(be + cur, (u32)(i + 1));
   }
 }
 
@@ -527,8 +544,15 @@ __device__ __forceinline__ void scan_a_body(const MsmJob& J, u32 kind, u32 x, u6
   const u64 len = scan_len(J.s, kind);
   const u64 base = (u64)ch * kScanChunk + (u64)threadIdx.x * 16;
   u32 sum = 0;
-  for (u32 j = 0; j < 16; j++)
-    if (base + j < len) sum += scan_in(J, kind, x, pw, base + j);
+  if (kind == SC_HIST && base + 16 <= len && aligned16(J.hist + (u64)pw * kRadix * J.s.tiles)) {
+    u32 v[16];
+    gld16(J.hist + (u64)pw * kRadix * J.s.tiles + base, v);
+#pragma unroll
+    for (u32 j = 0; j < 16; j++) sum += v[j];
+  } else {
+    for (u32 j = 0; j < 16; j++)
+      if (base + j < len) sum += scan_in(J, kind, x, pw, base + j);
+  }
   u32 total;
   block_excl_scan(sum, part, &total);
   if (threadIdx.x == 0) J.csum[(u64)pw * (J.s.chunks + 1) + ch] = total;
@@ -549,11 +573,14 @@ __device__ __forceinline__ void scan_c_body(const MsmJob& J, u32 kind, u32 x, u6
   const u64 base = (u64)ch * kScanChunk + (u64)threadIdx.x * 16;
   u32 v[16];
   u32 sum = 0;
+  if (kind == SC_HIST && base + 16 <= len && aligned16(J.hist + (u64)pw * kRadix * J.s.tiles)) {
+    gld16(J.hist + (u64)pw * kRadix * J.s.tiles + base, v);
+  } else {
 #pragma unroll
-  for (u32 j = 0; j < 16; j++) {
-    v[j] = base + j < len ? scan_in(J, kind, x, pw, base + j) : 0u;
-    sum += v[j];
+    for (u32 j = 0; j < 16; j++) v[j] = base + j < len ? scan_in(J, kind, x, pw, base + j) : 0u;
   }
+#pragma unroll
+  for (u32 j = 0; j < 16; j++) sum += v[j];
   u32 total;
   u32 run = block_excl_scan(sum, part, &total) + J.csum[(u64)pw * (J.s.chunks + 1) + ch];
   u32* out = scan_out(J, kind, pw);
@@ -641,7 +668,7 @@ __device__ __forceinline__ void tasks_body(const MsmJob& J, u64 gi) {
     if (b != 0) acc = xyzz29_madd(acc, unpack_entry(cur));
     if (j + 1 == hi || bn != b) {
       if (b != 0)
-        store_xyzz29(J.partA + kPt * (pbase + to[b] + (u32)(c - (bs[b] - z) / s.T)), acc);
+        store_xyzz29_g(J.partA + kPt * (pbase + to[b] + (u32)(c - (bs[b] - z) / s.T)), acc);
       acc = G1Xyzz29::identity();
     }
     cur = nxt;
@@ -676,10 +703,16 @@ __device__ __forceinline__ void combine_body(const MsmJob& J, u32 level, u64 gi)
   u32* dst = (level & 1) ? J.partB : J.partA;
   const u64 base = (u64)pw * s.task_cap + to[b];
   const u32 G = 1u << s.g;
-  G1Xyzz29 acc = load_xyzz29(src + kPt * (base + (u64)k * G));
-  for (u32 j = 1; j < G && k * G + j < n_prev; j++)
-    acc = xyzz29_add(acc, load_xyzz29(src + kPt * (base + (u64)k * G + j)));
-  store_xyzz29(dst + kPt * (base + k), acc);
+  // the next partial is loaded before each addition
+  G1Xyzz29 acc = load_xyzz29_g(src + kPt * (base + (u64)k * G));
+  const u32 jn = n_prev - k * G < G ? n_prev - k * G : G;
+  G1Xyzz29 nxt = jn > 1 ? load_xyzz29_g(src + kPt * (base + (u64)k * G + 1)) : acc;
+  for (u32 j = 1; j < jn; j++) {
+    const G1Xyzz29 cur = nxt;
+    if (j + 1 < jn) nxt = load_xyzz29_g(src + kPt * (base + (u64)k * G + j + 1));
+    acc = xyzz29_add(acc, cur);
+  }
+  store_xyzz29_g(dst + kPt * (base + k), acc);
 }
 
 // Window sum S = sum_b b * B_b (b = 1 .. nb) in three shallow stages.
@@ -705,12 +738,12 @@ __device__ __forceinline__ void reduce_body(const MsmJob& J, u64 t) {
     const u32 nt = to[b + 1] - to[b];
     if (nt) {
       const u32* pp = (bucket_levels(nt, s.g) & 1) ? J.partB : J.partA;
-      run = xyzz29_add(run, load_xyzz29(pp + kPt * (pbase + to[b])));
+      run = xyzz29_add(run, load_xyzz29_g(pp + kPt * (pbase + to[b])));
     }
     tot = xyzz29_add(tot, run);
   }
-  store_xyzz29(J.rparts + kPt * ((u64)(2 * pw) * s.nseg + sg), tot);
-  store_xyzz29(J.rparts + kPt * ((u64)(2 * pw + 1) * s.nseg + sg), run);
+  store_xyzz29_g(J.rparts + kPt * ((u64)(2 * pw) * s.nseg + sg), tot);
+  store_xyzz29_g(J.rparts + kPt * ((u64)(2 * pw + 1) * s.nseg + sg), run);
 }
 
 // pairwise tree over the 256 XYZZ values red[0..255] with the given stride
@@ -741,15 +774,19 @@ __device__ __forceinline__ void sum_body(const MsmJob& J, u32 level, u64 lb, u32
     const u32 count = q == 0 ? s.nseg : s.nseg / 2;
     const u32* src = J.rparts + kPt * ((u64)(2 * pw + (q ? 1 : 0)) * s.nseg);
     const u32 l = q ? q - 1 : 0;
+    // both inputs loaded before the addition
+    G1Xyzz29 in[2];
 #pragma unroll
     for (u32 h = 0; h < 2; h++) {
       const u32 i = g * 512 + h * 256 + t;
+      in[h] = G1Xyzz29::identity();
       if (i < count) {
         // q > 0: the i-th segment index with bit l set
         const u32 sg = q == 0 ? i : (((i >> l) << (l + 1)) | (1u << l) | (i & ((1u << l) - 1)));
-        acc = xyzz29_add(acc, load_xyzz29(src + kPt * (u64)sg));
+        in[h] = load_xyzz29_g(src + kPt * (u64)sg);
       }
     }
+    acc = xyzz29_add(in[0], in[1]);
     store_xyzz29(red + kPt * t, acc);
     __syncthreads();
     block_tree_sum(red);
@@ -757,7 +794,7 @@ __device__ __forceinline__ void sum_body(const MsmJob& J, u32 level, u64 lb, u32
   } else {
     const u64 v = lb;  // pw * nq + q
     for (u32 i = t; i < s.g0; i += 256)
-      acc = xyzz29_add(acc, load_xyzz29(J.sparts + kPt * (v * s.g0 + i)));
+      acc = xyzz29_add(acc, load_xyzz29_g(J.sparts + kPt * (v * s.g0 + i)));
     store_xyzz29(red + kPt * t, acc);
     __syncthreads();
     block_tree_sum(red);
@@ -783,7 +820,7 @@ __device__ __forceinline__ void final_body(const MsmJob& J, u32* red) {
     G1Xyzz29 v = G1Xyzz29::identity();
     if (i == 0 || i >= s.sbits) {
       const u32 q = i == 0 ? 0 : i - s.sbits + 1;
-      v = load_xyzz29(parts + kPt * (((u64)pw * (s.lbits + 1) + q) * G + gi));
+      v = load_xyzz29_g(parts + kPt * (((u64)pw * (s.lbits + 1) + q) * G + gi));
     }
     store_xyzz29(red + kPt * t, v);
   }
