@@ -521,16 +521,54 @@ __host__ __device__ inline u32 scan_chunks(const MsmShape& s, u32 kind) {
   return (u32)((scan_len(s, kind) + kScanChunk - 1) / kScanChunk);
 }
 
+// groups of combine level x of a bucket with nt tasks (0 once one is left)
+__device__ __forceinline__ u32 level_groups(u32 nt, u32 x, u32 g) {
+  if (x == 0) return nt;  // level 0: the tasks themselves
+  const u32 shp = g * (x - 1), sh = g * x;
+  const u32 nprev = (u32)(((u64)nt + (1ull << shp) - 1) >> shp);
+  return nprev > 1 ? (u32)(((u64)nt + (1ull << sh) - 1) >> sh) : 0u;
+}
+
 __device__ __forceinline__ u32 scan_in(const MsmJob& J, u32 kind, u32 x, u32 pw, u64 i) {
   const MsmShape& s = J.s;
   if (kind == SC_HIST) return J.hist[(u64)pw * kRadix * s.tiles + i];
   if (i == 0) return 0;
   const u64 bo = (u64)pw * (s.nb + 1) + i;
   const u32 nt = bucket_tasks(J.bstart[bo], J.bend[bo], J.bend[(u64)pw * (s.nb + 1)], s.T);
-  if (kind == SC_TASKS) return nt;
-  const u32 shp = s.g * (x - 1), sh = s.g * x;
-  const u32 nprev = (u32)(((u64)nt + (1ull << shp) - 1) >> shp);
-  return nprev > 1 ? (u32)(((u64)nt + (1ull << sh) - 1) >> sh) : 0u;
+  return kind == SC_TASKS ? nt : level_groups(nt, x, s.g);
+}
+
+// inputs base .. base + 15 of a scan: 16-byte loads when the run is whole and
+// aligned, one value at a time otherwise
+__device__ __forceinline__ void scan_vals16(const MsmJob& J, u32 kind, u32 x, u32 pw, u64 base,
+                                            u64 len, u32 v[16]) {
+  const MsmShape& s = J.s;
+  if (base + 16 <= len) {
+    if (kind == SC_HIST) {
+      const u32* h = J.hist + (u64)pw * kRadix * s.tiles + base;
+      if (aligned16(h)) {
+        gld16(h, v);
+        return;
+      }
+    } else {
+      const u32* bs = J.bstart + (u64)pw * (s.nb + 1) + base;
+      const u32* be = J.bend + (u64)pw * (s.nb + 1) + base;
+      if (aligned16(bs) && aligned16(be)) {
+        u32 a[16], b[16];
+        gld16(bs, a);
+        gld16(be, b);
+        const u32 z = gld(J.bend + (u64)pw * (s.nb + 1));
+#pragma unroll
+        for (u32 j = 0; j < 16; j++) {
+          const u32 nt = base + j == 0 ? 0u : bucket_tasks(a[j], b[j], z, s.T);
+          v[j] = kind == SC_TASKS ? nt : level_groups(nt, x, s.g);
+        }
+        return;
+      }
+    }
+  }
+#pragma unroll
+  for (u32 j = 0; j < 16; j++) v[j] = base + j < len ? scan_in(J, kind, x, pw, base + j) : 0u;
 }
 
 __device__ __forceinline__ u32* scan_out(const MsmJob& J, u32 kind, u32 pw) {
@@ -543,16 +581,11 @@ __device__ __forceinline__ void scan_a_body(const MsmJob& J, u32 kind, u32 x, u6
   const u32 pw = (u32)(lb / nch), ch = (u32)(lb % nch);
   const u64 len = scan_len(J.s, kind);
   const u64 base = (u64)ch * kScanChunk + (u64)threadIdx.x * 16;
+  u32 v[16];
+  scan_vals16(J, kind, x, pw, base, len, v);
   u32 sum = 0;
-  if (kind == SC_HIST && base + 16 <= len && aligned16(J.hist + (u64)pw * kRadix * J.s.tiles)) {
-    u32 v[16];
-    gld16(J.hist + (u64)pw * kRadix * J.s.tiles + base, v);
 #pragma unroll
-    for (u32 j = 0; j < 16; j++) sum += v[j];
-  } else {
-    for (u32 j = 0; j < 16; j++)
-      if (base + j < len) sum += scan_in(J, kind, x, pw, base + j);
-  }
+  for (u32 j = 0; j < 16; j++) sum += v[j];
   u32 total;
   block_excl_scan(sum, part, &total);
   if (threadIdx.x == 0) J.csum[(u64)pw * (J.s.chunks + 1) + ch] = total;
@@ -572,22 +605,27 @@ __device__ __forceinline__ void scan_c_body(const MsmJob& J, u32 kind, u32 x, u6
   const u64 len = scan_len(J.s, kind);
   const u64 base = (u64)ch * kScanChunk + (u64)threadIdx.x * 16;
   u32 v[16];
+  scan_vals16(J, kind, x, pw, base, len, v);
   u32 sum = 0;
-  if (kind == SC_HIST && base + 16 <= len && aligned16(J.hist + (u64)pw * kRadix * J.s.tiles)) {
-    gld16(J.hist + (u64)pw * kRadix * J.s.tiles + base, v);
-  } else {
-#pragma unroll
-    for (u32 j = 0; j < 16; j++) v[j] = base + j < len ? scan_in(J, kind, x, pw, base + j) : 0u;
-  }
 #pragma unroll
   for (u32 j = 0; j < 16; j++) sum += v[j];
   u32 total;
   u32 run = block_excl_scan(sum, part, &total) + J.csum[(u64)pw * (J.s.chunks + 1) + ch];
-  u32* out = scan_out(J, kind, pw);
+  u32* out = scan_out(J, kind, pw) + base;
+  u32 o[16];
 #pragma unroll
   for (u32 j = 0; j < 16; j++) {
-    if (base + j < len) out[base + j] = run;
+    o[j] = run;
     run += v[j];
+  }
+  if (base + 16 <= len && aligned16(out)) {
+    uint4* q = reinterpret_cast<uint4*>(out);
+#pragma unroll
+    for (u32 k = 0; k < 4; k++) gst(q + k, make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]));
+  } else {
+#pragma unroll
+    for (u32 j = 0; j < 16; j++)
+      if (base + j < len) out[j] = o[j];
   }
 }
 
