@@ -117,7 +117,7 @@ __device__ __forceinline__ Fr tw_load(const Twiddle& tw, bool two, u64 e) {
 
 // Lazy butterflies of the LDS passes (Harvey, "Faster arithmetic for
 // number-theoretic transforms", 2014): values live in [0, 4m) between levels
-// (4m < 2^256 for Fr).  (a, b) -> (a' + t, a' + 2m - t) with a' = a mod 2m
+// and between passes (4m < 2^256 for Fr; the last pass stores canonical values).  (a, b) -> (a' + t, a' + 2m - t) with a' = a mod 2m
 // folded into [0, 2m) and t = b*w by the Montgomery product without its final
 // subtraction (b < 4m, w < m give t < 2m), or b folded into [0, 2m) when w = 1.
 __device__ __forceinline__ Fr fold2m(const Fr& a) {  // [0, 4m) -> [0, 2m)
@@ -487,7 +487,10 @@ __device__ __forceinline__ void dit_body(const NttJob& J, u32 stage, u64 wg, u32
       p = (hi << (s + D)) | ((u64)g << s) | (lo0 + c);
     }
     Fr v = pl_load(T, dslot(g * C + c));
-    v = epi ? mul<FrParams>(v, factor_at(J.epi, p)) : canon4m(v);
+    // between passes the lazy value goes to memory as it is (the next
+    // pass's butterflies take [0, 4m)); the last pass stores canonical values
+    if (epi) v = mul<FrParams>(v, factor_at(J.epi, p));
+    else if (last) v = canon4m(v);
     store_fe<Fr>(rd_rev ? J.tmp + 8 * p : J.x.base + 8 * view_phys(J.x, p), v);
   }
 }
