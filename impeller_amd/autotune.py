@@ -5,7 +5,7 @@ The tuner times every candidate configuration of an operation on the GPU
 timing database.  The cost model (costmodel.CostModel, an FP32 MLP trained
 on those rows by the library's HIP kernels) then ranks candidates for a new
 problem without a sweep; the exhaustive result stays the fallback and the
-judge of the pick's regret.
+yardstick of the pick's regret.
 
 Configuration spaces:
   NTT  plan = [fuse, lds_log, d0, d1, ...]  (fuse in {0, 4, 5}: DIT levels

@@ -1,4 +1,4 @@
-"""Profiling driver for the NTT kernels: a few forward and coset NTTs of one
+"""Profiling harness for the NTT kernels: a few forward and coset NTTs of one
 size on device-resident data, plus one plain device copy of the same array
 (a known byte count, used to calibrate the HBM counters FETCH_SIZE and
 WRITE_SIZE).  Run it under rocprofv3, e.g.
