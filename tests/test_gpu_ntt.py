@@ -36,7 +36,8 @@ def host_twiddles(w_mont, k):
     return O.powers(w_mont, max(1, (1 << k) // 2))
 
 
-@pytest.mark.parametrize("k", list(range(1, 15)) + [16, 18])
+# 20: the default plan fuses the bit reversal into the first pass from 2^20
+@pytest.mark.parametrize("k", list(range(1, 15)) + [16, 18, 20])
 def test_ssip_ntt_random_omega(stream, k):
     print(f"ssip_ntt k={k}", flush=True)
     x = O.gen_scalars(1 << k, 1000 + k)
@@ -49,7 +50,7 @@ def test_ssip_ntt_random_omega(stream, k):
     assert (xa.to_numpy() == expect).all()
 
 
-@pytest.mark.parametrize("k", [10, 13])
+@pytest.mark.parametrize("k", [10, 13, 20])
 def test_ssip_ntt_rotated_slice(stream, k):
     # core/tests/ntt.rs:92-97: buffer of 2^(k+1), rotate(3), slice(2^k, 2^(k+1))
     x = O.gen_scalars(1 << k, 2000 + k)
@@ -78,7 +79,7 @@ def test_ssip_precompute_matches_powers():
     assert (tw.to_numpy() == O.powers(w, 1 << 11)).all()
 
 
-@pytest.mark.parametrize("k", [2, 9, 12, 15])
+@pytest.mark.parametrize("k", [2, 9, 12, 15, 20])
 def test_recompute_ntt(stream, k):
     # core/tests/ntt.rs:157-281: rotate(-2), pq/omegas from gen_pq_omegas
     g = N.GenPqOmegas()
