@@ -231,6 +231,8 @@ struct NttJob {
   u32 nst;
   u32 br_blocks;
   u32 fbr;   // bit reversal fused into stage 0 (out of place through tmp)
+  u32 zlev;  // fbr only: source positions >= 2^(L - zlev) read as zero, so
+             // stage 0's first zlev levels are copies (zlev <= its depth)
   u32* tmp;  // 2^L elements: stage 0 writes, stage 1 reads (fbr only)
   NttStage st[8];
 };
@@ -395,11 +397,18 @@ __device__ __forceinline__ void dit_body(const NttJob& J, u32 stage, u64 wg, u32
   }
   const Planar T{t, GC};
   __syncthreads();
+  // zero-extended source (zlev > 0): a row g = rev gq is nonzero only when
+  // its low zlev bits are clear (gq < 2^(D - zlev)), and DIT levels
+  // 0 .. zlev-1 with a zero partner (a, 0) -> (a, a) only copy it to the rows
+  // that differ in those bits: store those copies and start at level zlev
+  const u32 z = rd_rev ? J.zlev : 0;
   if (rd_rev) {
-    for (u32 idx = threadIdx.x; idx < GC; idx += blockDim.x) {
+    for (u32 idx = threadIdx.x; idx < (GC >> z); idx += blockDim.x) {
       const u32 c = idx & (C - 1), gq = idx >> logC;
       const u64 q = ((u64)gq << (L - D)) | (wg << logC) | c;
-      pl_store(T, dslot(brev_bits(gq, D) * C + c), src_load(J, q));
+      const Fr v = src_load(J, q);
+      const u32 g = brev_bits(gq, D);
+      for (u32 r = 0; r < (1u << z); r++) pl_store(T, dslot((g | r) * C + c), v);
     }
   } else {
     for (u32 idx = threadIdx.x; idx < GC; idx += blockDim.x) {
@@ -419,7 +428,7 @@ __device__ __forceinline__ void dit_body(const NttJob& J, u32 stage, u64 wg, u32
     }
   }
   __syncthreads();
-  u32 j = 0;
+  u32 j = z;
   for (; j + 1 < D; j += 2) {
     const u32 l = s + j;
     const u32 sh1 = L - l - 2;
@@ -678,6 +687,7 @@ static NttJob make_job(View x, View src, u64 src_len, u32 L, const Twiddle& tw, 
   J.tb = plan.fuse == 5 ? 5 : 4;
   J.fuse = (plan.fuse == J.tb && L >= 2 * J.tb) ? J.tb : 0;
   J.fbr = plan.fuse == 1 ? 1 : 0;
+  J.zlev = 0;
   J.tmp = nullptr;
   J.two = two ? 1 : 0;
   J.tw = tw;
@@ -698,6 +708,13 @@ static NttJob make_job(View x, View src, u64 src_len, u32 L, const Twiddle& tw, 
     st.logC = logC;
     st.blocks = (u32)(1ull << (L - D - logC));
     s += D;
+  }
+  if (J.fbr && J.nst > 0) {
+    // zero extension: the source fits in 2^(L - e) positions
+    u32 lg = 0;
+    while (lg < L && (1ull << lg) < src_len) lg++;
+    const u32 e = L - lg;
+    J.zlev = e < J.st[0].D ? e : J.st[0].D;
   }
   return J;
 }

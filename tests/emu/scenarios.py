@@ -127,13 +127,14 @@ def batched_ntt_mixed(E):
 def batched_ntt_plans(E):
     for second in (True, False):
         items, expects = ntt_items([("fwd_random", 14), ("inv", 12), ("ext", 14), ("fwd", 9),
-                                    ("ext", 13), ("inv", 11)], 5)
+                                    ("ext", 13), ("inv", 11), ("ext", 12)], 5)
         items[0]["plan"] = [5, 11, 9]
         items[1]["plan"] = [0, 10, 2, 3, 7]
         items[2]["plan"] = [5, 10, 4, 5]
         items[3]["plan"] = [1, 10, 4, 5]
         items[4]["plan"] = [1, 10, 6, 7]  # coset extension with the reversal fused
         items[5]["plan"] = [1, 11, 2, 9]
+        items[6]["plan"] = [1, 10, 2, 10]  # zero-extension levels beyond the first pass
         NttBatch(E, items, second_buffers=second).run(7 if second else 0)
         check_items(items, expects, f"plans second_buffers={second}")
 
