@@ -258,13 +258,16 @@ def main():
 
     def picked(prob, key):
         pk = picks.get(prob)
+        if pk is None and prob.startswith("ntt_ext:"):
+            # no zero-extension timings: the plain coset problem's pick
+            pk = picks.get("ntt_coset:" + prob.split(":")[1])
         return pk.get(key) if pk else None
 
     def ntt_plans(key):
         out = {}
         for kind, lg in WL.ntt_batch_spec():
             L = lg + 3 if kind == "ext" else lg
-            op = {"fwd": "ntt_fwd", "inv": "ntt_inv", "ext": "ntt_coset"}[kind]
+            op = {"fwd": "ntt_fwd", "inv": "ntt_inv", "ext": "ntt_ext"}[kind]
             pl = picked(f"{op}:{L}", key)
             if pl:
                 out[(kind, L)] = pl
@@ -353,7 +356,7 @@ def main():
 
         def chunk_plans(key):
             out = {}
-            for kind, L, op in (("inv", pc.k, "ntt_inv"), ("ext", K, "ntt_coset"),
+            for kind, L, op in (("inv", pc.k, "ntt_inv"), ("ext", K, "ntt_ext"),
                                 ("inv", K, "ntt_inv")):
                 pl = picked(f"{op}:{L}", key)
                 if pl:

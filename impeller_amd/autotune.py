@@ -21,6 +21,9 @@ import os
 import numpy as np
 
 OPS = ["ntt_fwd", "ntt_inv", "ntt_coset", "msm"]
+# ntt_ext: the coset extension of 2^(L-3) coefficients to 2^L points (a batch
+# item with a zero-extended source), featured as ntt_coset plus its levels
+EXT_LEVELS = 3
 N_FEATURES = 24
 
 
@@ -85,13 +88,18 @@ def default_msm(log_n):
 # / 5, 6 LDS tile log - 10, 7 passes, 8/9 deepest/shallowest pass, 16-19
 # pass depths (sorted, zero padded), 20 levels after the fused ones, 21
 # passes deeper than the LDS tile, 22 bit reversal fused into the first
-# pass, 23 log2 column run of that pass.  MSM: 10 window c, 11 precomputed tables,
+# pass, 23 log2 column run of that pass, 10 zero-extension levels (ntt_ext,
+# whose fused first pass skips them).  MSM: 10 window c, 11 precomputed tables,
 # 12 log2 task, 13/14/7 fraction of zero / one / small scalars, then the
 # derived shape: 16 log2 entries per bucket set, 17 log2 entries per bucket,
 # 18 sort passes, 19 bucket sets, 20 log2 tasks per bucket, 21 log2 buckets.
 def ntt_features(op, L, plan, batch=1):
     f = np.zeros(N_FEATURES, np.float32)
-    f[OPS.index(op)] = 1
+    if op == "ntt_ext":
+        f[OPS.index("ntt_coset")] = 1
+        f[10] = EXT_LEVELS
+    else:
+        f[OPS.index(op)] = 1
     f[4] = L
     fl = fused_levels(plan[0])
     f[5] = fl / 5.0
@@ -218,8 +226,16 @@ def tune_ntt(db, L, stream, ops=("ntt_fwd",), reps=3, plans=None):
             if op == "ntt_coset":
                 kw["coset_shift"] = w
 
-            def run(plan=plan, kw=kw):
-                N.ntt(x, tw, L, stream, passes=plan, **kw)
+            if op == "ntt_ext":
+                # one batch item: the first 2^(L-3) elements of x as its source
+                b = N.NttBatch([dict(data=x, src=x, log_n=L, log_src=L - EXT_LEVELS,
+                                     twiddle=tw, coset_shift=w, plan=plan)], stream)
+
+                def run(b=b):
+                    b.run(stream)
+            else:
+                def run(plan=plan, kw=kw):
+                    N.ntt(x, tw, L, stream, passes=plan, **kw)
 
             ms = _median_ms(run, stream, reps)
             db.add(problem, plan, ntt_features(op, L, plan), ms)

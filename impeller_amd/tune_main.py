@@ -59,10 +59,11 @@ def main():
             picks[prob] = {"exhaustive": best["config"], "exhaustive_ms": best["ms"]}
         print(f"[tune] reused {len(db.rows)} rows from {args.reuse}", flush=True)
     for L in (parse_range(args.ntt_logs) if sweep_ntt else []):
-        best = A.tune_ntt(db, L, stream, ops=("ntt_fwd", "ntt_inv", "ntt_coset"), reps=args.reps)
+        best = A.tune_ntt(db, L, stream, ops=("ntt_fwd", "ntt_inv", "ntt_coset", "ntt_ext"),
+                          reps=args.reps)
         for prob, (cfg, ms) in best.items():
             picks[prob] = {"exhaustive": cfg, "exhaustive_ms": ms}
-        print(f"[tune] ntt 2^{L}: {len(A.ntt_plans(L))} plans x 3 ops, {time.time() - t0:.1f}s",
+        print(f"[tune] ntt 2^{L}: {len(A.ntt_plans(L))} plans x 4 ops, {time.time() - t0:.1f}s",
               flush=True)
     msm_logs = parse_range(args.msm_logs)
     if sweep_msm:
