@@ -3,7 +3,7 @@ ctypes signatures; numpy arrays stand in for device buffers.  Used by
 tests/test_emu_kernels.py and runnable under AddressSanitizer:
 
   LD_PRELOAD=$(clang++ -print-file-name=libclang_rt.asan-x86_64.so) \
-  ASAN_OPTIONS=detect_leaks=0 python tests/emu/driver.py --asan [scenario ...]
+  ASAN_OPTIONS=detect_leaks=0 python tests/emu/harness.py --asan [scenario ...]
 """
 import ctypes
 import os

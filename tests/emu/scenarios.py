@@ -11,7 +11,7 @@ import time
 
 import numpy as np
 
-from tests.emu.driver import Emu, garbage_u32, ptr, u32
+from tests.emu.harness import Emu, garbage_u32, ptr, u32
 from impeller_amd import _lib
 from oracle import oracle as O
 

@@ -19,7 +19,7 @@ pytestmark = pytest.mark.skipif(not B.CLANG or not shutil.which(B.CLANG) and
 @pytest.fixture(scope="module")
 def emu():
     B.build()
-    from tests.emu.driver import Emu
+    from tests.emu.harness import Emu
     return Emu()
 
 
