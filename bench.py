@@ -42,18 +42,22 @@ def parse():
     return ap.parse_args()
 
 
-def dit_traffic(log_n):
-    """HBM bytes per k_ntt_dit launch from the committed counter summary
-    (profiles/ntt<log>_dit_hbm_traffic.txt, line "traffic_bytes_per_launch: B"),
-    or None."""
-    path = os.path.join(ROOT, "profiles", f"ntt{log_n}_dit_hbm_traffic.txt")
+def summary_value(name, key):
+    """An integer "key: value" line of a committed counter summary under
+    profiles/, or None."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
         for line in f:
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return int(line.split(":")[1])
     return None
+
+
+def dit_traffic(log_n):
+    """HBM bytes per k_ntt_dit launch (profiles/ntt<log>_dit_hbm_traffic.txt)."""
+    return summary_value(f"ntt{log_n}_dit_hbm_traffic.txt", "traffic_bytes_per_launch")
 
 
 def load_picks():
@@ -246,7 +250,10 @@ def main():
                 "achieved": round(10 * msm_madds / (tasks_avg_ms * 1e-3), 1),
                 "peak": round(rate_fq29, 1), "unit": "Fq mont-mul/s (9 x 29-bit form)",
                 "frac": round(10 * msm_madds / (tasks_avg_ms * 1e-3) / rate_fq29, 4),
-                "kernel_avg_ms": round(tasks_avg_ms, 4)},
+                "kernel_avg_ms": round(tasks_avg_ms, 4),
+                # fetch + write bytes per launch at 2^22, c = 20 (counter summary)
+                "traffic": summary_value("msm22_hbm_traffic.txt", "kb_tasks_traffic_bytes_per_launch")
+                if (nm, best_c) == (1 << 22, 20) else None},
     }
 
     # ---------------------------------------------------- batched vs plain
