@@ -282,6 +282,48 @@ def batched_msm_shared_tables(E, nb=1 << 12):
             assert (got[i] == e).all(), (i, items[i]["len"], max_blocks)
 
 
+def degenerate_msm_items(bases, tabs, k, nb):
+    """Batch items whose sums are the identity or trivial: an empty MSM, all
+    zero scalars, r - 1 and 1 on one repeated base, beside a normal item."""
+    zeros = np.zeros((300, 8), np.uint32)
+    cancel = np.zeros((64, 8), np.uint32)
+    cancel[0] = O.int_to_limbs(O.to_mont_int(O.R_MOD - 1))
+    cancel[1] = O.one()
+    bb = np.ascontiguousarray(bases[: 64 * 16]).reshape(64, 16).copy()
+    bb[1] = bb[0]
+    normal = O.gen_scalars(257, 808)
+    specs = [  # (scalars, bases, tables, len, c, target)
+        (np.zeros((1, 8), np.uint32), None, None, 0, 12, 0),
+        (zeros, None, (13, 1), 300, 13, 1),
+        (cancel, bb, None, 64, 10, 0),
+        (normal, None, (11, 3), 257, 11, 3),
+    ]
+    items, expects = [], []
+    for s, b, tab, n, c, target in specs:
+        if b is not None:
+            t = flat(b)
+            expects.append(O.best_multiexp(s, b))
+        else:
+            t = bases if tab is None else tabs[tab]
+            expects.append(O.msm_closed_form(s[:n], k[:n]) if n else np.zeros(16, np.uint32))
+        items.append(dict(tables=t, scalars=flat(s), len=n, window=c, target=target,
+                          table_stride=nb if target else 0))
+    return items, expects
+
+
+def batched_msm_degenerate(E, nb=1 << 10):
+    bases, k, tabs = msm_items(E, nb)
+    items, expects = degenerate_msm_items(bases, tabs, k, nb)
+    assert not expects[0].any() and not expects[1].any() and not expects[2].any()
+    b = MsmBatch(E, items)
+    for max_blocks in (0, 16):
+        b.results[:] = 0xFFFFFFFF
+        b.run(max_blocks)
+        got = b.results.reshape(-1, 16)
+        for i, e in enumerate(expects):
+            assert (got[i] == e).all(), (i, items[i]["len"], max_blocks)
+
+
 def msm_graph_replay(E):
     n = 1 << 9
     bases = u32(n * 16)
@@ -456,7 +498,8 @@ ALL = [("single_ntt", single_ntt), ("ntt_plans", ntt_plans),
        ("ntt_fused_reversal", ntt_fused_reversal), ("batched_ntt_mixed", batched_ntt_mixed),
        ("batched_ntt_plans", batched_ntt_plans), ("ntt_graph_replay", ntt_graph_replay),
        ("gate", gate), ("field_inverse", field_inverse), ("arith29", arith29), ("mlp_grad", mlp_grad), ("single_msm", single_msm), ("msm_hot_bucket", msm_hot_bucket), ("msm_graph_replay", msm_graph_replay),
-       ("batched_msm_shared_tables", batched_msm_shared_tables)]
+       ("batched_msm_shared_tables", batched_msm_shared_tables),
+       ("batched_msm_degenerate", batched_msm_degenerate)]
 
 
 def main(argv):

@@ -195,3 +195,46 @@ def test_batched_msm_graph_replay():
             got = b.results_host()
             for i, s in enumerate(ss):
                 assert (got[i] == O.msm_closed_form(s, k)).all(), (rep, i)
+
+
+def test_batched_msm_degenerate_items(stream):
+    """An empty MSM, all-zero scalars and a cancelling pair (identity (0, 0))
+    batched beside a normal item; results start as all-ones words."""
+    from impeller_amd import _lib
+    from impeller_amd import msm as M
+    nb = 1 << 12
+    bases = device_u32(nb * 16)
+    _lib.call("impeller_gen_bases", bases.data_ptr(), nb, 93, 0, stream.raw())
+    k = O.gen_scalars(nb, 93)
+    tabs = {}
+    for c, target in ((13, 1), (11, 3)):
+        t = device_u32(M.n_tables(c, target) * nb * 16)
+        t[: nb * 16].copy_(bases)
+        M.precompute_device(t, nb, c, target, stream)
+        tabs[(c, target)] = t
+    cancel = np.zeros((64, 8), np.uint32)
+    cancel[0] = O.int_to_limbs(O.to_mont_int(O.R_MOD - 1))
+    cancel[1] = O.one()
+    bb = to_host(bases, 16)[:64].copy()
+    bb[1] = bb[0]
+    normal = O.gen_scalars(1000, 808)
+    items = [
+        dict(tables=bases, scalars=to_device(np.zeros((1, 8), np.uint32)), len=0, window=12,
+             target=0),
+        dict(tables=tabs[(13, 1)], scalars=to_device(np.zeros((300, 8), np.uint32)), len=300,
+             window=13, target=1, table_stride=nb),
+        dict(tables=to_device(bb), scalars=to_device(cancel), len=64, window=10, target=0),
+        dict(tables=tabs[(11, 3)], scalars=to_device(normal), len=1000, window=11, target=3,
+             table_stride=nb),
+    ]
+    expects = [np.zeros(16, np.uint32)] * 2 + [O.best_multiexp(cancel, bb),
+                                               O.msm_closed_form(normal, k[:1000])]
+    assert not expects[2].any()
+    b = M.MsmBatch(items, stream)
+    for max_blocks in (0, 16):
+        b.results.fill_(-1)
+        b.run(stream, max_blocks)
+        stream.sync()
+        got = b.results_host()
+        for i, e in enumerate(expects):
+            assert (got[i] == e).all(), (i, items[i]["len"], max_blocks)
