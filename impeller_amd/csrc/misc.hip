@@ -10,24 +10,63 @@ namespace imp {
 // out = a*b*c - d + q*e*(1 - q) over Fr, one element per index, grid-stride
 // (the generated kernels' loop, core/src/fused_kernels/kernel_gen.rs:281-304).
 // Operands are PolyPtr views (make_slice_iter semantics, iter.cuh:309-327).
+// The gate is a Lagrange-form expression (its constant 1 applies at every
+// index), so the slice, which only places Coef-form constants
+// (kernel_gen.rs:525-555), does not change its values.
 struct GateViews {
   View in[6];
   View out;
 };
 
+__device__ __forceinline__ void gate_at(const GateViews& g, u64 i) {
+  Fr a = load_fe<Fr>(g.in[0].base + 8 * view_phys(g.in[0], i));
+  Fr b = load_fe<Fr>(g.in[1].base + 8 * view_phys(g.in[1], i));
+  Fr c = load_fe<Fr>(g.in[2].base + 8 * view_phys(g.in[2], i));
+  Fr d = load_fe<Fr>(g.in[3].base + 8 * view_phys(g.in[3], i));
+  Fr q = load_fe<Fr>(g.in[4].base + 8 * view_phys(g.in[4], i));
+  Fr e = load_fe<Fr>(g.in[5].base + 8 * view_phys(g.in[5], i));
+  Fr t1 = sub<FrParams>(mul<FrParams>(mul<FrParams>(a, b), c), d);
+  Fr t2 = mul<FrParams>(mul<FrParams>(q, e), sub<FrParams>(Fr::one(), q));
+  store_fe<Fr>(g.out.base + 8 * view_phys(g.out, i), add<FrParams>(t1, t2));
+}
+
 __global__ __launch_bounds__(256) void k_gate(GateViews g, u64 len) {
   const u64 stride = (u64)gridDim.x * blockDim.x;
-  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
-    Fr a = load_fe<Fr>(g.in[0].base + 8 * view_phys(g.in[0], i));
-    Fr b = load_fe<Fr>(g.in[1].base + 8 * view_phys(g.in[1], i));
-    Fr c = load_fe<Fr>(g.in[2].base + 8 * view_phys(g.in[2], i));
-    Fr d = load_fe<Fr>(g.in[3].base + 8 * view_phys(g.in[3], i));
-    Fr q = load_fe<Fr>(g.in[4].base + 8 * view_phys(g.in[4], i));
-    Fr e = load_fe<Fr>(g.in[5].base + 8 * view_phys(g.in[5], i));
-    Fr t1 = sub<FrParams>(mul<FrParams>(mul<FrParams>(a, b), c), d);
-    Fr t2 = mul<FrParams>(mul<FrParams>(q, e), sub<FrParams>(Fr::one(), q));
-    store_fe<Fr>(g.out.base + 8 * view_phys(g.out, i), add<FrParams>(t1, t2));
-  }
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) gate_at(g, i);
+}
+
+// A PolyPtr read from device memory as a view (make_slice_iter: a non-zero
+// rotate means offset = len - rotate on an unsliced array)
+__device__ __forceinline__ View dev_view(const PolyPtr* p) {
+  const PolyPtr v = gld(p);
+  View r;
+  r.base = v.ptr;
+  const u64 wl = v.whole_len ? v.whole_len : v.len;
+  r.wl = wl ? wl : 1;
+  r.off = v.rotate != 0 ? v.len - v.rotate : v.offset;
+  return r;
+}
+
+// The reference calling convention: the views live in device memory
+// (fused_kernels.rs:172-188 copies them there), so each thread reads them.
+__global__ __launch_bounds__(256) void k_gate_dev(const ConstPolyPtr* vars, const PolyPtr* mut_vars,
+                                                  u64 len) {
+  GateViews g;
+#pragma unroll
+  for (int k = 0; k < 6; k++) g.in[k] = dev_view(reinterpret_cast<const PolyPtr*>(vars) + k);
+  g.out = dev_view(mut_vars);
+  const u64 stride = (u64)gridDim.x * blockDim.x;
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) gate_at(g, i);
+}
+
+static hipError_t gate_grid(unsigned long long len, u32* blocks) {
+  int dev = 0, cus = 256;
+  IMP_CHECK(hipGetDevice(&dev));
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus = 256;
+  u64 b = (len + 255) / 256, cap = (u64)cus * 8;
+  *blocks = (u32)(b > cap ? cap : b);
+  return hipSuccess;
 }
 
 // ---------------------------------------------------------------- generators
@@ -181,35 +220,15 @@ extern "C" hipError_t impeller_mont_mul_rate(int form, unsigned int blocks, unsi
 extern "C" hipError_t fused_gate_abc_d_qe(const ConstPolyPtr* vars, const PolyPtr* mut_vars,
                                           unsigned long long len, const PolySlice* slice_info,
                                           void* local_buffer, hipStream_t stream) {
-  (void)slice_info;
-  (void)local_buffer;
+  (void)slice_info;  // Lagrange-form gate: no Coef-form constants to place
+  (void)local_buffer;  // no spills
   IMP_REQUIRE(vars && mut_vars, "null pointer");
-  // The generated kernels receive device arrays of PolyPtr (fused_kernels.rs:
-  // 172-180); the views are read back to the host once to build kernel args.
-  ConstPolyPtr in[6];
-  PolyPtr out;
-  IMP_CHECK(hipMemcpyAsync(in, vars, sizeof in, hipMemcpyDeviceToHost, stream));
-  IMP_CHECK(hipMemcpyAsync(&out, mut_vars, sizeof out, hipMemcpyDeviceToHost, stream));
-  IMP_CHECK(hipStreamSynchronize(stream));
-  GateViews g;
-  for (int k = 0; k < 6; k++) {
-    PolyPtr p = {const_cast<u32*>(in[k].ptr), in[k].len, in[k].rotate, in[k].offset,
-                 in[k].whole_len};
-    IMP_REQUIRE(p.ptr && p.len >= len, "gate input view");
-    IMP_REQUIRE(make_view(p, &g.in[k]), "gate input view");
-  }
-  IMP_REQUIRE(out.ptr && out.len >= len, "gate output view");
-  IMP_REQUIRE(make_view(out, &g.out), "gate output view");
   if (len == 0) return hipSuccess;
-  int dev = 0, cus = 256;
-  IMP_CHECK(hipGetDevice(&dev));
-  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  u64 blocks = (len + 255) / 256;
-  u64 cap = (u64)cus * 8;
-  if (blocks > cap) blocks = cap;
+  u32 blocks = 0;
+  IMP_CHECK(gate_grid(len, &blocks));
   {
     TimedScope ts(K_GATE, stream);
-    hipLaunchKernelGGL(k_gate, dim3((u32)blocks), dim3(256), 0, stream, g, (u64)len);
+    hipLaunchKernelGGL(k_gate_dev, dim3(blocks), dim3(256), 0, stream, vars, mut_vars, (u64)len);
   }
   return hipGetLastError();
 }
@@ -224,15 +243,11 @@ extern "C" hipError_t impeller_gate(const uint32_t* const* inputs, uint32_t* out
   }
   g.out = View{out, 0, len ? len : 1};
   if (len == 0) return hipSuccess;
-  int dev = 0, cus = 256;
-  IMP_CHECK(hipGetDevice(&dev));
-  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  u64 blocks = (len + 255) / 256;
-  u64 cap = (u64)cus * 8;
-  if (blocks > cap) blocks = cap;
+  u32 blocks = 0;
+  IMP_CHECK(gate_grid(len, &blocks));
   {
     TimedScope ts(K_GATE, stream);
-    hipLaunchKernelGGL(k_gate, dim3((u32)blocks), dim3(256), 0, stream, g, (u64)len);
+    hipLaunchKernelGGL(k_gate, dim3(blocks), dim3(256), 0, stream, g, (u64)len);
   }
   return hipGetLastError();
 }
@@ -297,7 +312,7 @@ extern "C" hipError_t impeller_graph_end(hipStream_t stream, void** graph_exec) 
   IMP_CHECK(hipStreamEndCapture(stream, &g));
   hipGraphExec_t e = nullptr;
   hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
-  hipGraphDestroy(g);
+  (void)hipGraphDestroy(g);
   IMP_CHECK(r);
   *graph_exec = (void*)e;
   return hipSuccess;

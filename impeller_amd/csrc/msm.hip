@@ -28,6 +28,8 @@
 //   k_msm_reduce      running sums over segments of 4 buckets
 //   k_msm_sum         segment sums split by segment-index bit
 //   k_msm_final       doubling tree per window -> Horner over windows -> affine
+#include <string.h>
+
 #include <algorithm>
 #include <mutex>
 #include <vector>
@@ -1118,11 +1120,20 @@ static void plan_stages(const std::vector<MsmShape>& shapes, std::vector<StageLa
   }
 }
 
-// timer of each stage kind (scans count with the sort)
+// timer of each stage kind; a scan is booked with the stage it serves (the
+// radix histograms with the sort, the task offsets with the bucket bounds,
+// the combine levels with the combine)
 static const u32 kTimerOf[] = {K_MSM_DIGITS,  K_MSM_SORT,   K_MSM_SORT, K_MSM_SORT, K_MSM_SORT,
                                K_MSM_SORT,    K_MSM_SCAN,   K_MSM_SCAN, K_MSM_BUCKETS,
                                K_MSM_COMBINE, K_MSM_REDUCE, K_MSM_SUM,  K_MSM_FINAL,
                                K_MSM_DIGITS};
+static u32 stage_timer(const StageLaunch& st) {
+  if (st.kind == MS_SCAN_A || st.kind == MS_SCAN_B || st.kind == MS_SCAN_C) {
+    const u32 sk = st.arg & 255;
+    return sk == SC_HIST ? K_MSM_SORT : (sk == SC_TASKS ? K_MSM_SCAN : K_MSM_COMBINE);
+  }
+  return kTimerOf[st.kind];
+}
 
 static hipError_t launch_stages(const MsmJob* jobs, u32 njobs, const u32* prefix_dev,
                                 const StageLaunch* st, u32 nst, u32 max_blocks,
@@ -1132,7 +1143,7 @@ static hipError_t launch_stages(const MsmJob* jobs, u32 njobs, const u32* prefix
     if (total == 0) continue;
     u32 grid = max_blocks && total > max_blocks ? max_blocks : total;
     const u32* pf = prefix_dev + (size_t)k * (njobs + 1);
-    TimedScope ts(kTimerOf[st[k].kind], stream);
+    TimedScope ts(stage_timer(st[k]), stream);
     switch (st[k].kind) {
       case MS_CONVERT: hipLaunchKernelGGL(kb_convert, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
       case MS_DIGITS: hipLaunchKernelGGL(kb_digits, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
@@ -1192,51 +1203,260 @@ static hipError_t precompute_device(u32* tables, u64 N, const MsmShape& s, hipSt
   return hipGetLastError();
 }
 
-// Host-memory contract of the reference (msm.cu:9-41): copy, run, copy back.
+// ---------------------------------------------------- host-memory contract
+// The reference's msm (msm.cu:9-41, msm_impl.cuh:373-580,703-756): points
+// and scalars in host memory, results back to host memory.  The points are
+// split into `parts` ranges of part_len = ceil(len / parts) points; each
+// part's tables go through one of `stage_points` device buffers and each
+// (part, batch) scalar range through one of `stage_scalers` buffers, copied
+// on a copy stream while the compute stream runs the MSM of the previous
+// range (events order the reuse of every buffer).  Batches run
+// `batch_per_run` at a time; each (part, batch) MSM leaves an affine partial
+// sum and a last kernel adds the parts of each batch.  Runs alternate the
+// order of the parts, so the part resident at the end of one run is not
+// copied again at the start of the next.  Everything runs on the first card
+// (this build drives one GPU; the other cards' buffer sizes are 0).
+
+// sum of count affine points per output (a batch's part partials) -> affine
+__global__ void k_sum_parts(const u32* partials, u32 count, u32 nout, u32* out) {
+  const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nout) return;
+  G1Xyzz acc = G1Xyzz::identity();
+  for (u32 p = 0; p < count; p++) {
+    const G1Aff a = load_aff(partials + 16 * ((u64)j * count + p));
+    if (!a.is_identity()) acc = xyzz_madd(acc, a);
+  }
+  store_aff(out + 16 * j, xyzz_to_aff(acc, true));
+}
+
+struct HostPlan {
+  MsmShape s;      // shape of a full part (part_len points)
+  u64 len, part_len;
+  u32 parts, bpr, sp, ss;
+  size_t pts_stage, scl_stage, ws_body, op_desc, descs, partials, finals, total;
+  size_t off_pts, off_scl, off_ws, off_desc, off_part, off_fin;
+};
+
+static size_t op_desc_bytes() {
+  return align256(sizeof(MsmJob)) + align256(kMaxStages * 2 * 4);
+}
+
+static HostPlan host_plan(u32 window, u32 target, u64 len, u32 bpr, u32 parts, u32 ss, u32 sp) {
+  HostPlan h;
+  h.len = len;
+  h.parts = parts ? parts : 1;
+  if ((u64)h.parts > len && len > 0) h.parts = (u32)len;
+  h.part_len = len ? (len + h.parts - 1) / h.parts : 0;
+  h.bpr = bpr ? bpr : 1;
+  h.sp = sp ? sp : 1;
+  if (h.sp > h.parts) h.sp = h.parts;
+  h.ss = ss ? ss : 1;
+  h.s = msm_shape(h.part_len, window, target);
+  h.pts_stage = align256((size_t)h.s.n_pre * h.part_len * 64);
+  h.scl_stage = align256((size_t)h.part_len * 32);
+  h.ws_body = align256(msm_layout(h.s).total);
+  h.op_desc = op_desc_bytes();
+  h.descs = (size_t)h.parts * h.bpr * h.op_desc;
+  h.partials = align256((size_t)h.bpr * h.parts * 64);
+  h.finals = align256((size_t)h.bpr * 64);
+  size_t o = 0;
+  h.off_pts = o; o += h.sp * h.pts_stage;
+  h.off_scl = o; o += h.ss * h.scl_stage;
+  h.off_ws = o; o += h.ws_body;
+  h.off_desc = o; o += h.descs;
+  h.off_part = o; o += h.partials;
+  h.off_fin = o; o += h.finals;
+  h.total = o;
+  return h;
+}
+
+// Owns the streams and events of one call and puts the caller's device back
+// on every exit path.
+struct HostCallGuard {
+  int prev = -1;
+  hipStream_t st = nullptr, cp = nullptr;
+  std::vector<hipEvent_t> evs;
+  ~HostCallGuard() {
+    if (st) (void)hipStreamSynchronize(st);
+    if (cp) (void)hipStreamSynchronize(cp);
+    for (hipEvent_t e : evs) (void)hipEventDestroy(e);
+    if (cp) (void)hipStreamDestroy(cp);
+    if (st) (void)hipStreamDestroy(st);
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+  hipError_t event(hipEvent_t* e) {
+    IMP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    evs.push_back(*e);
+    return hipSuccess;
+  }
+};
+
 static hipError_t msm_host(u32 window, u32 target, void* const* buffers,
-                           unsigned long* buffer_sizes, unsigned long long len,
+                           unsigned long* buffer_sizes, unsigned long long len, u32 batch_per_run,
+                           u32 parts, u32 stage_scalers, u32 stage_points,
                            unsigned int num_cards, const unsigned int* cards,
                            unsigned int const* const* h_points, unsigned int batches,
                            unsigned int const* const* h_scaler_batch,
                            unsigned int* const* h_result) {
   IMP_REQUIRE(valid_window(window), "window");
   IMP_REQUIRE(num_cards >= 1 && cards, "cards");
-  MsmShape s = msm_shape(len, window, target);
-  // per card: points (n_pre tables), scalars, results, MSM workspace
-  size_t pts = align256((size_t)s.n_pre * len * 64), scl = align256((size_t)len * 32),
-         res = align256(64);
-  size_t need = pts + scl + res + msm_ws_total(s);
+  const HostPlan h = host_plan(window, target, len, batch_per_run, parts, stage_scalers,
+                               stage_points);
+  IMP_REQUIRE((u64)h.s.n_pre * h.part_len < (1ull << 31), "point index range (n_pre * part)");
   if (buffers == nullptr) {
     IMP_REQUIRE(buffer_sizes, "buffer_sizes");
-    // the whole MSM runs on the first card; the others need nothing
-    for (u32 i = 0; i < num_cards; i++) buffer_sizes[i] = i == 0 ? need : 0;
+    for (u32 i = 0; i < num_cards; i++) buffer_sizes[i] = i == 0 ? h.total : 0;
     return hipSuccess;
   }
-  IMP_REQUIRE(h_points && (batches == 0 || (h_scaler_batch && h_result)), "null pointer");
-  int prev = 0;
-  IMP_CHECK(hipGetDevice(&prev));
-  IMP_CHECK(hipSetDevice((int)cards[0]));
-  char* buf = (char*)buffers[0];
-  u32* d_pts = (u32*)buf;
-  u32* d_scl = (u32*)(buf + pts);
-  u32* d_res = (u32*)(buf + pts + scl);
-  char* ws = buf + pts + scl + res;
-  hipStream_t st;
-  IMP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-  for (u32 t = 0; t < s.n_pre; t++)
-    IMP_CHECK(hipMemcpyAsync(d_pts + (size_t)16 * t * len, h_points[t], (size_t)len * 64,
-                             hipMemcpyHostToDevice, st));
-  if (s.n_pre > 1) IMP_CHECK(to_table(d_pts, d_pts, (u64)s.n_pre * len, st));
-  for (u32 b = 0; b < batches; b++) {
-    IMP_CHECK(hipMemcpyAsync(d_scl, h_scaler_batch[b], (size_t)len * 32, hipMemcpyHostToDevice,
-                             st));
-    IMP_CHECK(msm_run(d_pts, len, d_scl, s, d_res, ws, st));
-    IMP_CHECK(hipMemcpyAsync(h_result[b], d_res, 64, hipMemcpyDeviceToHost, st));
-    IMP_CHECK(hipStreamSynchronize(st));
+  IMP_REQUIRE(buffers[0] && h_points && (batches == 0 || (h_scaler_batch && h_result)),
+              "null pointer");
+  if (batches == 0) return hipSuccess;
+  if (len == 0) {
+    for (u32 b = 0; b < batches; b++) memset(h_result[b], 0, 64);
+    return hipSuccess;
   }
-  IMP_CHECK(hipStreamSynchronize(st));
-  IMP_CHECK(hipStreamDestroy(st));
-  IMP_CHECK(hipSetDevice(prev));
+  for (u32 t = 0; t < h.s.n_pre; t++) IMP_REQUIRE(h_points[t], "point table");
+  HostCallGuard g;
+  IMP_CHECK(hipGetDevice(&g.prev));
+  IMP_CHECK(hipSetDevice((int)cards[0]));
+  IMP_CHECK(hipStreamCreateWithFlags(&g.st, hipStreamNonBlocking));
+  IMP_CHECK(hipStreamCreateWithFlags(&g.cp, hipStreamNonBlocking));
+  char* buf = (char*)buffers[0];
+  std::vector<hipEvent_t> pt_ready(h.sp), pt_free(h.sp), sc_ready(h.ss), sc_free(h.ss);
+  for (u32 k = 0; k < h.sp; k++) {
+    IMP_CHECK(g.event(&pt_ready[k]));
+    IMP_CHECK(g.event(&pt_free[k]));
+    IMP_CHECK(hipEventRecord(pt_free[k], g.st));
+  }
+  for (u32 k = 0; k < h.ss; k++) {
+    IMP_CHECK(g.event(&sc_ready[k]));
+    IMP_CHECK(g.event(&sc_free[k]));
+    IMP_CHECK(hipEventRecord(sc_free[k], g.st));
+  }
+  std::vector<long long> resident(h.sp, -1);  // part held by each point stage
+  u32 next_pt = 0, next_sc = 0;
+  const u32 runs = (batches + h.bpr - 1) / h.bpr;
+  std::vector<MsmJob> jobs;
+  std::vector<StageLaunch> stages_full, stages_last;
+  std::vector<u32> pref_full, pref_last;
+  {
+    std::vector<MsmShape> one(1, h.s);
+    plan_stages(one, stages_full, pref_full);
+    const u64 last_len = h.len - (u64)(h.parts - 1) * h.part_len;
+    one[0] = msm_shape(last_len, window, target);
+    plan_stages(one, stages_last, pref_last);
+    IMP_REQUIRE(pref_full.size() <= kMaxStages * 2 && pref_last.size() <= kMaxStages * 2,
+                "stage plan");
+  }
+  for (u32 r = 0; r < runs; r++) {
+    const u32 b0 = r * h.bpr, nb = batches - b0 < h.bpr ? batches - b0 : h.bpr;
+    const bool fwd = (r & 1) == 0;
+    // the run's operations: one MSM per (part, batch), descriptors uploaded
+    // once (the previous run is complete: its results were read back)
+    struct Op {
+      u32 part, batch, pstage;
+    };
+    std::vector<Op> ops;
+    std::vector<u32> part_stage(h.parts);
+    {
+      std::vector<long long> res = resident;
+      u32 np = next_pt;
+      for (u32 q = 0; q < h.parts; q++) {
+        const u32 p = fwd ? q : h.parts - 1 - q;
+        u32 k = h.sp;
+        for (u32 i = 0; i < h.sp; i++)
+          if (res[i] == (long long)p) k = i;
+        if (k == h.sp) {
+          k = np;
+          np = (np + 1) % h.sp;
+          res[k] = p;
+        }
+        part_stage[p] = k;
+        for (u32 j = 0; j < nb; j++) ops.push_back(Op{p, j, k});
+      }
+    }
+    jobs.assign(ops.size(), MsmJob());
+    char* desc = buf + h.off_desc;
+    for (size_t o = 0; o < ops.size(); o++) {
+      const Op& op = ops[o];
+      const u64 cur = op.part + 1 == h.parts ? h.len - (u64)op.part * h.part_len : h.part_len;
+      const MsmShape s = msm_shape(cur, window, target);
+      u32* result = h.parts == 1 ? (u32*)(buf + h.off_fin) + 16 * op.batch
+                                 : (u32*)(buf + h.off_part) + 16 * ((u64)op.batch * h.parts + op.part);
+      // scalar stage pointers are patched below (the stage is picked when
+      // the copy is issued, in the same order)
+      jobs[o] = make_msm_job(s, h.part_len, (const u32*)(buf + h.off_pts + op.pstage * h.pts_stage),
+                             nullptr, result, buf + h.off_ws);
+    }
+    {
+      u32 ns = next_sc;
+      for (size_t o = 0; o < ops.size(); o++) {
+        jobs[o].scalars = (const u32*)(buf + h.off_scl + ns * h.scl_stage);
+        ns = (ns + 1) % h.ss;
+      }
+    }
+    for (size_t o = 0; o < ops.size(); o++) {
+      char* d = desc + o * h.op_desc;
+      IMP_CHECK(hipMemcpyAsync(d, &jobs[o], sizeof(MsmJob), hipMemcpyHostToDevice, g.st));
+      const std::vector<u32>& pf = ops[o].part + 1 == h.parts ? pref_last : pref_full;
+      IMP_CHECK(hipMemcpyAsync(d + align256(sizeof(MsmJob)), pf.data(), pf.size() * 4,
+                               hipMemcpyHostToDevice, g.st));
+    }
+    IMP_CHECK(hipStreamSynchronize(g.st));
+    // issue copies and MSMs in operation order
+    u32 prev_part = ~0u;
+    for (size_t o = 0; o < ops.size(); o++) {
+      const Op& op = ops[o];
+      const u32 k = op.pstage;
+      const u64 off = (u64)op.part * h.part_len;
+      const u64 cur = op.part + 1 == h.parts ? h.len - off : h.part_len;
+      if (op.part != prev_part) {
+        if (resident[k] != (long long)op.part) {
+          // the stage's previous part is done once its last MSM has run
+          IMP_CHECK(hipStreamWaitEvent(g.cp, pt_free[k], 0));
+          u32* dst = (u32*)(buf + h.off_pts + k * h.pts_stage);
+          for (u32 t = 0; t < h.s.n_pre; t++)
+            IMP_CHECK(hipMemcpyAsync(dst + (size_t)16 * t * h.part_len, h_points[t] + 16 * off,
+                                     (size_t)cur * 64, hipMemcpyHostToDevice, g.cp));
+          if (h.s.n_pre > 1) {
+            // table format, in place; rows past a short last part unused
+            for (u32 t = 0; t < h.s.n_pre; t++)
+              IMP_CHECK(to_table(dst + (size_t)16 * t * h.part_len,
+                                 dst + (size_t)16 * t * h.part_len, cur, g.cp));
+          }
+          IMP_CHECK(hipEventRecord(pt_ready[k], g.cp));
+          resident[k] = op.part;
+          next_pt = (k + 1) % h.sp;
+        }
+        IMP_CHECK(hipStreamWaitEvent(g.st, pt_ready[k], 0));
+        prev_part = op.part;
+      }
+      const u32 sk = next_sc;
+      next_sc = (next_sc + 1) % h.ss;
+      IMP_CHECK(hipStreamWaitEvent(g.cp, sc_free[sk], 0));
+      IMP_CHECK(hipMemcpyAsync(buf + h.off_scl + sk * h.scl_stage,
+                               h_scaler_batch[b0 + op.batch] + 8 * off, (size_t)cur * 32,
+                               hipMemcpyHostToDevice, g.cp));
+      IMP_CHECK(hipEventRecord(sc_ready[sk], g.cp));
+      IMP_CHECK(hipStreamWaitEvent(g.st, sc_ready[sk], 0));
+      char* d = desc + o * h.op_desc;
+      const std::vector<StageLaunch>& stl = op.part + 1 == h.parts ? stages_last : stages_full;
+      IMP_CHECK(launch_stages((const MsmJob*)d, 1, (const u32*)(d + align256(sizeof(MsmJob))),
+                              stl.data(), (u32)stl.size(), 0, g.st));
+      IMP_CHECK(hipEventRecord(sc_free[sk], g.st));
+      const bool last_of_part = o + 1 == ops.size() || ops[o + 1].part != op.part;
+      if (last_of_part) IMP_CHECK(hipEventRecord(pt_free[k], g.st));
+    }
+    u32* fin = (u32*)(buf + h.off_fin);
+    if (h.parts > 1) {
+      hipLaunchKernelGGL(k_sum_parts, dim3((nb + 63) / 64), dim3(64), 0, g.st,
+                         (const u32*)(buf + h.off_part), h.parts, nb, fin);
+      IMP_CHECK(hipGetLastError());
+    }
+    for (u32 j = 0; j < nb; j++)
+      IMP_CHECK(hipMemcpyAsync(h_result[b0 + j], fin + 16 * j, 64, hipMemcpyDeviceToHost, g.st));
+    IMP_CHECK(hipStreamSynchronize(g.st));
+  }
   return hipSuccess;
 }
 
@@ -1268,13 +1488,15 @@ using namespace imp;
 
 extern "C" hipError_t msm_cfg(unsigned int window, unsigned int target_window,
                               void* const* buffers, unsigned long* buffer_sizes,
-                              unsigned long long len, unsigned int, unsigned int, unsigned int,
-                              unsigned int, unsigned int num_cards, const unsigned int* cards,
-                              unsigned int const* const* h_points, unsigned int batches,
-                              unsigned int const* const* h_scaler_batch,
+                              unsigned long long len, unsigned int batch_per_run,
+                              unsigned int parts, unsigned int stage_scalers,
+                              unsigned int stage_points, unsigned int num_cards,
+                              const unsigned int* cards, unsigned int const* const* h_points,
+                              unsigned int batches, unsigned int const* const* h_scaler_batch,
                               unsigned int* const* h_result) {
-  return msm_host(window, target_window, buffers, buffer_sizes, len, num_cards, cards, h_points,
-                  batches, h_scaler_batch, h_result);
+  return msm_host(window, target_window, buffers, buffer_sizes, len, batch_per_run, parts,
+                  stage_scalers, stage_points, num_cards, cards, h_points, batches,
+                  h_scaler_batch, h_result);
 }
 
 extern "C" hipError_t msm(void* const* buffers, unsigned long* buffer_sizes,
@@ -1312,6 +1534,7 @@ extern "C" hipError_t impeller_msm(const uint32_t* bases_d, const uint32_t* scal
   IMP_REQUIRE(valid_window(window), "window");
   IMP_REQUIRE(result_d && workspace && (len == 0 || (bases_d && scalars_d)), "null pointer");
   MsmShape s = msm_shape(len, window, 0);
+  IMP_REQUIRE((u64)s.n_pre * len < (1ull << 31), "point index range (n_pre * len)");
   IMP_REQUIRE(workspace_bytes >= msm_ws_total(s), "workspace too small");
   return msm_run(bases_d, len, scalars_d, s, result_d, (char*)workspace, stream);
 }

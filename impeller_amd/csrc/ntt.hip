@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "common.h"
+#include "host_field.h"
 #include "timing.h"
 
 namespace imp {
@@ -92,27 +93,38 @@ static hipError_t launch_factor_tables(const Factor& f, const u32* base_d, const
 }
 
 // --------------------------------------------------------------- twiddles
+// Direct mode: tab[e] = omega^e for e < n/2 (the caller's table).
+// Recompute mode (recompute_ntt, for callers without room for that table):
+// pq[i] = omega^(i 2^(L - pq_deg)) for i < 2^(pq_deg - 1) and omegas[k] =
+// omega^(2^k); every LDS-pass work-group derives the twiddles of its own tile
+// from them into LDS before its butterflies (rc_prologue), so the transform
+// needs no memory beyond pq and omegas.  Field products are exact, so these
+// twiddles equal the table's entries bit for bit.
 struct Twiddle {
-  const u32* tab;  // omega^e for e < n/2 (direct mode)
-  const u32* lo;   // two-level mode: omega^e = lo[e & mask] * hi[e >> h]
-  const u32* hi;
-  u32 h;
+  const u32* tab;
+  const u32* pq;      // recompute mode; nullptr: everything from omegas
+  const u32* omegas;  // recompute mode
+  u32 pq_deg;
 };
-
-template <bool TWO>
-__device__ __forceinline__ Fr twiddle_at(const Twiddle& tw, u64 e) {
-  if (!TWO) return load_fe<Fr>(tw.tab + 8 * e);
-  u64 mask = (1ull << tw.h) - 1;
-  Fr a = load_fe<Fr>(tw.lo + 8 * (e & mask));
-  u64 eh = e >> tw.h;
-  if (eh == 0) return a;
-  return mul<FrParams>(a, load_fe<Fr>(tw.hi + 8 * eh));
-}
 
 // ------------------------------------------------------ butterfly helpers
 // radix-2: (a, b) <- (a + w b, a - w b); e == 0 means w = 1
-__device__ __forceinline__ Fr tw_load(const Twiddle& tw, bool two, u64 e) {
-  return two ? twiddle_at<true>(tw, e) : twiddle_at<false>(tw, e);
+__device__ __forceinline__ Fr tw_load(const Twiddle& tw, u64 e) {
+  return load_fe<Fr>(tw.tab + 8 * e);
+}
+
+// omega^e as the product of omegas[k] over the set bits k of e (e < 2^32)
+__device__ __forceinline__ Fr pow_omegas(const u32* omegas, u64 e) {
+  Fr acc = Fr::one();
+  bool first = true;
+  for (u32 k = 0; e; k++, e >>= 1) {
+    if (e & 1) {
+      const Fr w = load_fe<Fr>(omegas + 8 * k);
+      acc = first ? w : mul<FrParams>(acc, w);
+      first = false;
+    }
+  }
+  return acc;
 }
 
 // Lazy butterflies of the LDS passes (Harvey, "Faster arithmetic for
@@ -155,8 +167,8 @@ __device__ __forceinline__ void bfly2_lazy(Fr& a, Fr& b, bool m, const Fr& w) {
 // [0, 4m) -> [0, m)
 __device__ __forceinline__ Fr canon4m(const Fr& a) { return reduce_once<FrParams>(fold2m(a)); }
 
-__device__ __forceinline__ void bfly2_v(Fr& a, Fr& b, u64 e, const Twiddle& tw, bool two) {
-  if (e != 0) b = mul<FrParams>(b, tw_load(tw, two, e));
+__device__ __forceinline__ void bfly2_v(Fr& a, Fr& b, u64 e, const Twiddle& tw) {
+  if (e != 0) b = mul<FrParams>(b, tw_load(tw, e));
   Fr s = add<FrParams>(a, b);
   b = sub<FrParams>(a, b);
   a = s;
@@ -167,16 +179,16 @@ __device__ __forceinline__ void bfly2_v(Fr& a, Fr& b, u64 e, const Twiddle& tw, 
 // pairs (g0,g2) with e2 and (g1,g3) with e3.  The same butterflies, in the
 // same order per element, as two radix-2 levels.
 __device__ __forceinline__ void bfly4_v(Fr& a0, Fr& a1, Fr& a2, Fr& a3, u64 e1, u64 e2, u64 e3,
-                                        const Twiddle& tw, bool two) {
+                                        const Twiddle& tw) {
   if (e1 != 0) {
-    Fr w1 = tw_load(tw, two, e1);
+    Fr w1 = tw_load(tw, e1);
     a1 = mul<FrParams>(a1, w1);
     a3 = mul<FrParams>(a3, w1);
   }
   Fr b0 = add<FrParams>(a0, a1), b1 = sub<FrParams>(a0, a1);
   Fr b2 = add<FrParams>(a2, a3), b3 = sub<FrParams>(a2, a3);
-  if (e2 != 0) b2 = mul<FrParams>(b2, tw_load(tw, two, e2));
-  b3 = mul<FrParams>(b3, tw_load(tw, two, e3));
+  if (e2 != 0) b2 = mul<FrParams>(b2, tw_load(tw, e2));
+  b3 = mul<FrParams>(b3, tw_load(tw, e3));
   a0 = add<FrParams>(b0, b2);
   a2 = sub<FrParams>(b0, b2);
   a1 = add<FrParams>(b1, b3);
@@ -225,7 +237,7 @@ struct NttJob {
   u32 fuse;  // DIT levels fused into the bit reversal: 0, or tb (L >= 2 tb)
   u32 tb;    // bit-reversal tile side 2^tb (4: 16 x 16, two tile pairs per
              // 256-thread work-group; 5: 32 x 32, one pair)
-  u32 two;   // two-level twiddles
+  u32 rc;    // recompute-mode twiddles (then fuse == 0 and no fused reversal)
   Twiddle tw;
   Factor pro, epi;  // tab == nullptr: none
   u32 nst;
@@ -273,7 +285,6 @@ __device__ __forceinline__ void br_tile_pair(const NttJob& J, u64 b, bool valid,
   }
   __syncthreads();
   if (J.fuse == TB) {
-    const bool two = J.two != 0;
     const u32 nt = self ? 1 : 2;
     for (u32 j = 0; j + 1 < TB; j += 2) {
       if (active) {
@@ -287,7 +298,7 @@ __device__ __forceinline__ void br_tile_pair(const NttJob& J, u64 b, bool valid,
           const u32 s2 = rs + ((c0 | (2u << j)) ^ row), s3 = rs + ((c0 | (3u << j)) ^ row);
           Fr a0 = pl_load(T, s0), a1 = pl_load(T, s1), a2 = pl_load(T, s2), a3 = pl_load(T, s3);
           bfly4_v(a0, a1, a2, a3, (u64)glo << (L - j - 1), (u64)glo << (L - j - 2),
-                  ((u64)glo + (1u << j)) << (L - j - 2), J.tw, two);
+                  ((u64)glo + (1u << j)) << (L - j - 2), J.tw);
           pl_store(T, s0, a0);
           pl_store(T, s1, a1);
           pl_store(T, s2, a2);
@@ -305,7 +316,7 @@ __device__ __forceinline__ void br_tile_pair(const NttJob& J, u64 b, bool valid,
           const u32 rs = tl * TS + row * NT;
           const u32 s0 = rs + (k ^ row), s1 = rs + ((k | (1u << j)) ^ row);
           Fr a0 = pl_load(T, s0), a1 = pl_load(T, s1);
-          bfly2_v(a0, a1, (u64)k << (L - TB), J.tw, two);
+          bfly2_v(a0, a1, (u64)k << (L - TB), J.tw);
           pl_store(T, s0, a0);
           pl_store(T, s1, a1);
         }
@@ -362,8 +373,9 @@ __device__ __forceinline__ u32 dslot(u32 s) { return s ^ (((s >> 4) ^ (s >> 8)) 
 // contiguous (s == 0): work-group = C consecutive groups of G positions.
 // strided (2^s >= C): work-group = fixed hi, C consecutive lo.  The last pass
 // multiplies each output by the epilogue factor (n^-1, coset un-shift).
-// TWO: two-level twiddle table (J.two).  Each butterfly issues its twiddle
-// loads before its LDS reads, so one wait covers both.
+// RC: recompute-mode twiddles (J.rc), derived per work-group into LDS past
+// the tile.  Each butterfly issues its twiddle loads before its LDS reads, so
+// one wait covers both.
 //
 // With the bit reversal fused (J.fbr), stage 0 takes for work-group m the
 // source positions q = (g' << (L-D)) | (m << logC) | c (runs of C elements),
@@ -371,7 +383,37 @@ __device__ __forceinline__ u32 dslot(u32 s) { return s ^ (((s >> 4) ^ (s >> 8)) 
 // (rev m << D) | rev g': tile row g = rev g' of column c, the same tile as a
 // contiguous pass over those columns.  It writes p to J.tmp (runs of G
 // elements); stage 1 reads J.tmp and writes x.
-template <bool TWO>
+// Recompute-mode twiddles of one work-group (levels s .. s+D-1, columns lo =
+// lo0 + c; lo = 0 in a contiguous pass).  The twiddle of level s + j, group
+// offset glo < 2^j, column c is omega^e with e = (glo 2^s + lo) 2^(L-s-j-1)
+// = glo 2^(L-j-1) + lo 2^(L-s-j-1), the product of
+//   P[glo << (D-1-j)],  P[k] = omega^(k 2^(L-D)) for k < 2^(D-1)
+//                       (= pq[k << (pq_deg - D)] when D <= pq_deg <= L), and
+//   Q[c D + j] = omega^(lo 2^(L-s-j-1)) (1 for lo = 0),
+// both left in LDS at P = tile end, Q = P + 8 2^(D-1).
+__device__ __forceinline__ void rc_prologue(const NttJob& J, u32 L, u32 s, u32 D, u32 C,
+                                            bool contig, u64 lo0, u32* P) {
+  const u32 np = 1u << (D - 1);
+  const bool use_pq = J.tw.pq != nullptr && J.tw.pq_deg >= D && J.tw.pq_deg <= L;
+  for (u32 k = threadIdx.x; k < np; k += blockDim.x) {
+    const Fr v = use_pq ? load_fe<Fr>(J.tw.pq + 8 * ((u64)k << (J.tw.pq_deg - D)))
+                        : pow_omegas(J.tw.omegas, (u64)k << (L - D));
+    store_fe<Fr>(P + 8 * k, v);
+  }
+  u32* Q = P + 8 * np;
+  for (u32 q = threadIdx.x; q < C * D; q += blockDim.x) {
+    const u32 c = q / D, j = q % D;
+    const u64 lo = contig ? 0 : lo0 + c;
+    store_fe<Fr>(Q + 8 * q, pow_omegas(J.tw.omegas, lo << (L - s - j - 1)));
+  }
+}
+
+__device__ __forceinline__ Fr rc_twiddle(const u32* P, const u32* Q, u32 k, u32 q, bool qm) {
+  const Fr w = load_fe<Fr>(P + 8 * k);
+  return qm ? mul<FrParams>(w, load_fe<Fr>(Q + 8 * q)) : w;
+}
+
+template <bool RC>
 __device__ __forceinline__ void dit_body(const NttJob& J, u32 stage, u64 wg, u32* t) {
   const NttStage& S = J.st[stage];
   const bool last = stage + 1 == J.nst;
@@ -396,7 +438,10 @@ __device__ __forceinline__ void dit_body(const NttJob& J, u32 stage, u64 wg, u32
     }
   }
   const Planar T{t, GC};
+  u32* const P = t + 8 * GC;  // recompute-mode twiddles past the tile
+  const u32* const Q = P + 8 * (1u << (D - 1));
   __syncthreads();
+  if (RC) rc_prologue(J, L, s, D, C, contig, lo0, P);
   // zero-extended source (zlev > 0): a row g = rev gq is nonzero only when
   // its low zlev bits are clear (gq < 2^(D - zlev)), and DIT levels
   // 0 .. zlev-1 with a zero partner (a, 0) -> (a, a) only copy it to the rows
@@ -441,8 +486,17 @@ __device__ __forceinline__ void dit_body(const NttJob& J, u32 stage, u64 wg, u32
       u64 e1 = base << (sh1 + 1), e2 = base << sh1, e3 = (base + (1ull << l)) << sh1;
       const u32 s0 = dslot(g0 * C + c), s1 = dslot((g0 | (1u << j)) * C + c);
       const u32 s2 = dslot((g0 | (2u << j)) * C + c), s3 = dslot((g0 | (3u << j)) * C + c);
-      const Fr w1 = twiddle_at<TWO>(J.tw, e1), w2 = twiddle_at<TWO>(J.tw, e2);
-      const Fr w3 = twiddle_at<TWO>(J.tw, e3);
+      Fr w1, w2, w3;
+      if (RC) {
+        const bool qm = !contig && lo != 0;
+        w1 = rc_twiddle(P, Q, glo << (D - 1 - j), c * D + j, qm);
+        w2 = rc_twiddle(P, Q, glo << (D - 2 - j), c * D + j + 1, qm);
+        w3 = rc_twiddle(P, Q, (glo + (1u << j)) << (D - 2 - j), c * D + j + 1, qm);
+      } else {
+        w1 = tw_load(J.tw, e1);
+        w2 = tw_load(J.tw, e2);
+        w3 = tw_load(J.tw, e3);
+      }
       Fr a0 = pl_load(T, s0), a1 = pl_load(T, s1), a2 = pl_load(T, s2), a3 = pl_load(T, s3);
       // two levels: (a0, a1), (a2, a3) with w1, then (a0, a2) with w2 and
       // (a1, a3) with w3 -- best_fft's butterflies, in its order per element
@@ -467,7 +521,8 @@ __device__ __forceinline__ void dit_body(const NttJob& J, u32 stage, u64 wg, u32
       u64 lo = contig ? 0 : lo0 + c;
       u64 e = ((((u64)glo) << s) + lo) << sh;
       const u32 s0 = dslot(g0 * C + c), s1 = dslot(g1 * C + c);
-      const Fr w = twiddle_at<TWO>(J.tw, e);
+      const Fr w = RC ? rc_twiddle(P, Q, glo << (D - 1 - j), c * D + j, !contig && lo != 0)
+                      : tw_load(J.tw, e);
       Fr a0 = pl_load(T, s0), a1 = pl_load(T, s1);
       bfly2_lazy(a0, a1, e != 0, w);
       pl_store(T, s0, a0);
@@ -510,10 +565,10 @@ __global__ __launch_bounds__(256, 4) void k_ntt_br(NttJob J) {
   br_body(J, blockIdx.x, lds);
 }
 
-template <bool TWO>
+template <bool RC>
 __global__ __launch_bounds__(256, 4) void k_ntt_dit(NttJob J, u32 stage) {
   IMP_DYNAMIC_LDS(lds);
-  dit_body<TWO>(J, stage, blockIdx.x, lds);
+  dit_body<RC>(J, stage, blockIdx.x, lds);
 }
 
 __device__ __forceinline__ u32 find_job(const u32* prefix, u32 njobs, u32 t) {
@@ -545,9 +600,8 @@ __global__ __launch_bounds__(256, 4) void k_ntt_dit_batch(const NttJob* jobs, u3
   const u32 total = prefix[njobs];
   for (u32 t = blockIdx.x; t < total; t += gridDim.x) {
     u32 j = find_job(prefix, njobs, t);
-    const NttJob& J = jobs[j];
-    if (J.two) dit_body<true>(J, stage, t - prefix[j], lds);
-    else dit_body<false>(J, stage, t - prefix[j], lds);
+    // batch jobs always carry a twiddle table (never recompute mode)
+    dit_body<false>(jobs[j], stage, t - prefix[j], lds);
   }
 }
 
@@ -582,26 +636,6 @@ __global__ void k_powers(u32* out, u64 len, const u32* base_p, u32 chunk) {
   }
 }
 
-// omega^e for e < count from omegas[k] = omega^(2^k): product over set bits
-__global__ void k_pow_from_omegas(u32* out, u64 count, u64 stride_log, const u32* omegas) {
-  u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= count) return;
-  u64 e = j << stride_log;
-  Fr acc = Fr::one();
-  for (u32 k = 0; e && k < 32; k++, e >>= 1)
-    if (e & 1) acc = mul<FrParams>(acc, load_fe<Fr>(omegas + 8 * k));
-  store_fe<Fr>(out + 8 * j, acc);
-}
-
-// om[i] = om[i-1]^2 for i < count (single thread)
-__global__ void k_square_chain(u32* om, u32 count) {
-  Fr w = load_fe<Fr>(om);
-  for (u32 i = 1; i < count; i++) {
-    w = sqr<FrParams>(w);
-    store_fe<Fr>(om + 8 * i, w);
-  }
-}
-
 // n^-1 for n = 2^L: Fermat inversion of from_number(2^L)
 __global__ void k_ninv(u32* out, u32 L) {
   Fr n = Fr::zero();
@@ -623,10 +657,10 @@ struct PassPlan {
   u32 d[8];
 };
 
-// Default: from 2^20 up, the bit reversal fused into a first pass of 8
-// levels (runs of 4 elements) and the rest in passes of at most 9 levels
-// (measured best or within a few percent of it for 2^20 .. 2^26); below,
-// 4 levels in the bit-reversal kernel and passes of at most 8.
+// Default with a second buffer: from 2^20 up, the bit reversal fused into a
+// first pass of 8 levels (runs of 4 elements) and the rest in passes of at
+// most 9 levels (measured best or within a few percent of it for 2^20 ..
+// 2^26); below, 4 levels in the bit-reversal kernel and passes of at most 8.
 static void default_plan(u32 L, PassPlan* p) {
   p->lds_log = 10;
   if (L >= 20) {
@@ -643,6 +677,46 @@ static void default_plan(u32 L, PassPlan* p) {
   if (np == 0) np = 1;
   p->n = np;
   for (u32 i = 0; i < np; i++) p->d[i] = rest / np + (i < rest % np ? 1 : 0);
+}
+
+// Default in place (no second buffer: the reference-ABI ssip_ntt, and any
+// call without room for one).  The fastest in-place plans the tuner measured
+// on MI355X: 2^20 .. 2^23 four levels in the bit-reversal kernel and two or
+// three passes; from 2^24 a plain bit reversal, a first pass of 10 levels
+// (1024 consecutive elements per group) and passes of at most 8.
+static void default_inplace_plan(u32 L, PassPlan* p) {
+  static const u32 mid[4][4] = {{2, 8, 8, 0}, {2, 9, 8, 0}, {3, 6, 6, 6}, {3, 7, 6, 6}};
+  if (L < 20) {
+    default_plan(L, p);
+    return;
+  }
+  p->lds_log = 10;
+  if (L <= 23) {
+    p->fuse = 4;
+    p->n = mid[L - 20][0];
+    for (u32 i = 0; i < p->n; i++) p->d[i] = mid[L - 20][1 + i];
+    return;
+  }
+  p->fuse = 0;
+  p->d[0] = 10;
+  u32 rest = L - 10, n = 1;
+  while (rest > 0 && n < 8) {
+    const u32 d = rest > 8 ? 8 : rest;
+    p->d[n++] = d;
+    rest -= d;
+  }
+  p->n = n;
+}
+
+// recompute_ntt: a plain bit reversal (its kernel never needs twiddles) and
+// passes of at most 8 levels, split evenly
+static void recompute_plan(u32 L, PassPlan* p) {
+  p->lds_log = 10;
+  p->fuse = 0;
+  u32 np = (L + 7) / 8;
+  if (np == 0) np = 1;
+  p->n = np;
+  for (u32 i = 0; i < np; i++) p->d[i] = L / np + (i < L % np ? 1 : 0);
 }
 
 // plan array: {fuse, lds_log, d0, d1, ...}; fuse + sum(d) == L
@@ -670,14 +744,18 @@ static bool plan_from(const unsigned int* plan, unsigned int n, u32 L, PassPlan*
 }
 
 static size_t stage_lds(const NttJob& J, u32 stage) {
-  return ((size_t)32 << J.st[stage].D) << J.st[stage].logC;
+  const NttStage& st = J.st[stage];
+  size_t b = ((size_t)32 << st.D) << st.logC;
+  // recompute mode: P (2^(D-1) twiddles) and Q (C x D column factors)
+  if (J.rc) b += (size_t)32 * ((1u << (st.D - 1)) + (st.D << st.logC));
+  return b;
 }
 static size_t br_lds(const NttJob& J) {
   if (J.L < 2 * J.tb) return (size_t)32 << J.L;
   return J.tb == 5 ? (size_t)2 * 1024 * 32 : (size_t)2 * 2 * 256 * 32;
 }
 
-static NttJob make_job(View x, View src, u64 src_len, u32 L, const Twiddle& tw, bool two,
+static NttJob make_job(View x, View src, u64 src_len, u32 L, const Twiddle& tw, bool rc,
                        Factor pro, Factor epi, const PassPlan& plan) {
   NttJob J;
   J.x = x;
@@ -689,7 +767,7 @@ static NttJob make_job(View x, View src, u64 src_len, u32 L, const Twiddle& tw, 
   J.fbr = plan.fuse == 1 ? 1 : 0;
   J.zlev = 0;
   J.tmp = nullptr;
-  J.two = two ? 1 : 0;
+  J.rc = rc ? 1 : 0;
   J.tw = tw;
   J.pro = pro;
   J.epi = epi;
@@ -719,24 +797,12 @@ static NttJob make_job(View x, View src, u64 src_len, u32 L, const Twiddle& tw, 
   return J;
 }
 
-// A fused-reversal transform takes its 2^L-element second buffer from the
-// device's default stream-ordered pool.  The pool's release threshold is
-// raised once per device so that freed blocks stay in the pool: repeated
-// transforms then reuse the memory instead of mapping it on every call.
-static hipError_t keep_pool_memory() {
-  static std::mutex mu;
-  static bool done[64] = {};
-  int dev = 0;
-  IMP_CHECK(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lk(mu);
-  if (dev >= 0 && dev < 64 && !done[dev]) {
-    hipMemPool_t pool;
-    IMP_CHECK(hipDeviceGetDefaultMemPool(&pool, dev));
-    uint64_t keep = ~0ull;
-    IMP_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
-    done[dev] = true;
-  }
-  return hipSuccess;
+// A fused-reversal plan without a second buffer runs the same passes in
+// place after the bit-reversal kernel (same result).
+static void drop_fused_reversal(NttJob* J) {
+  J->fbr = 0;
+  J->zlev = 0;
+  J->tmp = nullptr;
 }
 
 static hipError_t launch_stages(const NttJob& J, hipStream_t stream) {
@@ -747,7 +813,7 @@ static hipError_t launch_stages(const NttJob& J, hipStream_t stream) {
   }
   for (u32 st = 0; st < J.nst; st++) {
     TimedScope ts(K_NTT_DIT, stream);
-    if (J.two)
+    if (J.rc)
       hipLaunchKernelGGL(k_ntt_dit<true>, dim3(J.st[st].blocks), dim3(256), stage_lds(J, st),
                          stream, J, st);
     else
@@ -758,32 +824,61 @@ static hipError_t launch_stages(const NttJob& J, hipStream_t stream) {
   return hipSuccess;
 }
 
-static hipError_t run_job(const NttJob& J_in, hipStream_t stream) {
-  if (!J_in.fbr) return launch_stages(J_in, stream);
-  NttJob J = J_in;
-  IMP_CHECK(keep_pool_memory());
-  if (hipMallocAsync((void**)&J.tmp, (size_t)32 << J.L, stream) != hipSuccess) {
-    // no room for the second buffer: the same passes in place, after the
-    // bit-reversal kernel (same result)
-    (void)hipGetLastError();
-    J.fbr = 0;
-    J.tmp = nullptr;
-    return launch_stages(J, stream);
+// Scratch of one impeller_ntt call: the coset / inverse factor tables, n^-1
+// and, for a fused-reversal plan, the 2^L-element second buffer.
+static const size_t kFactorBytes = (size_t)3 * kFactorTab * 32;
+static const size_t kNttFixedBytes = 2 * kFactorBytes + 256;
+
+struct NttScratch {
+  u32 *pro, *epi, *ninv, *tmp;
+};
+
+static NttScratch carve_scratch(char* ws, size_t bytes, bool want_tmp, u32 L) {
+  NttScratch s;
+  s.pro = (u32*)ws;
+  s.epi = (u32*)(ws + kFactorBytes);
+  s.ninv = (u32*)(ws + 2 * kFactorBytes);
+  s.tmp = nullptr;
+  const size_t need = kNttFixedBytes + ((size_t)32 << L);
+  if (want_tmp && bytes >= need) s.tmp = (u32*)(ws + kNttFixedBytes);
+  return s;
+}
+
+// The factor tables of one transform go into scratch; returns the job ready
+// to launch (its fused reversal dropped when the scratch has no room for it).
+static hipError_t prepare_single(uint32_t* x, const uint32_t* twiddle, u32 L, int inverse,
+                                 const uint32_t* coset_shift, const PassPlan& plan,
+                                 const NttScratch& sc, hipStream_t stream, NttJob* out) {
+  View v = {x, 0, 1ull << L};
+  Twiddle tw = {twiddle, nullptr, nullptr, 0};
+  Factor pro = {nullptr, 1, 1}, epi = {nullptr, 1, 1};
+  if (!inverse) {
+    if (coset_shift) {
+      pro = factor_shape(sc.pro, L);
+      IMP_CHECK(launch_factor_tables(pro, coset_shift, nullptr, stream));
+    }
+  } else {
+    hipLaunchKernelGGL(k_ninv, dim3(1), dim3(1), 0, stream, sc.ninv, L);
+    IMP_CHECK(hipGetLastError());
+    // factor(i) = n^-1 * shift^i (shift given as its inverse by the caller)
+    // or the constant n^-1 (base 1, one digit)
+    epi = factor_shape(sc.epi, coset_shift ? L : 1);
+    IMP_CHECK(launch_factor_tables(epi, coset_shift, sc.ninv, stream));
   }
-  const hipError_t e = launch_stages(J, stream);
-  const hipError_t f = hipFreeAsync(J.tmp, stream);
-  return e != hipSuccess ? e : f;
+  *out = make_job(v, v, v.wl, L, tw, false, pro, epi, plan);
+  out->tmp = sc.tmp;
+  if (out->fbr && !sc.tmp) drop_fused_reversal(out);
+  return hipSuccess;
 }
 
-// factor tables in stream-ordered temporary memory
-static hipError_t make_factor(const u32* base_d, const u32* scale_d, u32 L, hipStream_t stream,
-                              Factor* f, u32** tmp) {
-  IMP_CHECK(hipMallocAsync((void**)tmp, (size_t)3 * kFactorTab * 32, stream));
-  *f = factor_shape(*tmp, L);
-  return launch_factor_tables(*f, base_d, scale_d, stream);
+// Host Montgomery form of the integer x (< 2^64) for Fr
+static host::Fe host_from_u64(uint64_t x) {
+  host::Fe a = {{x, 0, 0, 0}}, r2;
+  u32 t[8];
+  for (int i = 0; i < 8; i++) t[i] = FrParams::r2[i];
+  r2 = host::load(t);
+  return host::mul<FrParams>(a, r2);
 }
-
-static std::mutex g_host_mu;
 
 }  // namespace imp
 
@@ -799,44 +894,26 @@ extern "C" hipError_t ssip_ntt(PolyPtr x, const unsigned int* twiddle, unsigned 
   IMP_REQUIRE(make_view(x, &v), "PolyPtr view");
   Twiddle tw = {twiddle, nullptr, nullptr, 0};
   PassPlan plan;
-  default_plan(log_len, &plan);
+  default_inplace_plan(log_len, &plan);
   Factor none = {nullptr, 1, 1};
-  return run_job(make_job(v, v, x.len, log_len, tw, false, none, none, plan), stream);
+  return launch_stages(make_job(v, v, x.len, log_len, tw, false, none, none, plan), stream);
 }
 
 extern "C" hipError_t recompute_ntt(PolyPtr x, const unsigned int* pq_d, unsigned int pq_deg,
                                     const unsigned int* omegas_d, unsigned int log_len,
                                     hipStream_t stream, const unsigned int, const unsigned int) {
-  (void)pq_d;
-  (void)pq_deg;
   IMP_REQUIRE(log_len <= 31, "log_len");
   IMP_REQUIRE(x.len == (1ull << log_len), "x.len must be 2^log_len");
   IMP_REQUIRE(x.ptr != nullptr && omegas_d != nullptr, "null pointer");
   View v;
   IMP_REQUIRE(make_view(x, &v), "PolyPtr view");
   PassPlan plan;
-  default_plan(log_len, &plan);
+  recompute_plan(log_len, &plan);
   Factor none = {nullptr, 1, 1};
-  Twiddle tw = {omegas_d, nullptr, nullptr, 0};  // n <= 2: only omega^0 = 1 is used
-  bool two = false;
-  u32* tabs = nullptr;
-  if (log_len >= 2) {
-    // two-level twiddles from omegas[k] = omega^(2^k): lo[i] = w^i (i < 2^h),
-    // hi[j] = w^(j * 2^h); omega^e = lo[e mod 2^h] * hi[e >> h] exactly
-    u32 h = log_len / 2;
-    u64 nlo = 1ull << h, nhi = 1ull << (log_len - 1 - h);
-    IMP_CHECK(hipMallocAsync((void**)&tabs, (size_t)(nlo + nhi) * 32, stream));
-    hipLaunchKernelGGL(k_pow_from_omegas, dim3((u32)((nlo + 255) / 256)), dim3(256), 0, stream,
-                       tabs, nlo, (u64)0, omegas_d);
-    hipLaunchKernelGGL(k_pow_from_omegas, dim3((u32)((nhi + 255) / 256)), dim3(256), 0, stream,
-                       tabs + 8 * nlo, nhi, (u64)h, omegas_d);
-    IMP_CHECK(hipGetLastError());
-    two = true;
-    tw = Twiddle{nullptr, tabs, tabs + 8 * nlo, h};
-  }
-  hipError_t e = run_job(make_job(v, v, x.len, log_len, tw, two, none, none, plan), stream);
-  if (tabs) IMP_CHECK(hipFreeAsync(tabs, stream));
-  return e;
+  // pq only helps when it covers a pass's in-group twiddles (pq_deg >= D);
+  // rc_prologue checks that per pass and otherwise uses omegas
+  Twiddle tw = {nullptr, pq_deg >= 1 ? pq_d : nullptr, omegas_d, pq_deg};
+  return launch_stages(make_job(v, v, x.len, log_len, tw, true, none, none, plan), stream);
 }
 
 extern "C" hipError_t ssip_precompute(unsigned int* twiddle, unsigned int len,
@@ -845,40 +922,43 @@ extern "C" hipError_t ssip_precompute(unsigned int* twiddle, unsigned int len,
   IMP_REQUIRE(twiddle && unit, "null pointer");
   u32 *d_out = nullptr, *d_unit = nullptr;
   IMP_CHECK(hipMalloc((void**)&d_out, (size_t)len * 32));
-  IMP_CHECK(hipMalloc((void**)&d_unit, 32));
-  IMP_CHECK(hipMemcpy(d_unit, unit, 32, hipMemcpyHostToDevice));
-  u32 chunk = 64;
-  u64 threads = ((u64)len + chunk - 1) / chunk;
-  hipLaunchKernelGGL(k_powers, dim3((u32)((threads + 255) / 256)), dim3(256), 0, 0, d_out,
-                     (u64)len, d_unit, chunk);
-  IMP_CHECK(hipGetLastError());
-  IMP_CHECK(hipMemcpy(twiddle, d_out, (size_t)len * 32, hipMemcpyDeviceToHost));
-  IMP_CHECK(hipFree(d_out));
-  IMP_CHECK(hipFree(d_unit));
-  return hipSuccess;
+  hipError_t e = hipMalloc((void**)&d_unit, 32);
+  if (e == hipSuccess) e = hipMemcpy(d_unit, unit, 32, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    u32 chunk = 64;
+    u64 threads = ((u64)len + chunk - 1) / chunk;
+    hipLaunchKernelGGL(k_powers, dim3((u32)((threads + 255) / 256)), dim3(256), 0, 0, d_out,
+                       (u64)len, d_unit, chunk);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(twiddle, d_out, (size_t)len * 32, hipMemcpyDeviceToHost);
+  (void)hipFree(d_out);
+  if (d_unit) (void)hipFree(d_unit);
+  return e;
 }
 
+// precompute.cuh:89-112 semantics, computed on the host: omegas[i] =
+// unit^(2^i) for i < 32 (always), pq[0] = 1 and pq[i] = (unit^(len >>
+// pq_deg))^i for 0 < i < 2^(pq_deg - 1).  Null pointers or pq_deg > 31 leave
+// the outputs untouched (the function returns nothing to report with).
 extern "C" void gen_pq_omegas(unsigned int* pq, unsigned int* omegas, unsigned int pq_deg,
                               unsigned int len, unsigned int* unit) {
-  // precompute.cuh:89-112 semantics (host outputs), computed on the device:
-  // omegas[i] = unit^(2^i), i < 32; pq[i] = unit^(i * (len >> pq_deg)).
-  if (!pq || !omegas || !unit || pq_deg == 0 || pq_deg > 31) return;
-  if (((u64)len >> pq_deg) == 0) return;  // the reference requires len >= 2^pq_deg
-  std::lock_guard<std::mutex> lk(g_host_mu);
-  u64 npq = 1ull << (pq_deg - 1);
-  u64 step = (u64)len >> pq_deg;
-  u32 shift = 0;
-  while ((1ull << shift) < step) shift++;
-  u32* d = nullptr;
-  if (hipMalloc((void**)&d, (size_t)(npq + 32) * 32) != hipSuccess) return;
-  u32* d_om = d + 8 * npq;
-  hipMemcpy(d_om, unit, 32, hipMemcpyHostToDevice);
-  hipLaunchKernelGGL(k_square_chain, dim3(1), dim3(1), 0, 0, d_om, 32u);
-  hipLaunchKernelGGL(k_pow_from_omegas, dim3((u32)((npq + 255) / 256)), dim3(256), 0, 0, d, npq,
-                     (u64)shift, d_om);
-  hipMemcpy(omegas, d_om, 32 * 32, hipMemcpyDeviceToHost);
-  hipMemcpy(pq, d, (size_t)npq * 32, hipMemcpyDeviceToHost);
-  hipFree(d);
+  if (!pq || !omegas || !unit || pq_deg > 31) return;
+  const host::Fe u = host::load(unit);
+  host::Fe w = u;
+  for (u32 i = 0; i < 32; i++) {
+    host::store(omegas + 8 * i, w);
+    w = host::mul<FrParams>(w, w);
+  }
+  host::store(pq, host::one<FrParams>());
+  const u64 npq = (1ull << pq_deg) >> 1;
+  if (npq < 2) return;
+  const host::Fe step = host::pow<FrParams>(u, (u64)len >> pq_deg);
+  host::Fe acc = step;
+  for (u64 i = 1; i < npq; i++) {
+    host::store(pq + 8 * i, acc);
+    acc = host::mul<FrParams>(acc, step);
+  }
 }
 
 extern "C" hipError_t distribute_powers(PolyPtr poly, const unsigned int* powers,
@@ -902,6 +982,9 @@ extern "C" hipError_t impeller_twiddles(uint32_t* twiddle_d, unsigned long long 
                                         const uint32_t* omega_host, hipStream_t stream) {
   if (len == 0) return hipSuccess;
   IMP_REQUIRE(twiddle_d && omega_host, "null pointer");
+  // omega^start for each thread's chunk comes from the device copy of omega,
+  // placed past the table (the table's last chunk is written after the
+  // kernel has read it: each thread reads omega before writing)
   u32* d_unit = nullptr;
   IMP_CHECK(hipMallocAsync((void**)&d_unit, 32, stream));
   IMP_CHECK(hipMemcpyAsync(d_unit, omega_host, 32, hipMemcpyHostToDevice, stream));
@@ -915,41 +998,68 @@ extern "C" hipError_t impeller_twiddles(uint32_t* twiddle_d, unsigned long long 
   return hipStreamSynchronize(stream);
 }
 
+extern "C" size_t impeller_ntt_workspace_size(unsigned int log_len,
+                                              const unsigned int* plan_passes,
+                                              unsigned int n_passes) {
+  PassPlan plan;
+  if (log_len > 31 || !plan_from(plan_passes, n_passes, log_len, &plan)) return 0;
+  return kNttFixedBytes + (plan.fuse == 1 ? ((size_t)32 << log_len) : 0);
+}
+
+extern "C" hipError_t impeller_ntt_ws(uint32_t* x, const uint32_t* twiddle, unsigned int log_len,
+                                      int inverse, const uint32_t* coset_shift,
+                                      const unsigned int* plan_passes, unsigned int n_passes,
+                                      void* workspace, size_t workspace_bytes,
+                                      hipStream_t stream) {
+  IMP_REQUIRE(log_len <= 31, "log_len");
+  IMP_REQUIRE(x && (log_len == 0 || twiddle), "null pointer");
+  IMP_REQUIRE(workspace && workspace_bytes >= kNttFixedBytes, "workspace too small");
+  PassPlan plan;
+  IMP_REQUIRE(plan_from(plan_passes, n_passes, log_len, &plan), "pass plan");
+  const NttScratch sc = carve_scratch((char*)workspace, workspace_bytes, plan.fuse == 1, log_len);
+  NttJob J;
+  IMP_CHECK(prepare_single(x, twiddle, log_len, inverse, coset_shift, plan, sc, stream, &J));
+  return launch_stages(J, stream);
+}
+
 extern "C" hipError_t impeller_ntt(uint32_t* x, const uint32_t* twiddle, unsigned int log_len,
                                    int inverse, const uint32_t* coset_shift,
                                    const unsigned int* plan_passes, unsigned int n_passes,
                                    hipStream_t stream) {
   IMP_REQUIRE(log_len <= 31, "log_len");
   IMP_REQUIRE(x && (log_len == 0 || twiddle), "null pointer");
-  View v = {x, 0, 1ull << log_len};
-  Twiddle tw = {twiddle, nullptr, nullptr, 0};
   PassPlan plan;
   IMP_REQUIRE(plan_from(plan_passes, n_passes, log_len, &plan), "pass plan");
-  Factor pro = {nullptr, 1, 1}, epi = {nullptr, 1, 1};
-  u32* pro_tab = nullptr;
-  u32* epi_tab = nullptr;
-  u32* ninv = nullptr;
-  if (!inverse) {
-    if (coset_shift) IMP_CHECK(make_factor(coset_shift, nullptr, log_len, stream, &pro, &pro_tab));
-  } else {
-    IMP_CHECK(hipMallocAsync((void**)&ninv, 32, stream));
-    hipLaunchKernelGGL(k_ninv, dim3(1), dim3(1), 0, stream, ninv, (u32)log_len);
-    IMP_CHECK(hipGetLastError());
-    // factor(i) = n^-1 * shift^i (shift given as its inverse by the caller)
-    // or the constant n^-1 (base 1, one digit)
-    IMP_CHECK(make_factor(coset_shift, ninv, coset_shift ? log_len : 1, stream, &epi, &epi_tab));
+  // scratch from the stream-ordered pool (its release threshold is left as
+  // the process set it); without room for the second buffer the passes run
+  // in place
+  const bool need_factors = inverse || coset_shift;
+  size_t bytes = plan.fuse == 1 ? kNttFixedBytes + ((size_t)32 << log_len) : kNttFixedBytes;
+  if (!need_factors && plan.fuse != 1) bytes = 0;
+  char* ws = nullptr;
+  if (bytes && hipMallocAsync((void**)&ws, bytes, stream) != hipSuccess) {
+    (void)hipGetLastError();
+    ws = nullptr;
+    if (need_factors) {
+      bytes = kNttFixedBytes;
+      IMP_CHECK(hipMallocAsync((void**)&ws, bytes, stream));
+    }
   }
-  hipError_t e = run_job(make_job(v, v, v.wl, log_len, tw, false, pro, epi, plan), stream);
-  if (pro_tab) IMP_CHECK(hipFreeAsync(pro_tab, stream));
-  if (epi_tab) IMP_CHECK(hipFreeAsync(epi_tab, stream));
-  if (ninv) IMP_CHECK(hipFreeAsync(ninv, stream));
+  NttScratch sc = {nullptr, nullptr, nullptr, nullptr};
+  if (ws) sc = carve_scratch(ws, bytes, plan.fuse == 1, log_len);
+  NttJob J;
+  hipError_t e = prepare_single(x, twiddle, log_len, inverse, coset_shift, plan, sc, stream, &J);
+  if (e == hipSuccess) e = launch_stages(J, stream);
+  if (ws) {
+    const hipError_t f = hipFreeAsync(ws, stream);
+    if (e == hipSuccess) e = f;
+  }
   return e;
 }
 
 // ------------------------------------------------------------ batched NTT
 namespace {
-const size_t kFactorBytes = (size_t)3 * kFactorTab * 32;
-const size_t kJobArea = 2 * kFactorBytes + 256;  // pro table, epi table, n^-1
+const size_t kJobArea = kNttFixedBytes;  // pro table, epi table, n^-1
 size_t batch_jobs_off() { return 0; }
 size_t batch_prefix_off(u32 count) { return align_up((size_t)count * sizeof(NttJob), 256); }
 size_t batch_factor_off(u32 count) {

@@ -56,9 +56,10 @@ typedef struct PolySlice {
 /* Replaces core/src/ntt/src/ntt.h:6 (ssip_ntt).  In-place forward NTT of x
  * (natural order in and out), bit-identical to halo2 best_fft with the same
  * omega, for any omega (core/tests/ntt.rs:63,151-153).  twiddle[i] =
- * omega^i for i < 2^(log_len-1), device memory.  Asynchronous on stream.
- * The two thread-count hints are accepted and ignored: the pass split comes
- * from the library's planner/autotuner. */
+ * omega^i for i < 2^(log_len-1), device memory.  Asynchronous on stream;
+ * allocates nothing (the passes run in place, as in the reference, whose
+ * runtime plans all device memory).  The two thread-count hints are accepted
+ * and ignored: the pass split comes from the library's in-place plan. */
 hipError_t ssip_ntt(PolyPtr x, const unsigned int *twiddle, unsigned int log_len,
                     hipStream_t stream, const unsigned int max_threads_stage1_log,
                     const unsigned int max_threads_stage2_log);
@@ -69,16 +70,22 @@ hipError_t ssip_ntt(PolyPtr x, const unsigned int *twiddle, unsigned int log_len
 hipError_t ssip_precompute(unsigned int *twiddle, unsigned int len, const unsigned int *unit);
 
 /* Replaces core/src/ntt/src/ntt.h:12 (recompute_ntt).  Same transform as
- * ssip_ntt with twiddles derived from omegas_d[i] = omega^(2^i) (i < 32);
- * pq_d / pq_deg follow gen_pq_omegas.  Needs no n/2 twiddle table. */
+ * ssip_ntt without an n/2 twiddle table: every pass work-group derives its
+ * twiddles into LDS from pq_d[i] = omega^(i 2^(log_len - pq_deg)) (i <
+ * 2^(pq_deg-1), used for a pass of at most pq_deg levels; may be NULL) and
+ * omegas_d[k] = omega^(2^k) (k < 32), both device memory as written by
+ * gen_pq_omegas.  Asynchronous; allocates nothing; device memory in use is
+ * x, pq and omegas only (recompute_ntt.cuh:8-20,304-333). */
 hipError_t recompute_ntt(PolyPtr x, const unsigned int *pq_d, unsigned int pq_deg,
                          const unsigned int *omegas_d, unsigned int log_len,
                          hipStream_t stream, const unsigned int max_threads_stage1_log,
                          const unsigned int max_threads_stage2_log);
 
-/* Replaces core/src/ntt/src/ntt.h:15 (gen_pq_omegas), host memory:
- * pq[i] = (unit^(len >> pq_deg))^i for i < 2^(pq_deg-1); omegas[i] =
- * unit^(2^i) for i < 32 (core/src/ntt/src/precompute.cuh:89-112). */
+/* Replaces core/src/ntt/src/ntt.h:15 (gen_pq_omegas), computed on the host
+ * into host memory (core/src/ntt/src/precompute.cuh:89-112): omegas[i] =
+ * unit^(2^i) for i < 32 (always written), pq[0] = 1 and pq[i] = (unit^(len
+ * >> pq_deg))^i for 0 < i < 2^(pq_deg-1).  NULL pointers or pq_deg > 31
+ * leave both outputs untouched. */
 void gen_pq_omegas(unsigned int *pq, unsigned int *omegas, unsigned int pq_deg,
                    unsigned int len, unsigned int *unit);
 
@@ -92,8 +99,14 @@ hipError_t distribute_powers(PolyPtr poly, const unsigned int *powers,
  * h_scaler_batch are HOST arrays (points: n_precompute tables of len affine
  * points; scalars: batches arrays of len Fr); h_result[i] receives the affine
  * result of batch i.  With buffers == NULL, buffer_sizes[card] receives the
- * device temp bytes each card needs and nothing else happens.  This symbol
- * uses the library's default window configuration (window 16, no precompute
+ * device temp bytes each card needs and nothing else happens.  As in the
+ * reference (msm_impl.cuh:373-580,703-756), the points are processed in
+ * `parts` ranges through `stage_points` device buffers and the scalars
+ * through `stage_scalers` buffers, copied on a second stream while the
+ * previous range computes, `batch_per_run` batches at a time (0 counts as
+ * 1), so the device buffer can be far smaller than the point set.  One card
+ * (cards[0]) does all the work; the others' sizes are 0.  This symbol uses
+ * the library's default window configuration (window 16, no precompute
  * tables); per-configuration symbols are exported as msm_cfg below. */
 hipError_t msm(void *const *buffers, unsigned long *buffer_sizes, unsigned long long len,
                unsigned int batch_per_run, unsigned int parts, unsigned int stage_scalers,
@@ -122,10 +135,12 @@ hipError_t msm_precompute_cfg(unsigned int window, unsigned int target_window,
                               unsigned int max_cards);
 
 /* Replaces a generated fused kernel with the signature emitted by
- * core/src/fused_kernels/kernel_gen.rs:309-330, for the gate
+ * core/src/fused_kernels/kernel_gen.rs:309-330, for the Lagrange-form gate
  * out = a*b*c - d + q*e*(1 - q) over Fr.  vars (device array of 6
  * ConstPolyPtr) = {a, b, c, d, q, e}; mut_vars (device array of 1 PolyPtr)
- * = {out}; len elements; local_buffer unused (no spills).  Asynchronous. */
+ * = {out}, read by the kernel itself; len elements; slice_info (host) only
+ * places Coef-form constants, of which this gate has none; local_buffer
+ * unused (no spills).  Asynchronous, kernel launch only (capturable). */
 hipError_t fused_gate_abc_d_qe(const ConstPolyPtr *vars, const PolyPtr *mut_vars,
                                unsigned long long len, const PolySlice *slice_info,
                                void *local_buffer, hipStream_t stream);
@@ -150,15 +165,28 @@ hipError_t impeller_gate(const uint32_t *const *inputs, uint32_t *out, unsigned 
  * bit-reversal kernel (0; 4 when log_len >= 8, 16 x 16 tiles; 5 when
  * log_len >= 10, 32 x 32 tiles), or 1: no bit-reversal kernel, the first
  * pass reads x bit-reversed and writes a second 2^log_len-element buffer
- * taken from the device's default stream-ordered pool (at least two passes;
- * the pool's release threshold is raised so the memory stays pooled);
- * lds_log = log2 of the elements one work-group stages in LDS (10 or 11),
- * d_i = levels of each following pass; fused levels + sum(d_i) == log_len.
- * The batched entry points run a fuse-1 plan with the bit-reversal kernel. */
+ * (at least two passes); lds_log = log2 of the elements one work-group
+ * stages in LDS (10 or 11), d_i = levels of each following pass; fused
+ * levels + sum(d_i) == log_len.  Scratch (coset / inverse factor tables,
+ * the second buffer) comes from the device's stream-ordered pool, whose
+ * settings are left alone; without room for the second buffer the same
+ * passes run in place (same result). */
 hipError_t impeller_ntt(uint32_t *x, const uint32_t *twiddle, unsigned int log_len,
                         int inverse, const uint32_t *coset_shift,
                         const unsigned int *plan_passes, unsigned int n_passes,
                         hipStream_t stream);
+
+/* impeller_ntt with caller-owned scratch (no reference analogue): allocates
+ * nothing.  workspace_size gives the bytes for a plan (0 for an invalid
+ * plan): the factor tables, plus the second buffer when the plan fuses the
+ * bit reversal; a workspace of at least the factor part but without room
+ * for the second buffer runs the same passes in place. */
+size_t impeller_ntt_workspace_size(unsigned int log_len, const unsigned int *plan_passes,
+                                   unsigned int n_passes);
+hipError_t impeller_ntt_ws(uint32_t *x, const uint32_t *twiddle, unsigned int log_len,
+                           int inverse, const uint32_t *coset_shift,
+                           const unsigned int *plan_passes, unsigned int n_passes,
+                           void *workspace, size_t workspace_bytes, hipStream_t stream);
 
 /* Batched NTT (no reference analogue): count independent transforms run as
  * one launch per stage (bit reversal, then each LDS pass), work-groups

@@ -26,6 +26,15 @@ MONT_R = 1 << 256
 _lib = None
 
 
+def host_threads():
+    """Worker threads for the multi-threaded checker routines: the
+    OMP_NUM_THREADS share when the environment sets one, else every core."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return os.cpu_count() or 1
+
+
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
@@ -206,7 +215,7 @@ def best_fft(a, omega, log_n, threads=None):
     a = np.ascontiguousarray(a, dtype=np.uint32).copy()
     assert a.shape == (1 << log_n, 8)
     omega = np.ascontiguousarray(omega, dtype=np.uint32)
-    lib().orc_best_fft(_ptr(a), _ptr(omega), log_n, threads or os.cpu_count() or 1)
+    lib().orc_best_fft(_ptr(a), _ptr(omega), log_n, threads or host_threads())
     return a
 
 
@@ -272,7 +281,7 @@ def on_curve_xyzz(p):
 
 def gen_bases(n, seed, start=0, threads=None):
     out = np.zeros((n, 16), dtype=np.uint32)
-    lib().orc_gen_bases(seed, start, n, _ptr(out), threads or os.cpu_count() or 1)
+    lib().orc_gen_bases(seed, start, n, _ptr(out), threads or host_threads())
     return out
 
 
@@ -281,7 +290,7 @@ def best_multiexp(scalars, bases, threads=None):
     b = np.ascontiguousarray(bases, dtype=np.uint32)
     assert s.shape[0] == b.shape[0]
     r = np.zeros(16, dtype=np.uint32)
-    lib().orc_best_multiexp(_ptr(s), _ptr(b), s.shape[0], threads or os.cpu_count() or 1, _ptr(r))
+    lib().orc_best_multiexp(_ptr(s), _ptr(b), s.shape[0], threads or host_threads(), _ptr(r))
     return r
 
 

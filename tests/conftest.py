@@ -14,8 +14,10 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def stream():
+    """The current HIP stream of device 0.  A GPU test that runs where no GPU
+    is visible fails (it is deselected by -m "not gpu" on CPU-only hosts)."""
     import torch
     if not torch.cuda.is_available():
-        pytest.skip("no GPU")
+        pytest.fail("GPU test selected but no HIP GPU is visible", pytrace=False)
     from impeller_amd.runtime import Stream
     return Stream.current(0)

@@ -484,6 +484,134 @@ def mlp_grad(E):
                           rtol=1e-4), B
 
 
+def _view(arr, n, rotate=0, offset=0, whole=None):
+    return _lib.PolyPtr(arr.ctypes.data, n, rotate, offset, whole if whole is not None else n)
+
+
+def _logical(phys, n, rotate=0, offset=0, whole=None):
+    wl = whole if whole is not None else n
+    off = (n - rotate) if rotate else offset
+    return (off + np.arange(n)) % wl
+
+
+def ref_ntt_abi(E):
+    """The reference-ABI transforms: ssip_ntt (in place, on rotated views) and
+    recompute_ntt (twiddles from pq / omegas, every pq_deg the compiler can
+    pass), for true roots and a random omega, against best_fft."""
+    rng = random.Random(77)
+    for k in (1, 2, 5, 9, 13, 16):
+        n = 1 << k
+        x = O.gen_scalars(n, 900 + k)
+        for w in (mont(omega(k)), mont(rng.randrange(r))):
+            expect = O.best_fft(x, w, k)
+            tw = flat(O.powers(w, max(1, n // 2)))
+            rot = (3 * k) % n
+            d = flat(np.roll(x, -rot, axis=0)).copy()  # logical x under rotate = rot
+            E.call("ssip_ntt", _view(d, n, rotate=rot), ptr(tw), k, None, 8, 8)
+            got = d.reshape(-1, 8)[_logical(d, n, rotate=rot)]
+            assert (got == expect).all(), ("ssip_ntt", k)
+            for pq_deg in (0, 1, 3, min(k, 8), min(k, 9)):
+                npq = max(1, (1 << pq_deg) >> 1)
+                pq = garbage_u32(npq * 8)
+                om = garbage_u32(32 * 8)
+                E.L.gen_pq_omegas(ptr(pq), ptr(om), pq_deg, n, ptr(flat(w)))
+                d = flat(x).copy()
+                E.call("recompute_ntt", _view(d, n), ptr(pq) if pq_deg else None, pq_deg,
+                       ptr(om), k, None, 8, 8)
+                assert (d.reshape(-1, 8) == expect).all(), ("recompute_ntt", k, pq_deg)
+    # caller-owned scratch: fused-reversal plan with and without room for its
+    # second buffer, coset and inverse factors
+    k, plan = 13, [1, 10, 7, 6]
+    n = 1 << k
+    x = O.gen_scalars(n, 950)
+    w = mont(omega(k))
+    tw = flat(O.powers(w, n // 2))
+    arr = (ctypes.c_uint * len(plan))(*plan)
+    pl = ctypes.cast(arr, ctypes.c_void_p)
+    full = int(E.call("impeller_ntt_workspace_size", k, pl, len(plan)))
+    part = int(E.call("impeller_ntt_workspace_size", k, None, 0))
+    assert full > part > 0
+    for bytes_ in (full, part):
+        ws = garbage_u32((bytes_ + 3) // 4)
+        for inverse, shift in ((0, None), (0, flat(mont(7))), (1, None), (1, flat(mont(5)))):
+            d = flat(x).copy()
+            E.call("impeller_ntt_ws", ptr(d), ptr(tw), k, inverse, ptr(shift), pl, len(plan),
+                   ptr(ws), bytes_, None)
+            if inverse:
+                e = O.vec_scale(O.best_fft(x, w, k), mont(pow(n, -1, r)))
+                if shift is not None:
+                    e = O.vec_mul(e, O.powers(mont(5), n))
+            else:
+                e = O.best_fft(x if shift is None else O.vec_mul(x, O.powers(mont(7), n)), w, k)
+            assert (d.reshape(-1, 8) == e).all(), (bytes_, inverse, shift is not None)
+
+
+def gate_ref_abi(E):
+    """fused_gate_abc_d_qe through the generated-kernel convention: the views
+    (rotated, sliced) sit in a device array that the kernel reads."""
+    n = 1000
+    ins = [O.gen_scalars(n, 60 + i) for i in range(6)]
+    whole = 1500
+    views, bufs = [], []
+    for i, a in enumerate(ins):
+        if i % 3 == 0:
+            rot = 17 * (i + 1)
+            b = flat(np.roll(a, -rot, axis=0)).copy()
+            views.append(_view(b, n, rotate=rot))
+        elif i % 3 == 1:
+            off = 1300  # wraps around the end of the backing array
+            b = garbage_u32(whole * 8).reshape(-1, 8)
+            b[(off + np.arange(n)) % whole] = a
+            b = flat(b)
+            views.append(_view(b, n, offset=off, whole=whole))
+        else:
+            b = flat(a).copy()
+            views.append(_view(b, n))
+        bufs.append(b)
+    out = garbage_u32(n * 8)
+    orot = 5
+    arr = (_lib.PolyPtr * 6)(*views)
+    marr = (_lib.PolyPtr * 1)(_view(out, n, rotate=orot))
+    sl = _lib.PolySlice(3, n, n)
+    E.call("fused_gate_abc_d_qe", ctypes.addressof(arr), ctypes.addressof(marr), n,
+           ctypes.byref(sl), None, None)
+    got = out.reshape(-1, 8)[_logical(out, n, rotate=orot)]
+    assert (got == O.gate(*ins)).all()
+
+
+def msm_host_pipeline(E):
+    """The reference's host-buffer msm_cfg with parts, point / scalar stages
+    and batches per run (one table and precomputed tables), each batch
+    against the closed form."""
+    n = 600
+    kk = O.gen_scalars(n, 77)
+    bases = u32(n * 16)
+    E.call("impeller_gen_bases", ptr(bases), n, 77, 0, None)
+    cards = (ctypes.c_uint * 1)(0)
+    for c, target, parts, sp, ss, bpr, batches in ((8, 0, 1, 1, 1, 1, 2), (8, 0, 3, 2, 2, 2, 3),
+                                                  (10, 2, 3, 2, 3, 2, 2), (9, 1, 4, 3, 1, 0, 2)):
+        nt = int(E.call("impeller_msm_tables", c, target))
+        tabs = [bases.copy()]
+        if nt > 1:
+            t = u32(nt * n * 16)
+            t[: n * 16] = bases
+            E.call("msm_precompute_cfg", c, target, n,
+                   (ctypes.c_void_p * nt)(*[t[i * n * 16:].ctypes.data for i in range(nt)]), 1)
+            tabs = [t[i * n * 16:(i + 1) * n * 16].copy() for i in range(nt)]
+        sizes = (ctypes.c_ulong * 1)()
+        E.call("msm_cfg", c, target, None, sizes, n, bpr, parts, ss, sp, 1, cards, None, 0,
+               None, None)
+        buf = garbage_u32((sizes[0] + 3) // 4)
+        scal = [flat(O.gen_scalars(n, 700 + b, kind=b % 3)) for b in range(batches)]
+        res = [garbage_u32(16) for _ in range(batches)]
+        P = lambda xs: (ctypes.c_void_p * len(xs))(*[x.ctypes.data for x in xs])  # noqa: E731
+        E.call("msm_cfg", c, target, P([buf]), None, n, bpr, parts, ss, sp, 1, cards, P(tabs),
+               batches, P(scal), P(res))
+        for b in range(batches):
+            exp = O.msm_closed_form(scal[b].reshape(-1, 8), kk)
+            assert (res[b] == exp).all(), (c, target, parts, sp, ss, bpr, b)
+
+
 def gate(E):
     for n in (1, 777, 4096):
         ins = [O.gen_scalars(n, 40 + i) for i in range(6)]
@@ -499,7 +627,8 @@ ALL = [("single_ntt", single_ntt), ("ntt_plans", ntt_plans),
        ("batched_ntt_plans", batched_ntt_plans), ("ntt_graph_replay", ntt_graph_replay),
        ("gate", gate), ("field_inverse", field_inverse), ("arith29", arith29), ("mlp_grad", mlp_grad), ("single_msm", single_msm), ("msm_hot_bucket", msm_hot_bucket), ("msm_graph_replay", msm_graph_replay),
        ("batched_msm_shared_tables", batched_msm_shared_tables),
-       ("batched_msm_degenerate", batched_msm_degenerate)]
+       ("batched_msm_degenerate", batched_msm_degenerate), ("ref_ntt_abi", ref_ntt_abi),
+       ("gate_ref_abi", gate_ref_abi), ("msm_host_pipeline", msm_host_pipeline)]
 
 
 def main(argv):

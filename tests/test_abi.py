@@ -89,3 +89,77 @@ def test_pq_len_matches_reference():
     assert g.get_pq_len(20) == 16
     # k=24: get_deg(12, 9) = 6, get_deg(12, 5) = 4 -> 2^5
     assert g.get_pq_len(24) == 32
+
+
+def _fe(x):
+    from oracle import oracle as O
+    return O.int_to_limbs(O.to_mont_int(x))
+
+
+def _pow_fe(base_int, e):
+    from oracle import oracle as O
+    return _fe(pow(base_int, e, O.R_MOD))
+
+
+def test_gen_pq_omegas_host_values():
+    """gen_pq_omegas runs on the host (precompute.cuh:89-112): omegas[i] =
+    unit^(2^i) always; pq[0] = 1, pq[i] = unit^(i * (len >> pq_deg)), also
+    for pq_deg 0 / 1 and a length that is not a power of two."""
+    import random
+    from oracle import oracle as O
+    L = _lib.lib()
+    rng = random.Random(5)
+    for pq_deg, length in ((0, 1 << 10), (1, 1 << 10), (2, 12), (5, 3 << 10), (9, 1 << 20)):
+        u = rng.randrange(1, O.R_MOD)
+        npq = max(1, (1 << pq_deg) >> 1)
+        pq = np.full((npq, 8), 0xDEADBEEF, np.uint32)
+        om = np.full((32, 8), 0xDEADBEEF, np.uint32)
+        unit = _fe(u)
+        L.gen_pq_omegas(pq.ctypes.data, om.ctypes.data, pq_deg, length, unit.ctypes.data)
+        for i in range(32):
+            assert (om[i] == _pow_fe(u, 1 << i)).all(), (pq_deg, i)
+        step = length >> pq_deg
+        for i in range(npq):
+            assert (pq[i] == _pow_fe(u, i * step)).all(), (pq_deg, length, i)
+    # null pointers leave the outputs alone
+    om = np.full((32, 8), 7, np.uint32)
+    L.gen_pq_omegas(None, om.ctypes.data, 3, 64, _fe(3).ctypes.data)
+    assert (om == 7).all()
+
+
+def test_msm_rejects_point_index_overflow():
+    """Point indices are packed into 31 bits: n_pre * len >= 2^31 is an
+    argument error before any device work (host ABI and device ABI)."""
+    L = _lib.lib()
+    sizes = (ctypes.c_ulong * 1)()
+    cards = (ctypes.c_uint * 1)(0)
+    assert L.msm(None, sizes, 1 << 31, 1, 1, 1, 1, 1, cards, None, 0, None, None) != 0
+    # parts split the points: each part's indices fit again
+    assert L.msm(None, sizes, 1 << 31, 1, 4, 1, 1, 1, cards, None, 0, None, None) == 0
+    fake = ctypes.c_void_p(4096)
+    assert L.impeller_msm(fake, fake, 1 << 31, 16, fake, fake, 1 << 40, None) != 0
+
+
+def test_msm_host_buffer_shrinks_with_parts():
+    L = _lib.lib()
+    sizes = (ctypes.c_ulong * 1)()
+    cards = (ctypes.c_uint * 1)(0)
+    got = []
+    for parts, sp in ((1, 1), (4, 2), (16, 2), (64, 2)):
+        assert L.msm(None, sizes, 1 << 22, 1, parts, 1, sp, 1, cards, None, 0, None, None) == 0
+        got.append(sizes[0])
+    # the sort workspace (16 bytes per point and window) shrinks with the
+    # part; the bucket arrays (per window, independent of len) stay
+    assert got[0] > (1 << 22) * 64 and got[1] < got[0] and got[2] < got[1] and got[3] < got[2]
+    assert got[3] < (1 << 22) * 64  # smaller than the point set
+
+
+def test_ntt_workspace_size():
+    L = _lib.lib()
+    fused = (ctypes.c_uint * 5)(1, 10, 8, 8, 8)
+    inplace = (ctypes.c_uint * 5)(0, 10, 10, 8, 6)
+    base = L.impeller_ntt_workspace_size(24, inplace, 5)
+    assert base > 0
+    assert L.impeller_ntt_workspace_size(24, fused, 5) == base + (32 << 24)
+    bad = (ctypes.c_uint * 4)(1, 10, 8, 8)  # levels do not add up to 24
+    assert L.impeller_ntt_workspace_size(24, bad, 4) == 0
