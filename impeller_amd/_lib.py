@@ -83,6 +83,7 @@ SIGNATURES = {
     "impeller_ntt": (_ERR, [_P, _P, _U32, _INT, _P, _P, _U32, _P]),
     "impeller_ntt_workspace_size": (ctypes.c_size_t, [_U32, _P, _U32]),
     "impeller_ntt_ws": (_ERR, [_P, _P, _U32, _INT, _P, _P, _U32, _P, ctypes.c_size_t, _P]),
+    "impeller_ntt_noalloc": (_ERR, [_P, _P, _U32, _INT, _P, _P, _U32, _P]),
     "impeller_twiddles": (_ERR, [_P, _U64, _P, _P]),
     "impeller_ntt_batch_bytes": (ctypes.c_size_t, [_U32]),
     "impeller_ntt_batch_bytes_items": (ctypes.c_size_t, [ctypes.POINTER(ImpNttItem), _U32]),

@@ -828,6 +828,9 @@ static hipError_t launch_stages(const NttJob& J, hipStream_t stream) {
 // and, for a fused-reversal plan, the 2^L-element second buffer.
 static const size_t kFactorBytes = (size_t)3 * kFactorTab * 32;
 static const size_t kNttFixedBytes = 2 * kFactorBytes + 256;
+// the factor tables cover 3 * kFactorDigit = 39 index bits; transforms stop
+// at 2^31 points (32-bit block counts), for every entry point
+static const u32 kNoAllocMaxLog = 31;
 
 struct NttScratch {
   u32 *pro, *epi, *ninv, *tmp;
@@ -1055,6 +1058,56 @@ extern "C" hipError_t impeller_ntt(uint32_t* x, const uint32_t* twiddle, unsigne
     if (e == hipSuccess) e = f;
   }
   return e;
+}
+
+// ------------------------------------------------- allocation-free variant
+// impeller_ntt_noalloc keeps its scratch (the coset / inverse factor tables
+// and n^-1: kNttFixedBytes = 1.5 MiB + 256 B) in this file-scope device
+// array, one copy per GPU, held until the process ends.  Host code never
+// passes the array's name to a kernel (on the host it only names the
+// symbol): the device address comes from hipGetSymbolAddress, once per
+// device, and is kept per device.
+static const size_t kNoAllocWords = kNttFixedBytes / 4;
+__device__ __align__(256) u32 scratch[kNoAllocWords];
+
+namespace imp {
+static const int kMaxDevices = 64;
+
+static hipError_t noalloc_scratch(char** out) {
+  static std::mutex mu;
+  static char* ptr[kMaxDevices] = {};
+  int dev = 0;
+  IMP_CHECK(hipGetDevice(&dev));
+  IMP_REQUIRE(dev >= 0 && dev < kMaxDevices, "device ordinal");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    IMP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    IMP_REQUIRE(p != nullptr, "scratch symbol address");
+    ptr[dev] = (char*)p;
+  }
+  *out = ptr[dev];
+  return hipSuccess;
+}
+}  // namespace imp
+
+extern "C" hipError_t impeller_ntt_noalloc(uint32_t* x, const uint32_t* twiddle,
+                                           unsigned int log_len, int inverse,
+                                           const uint32_t* coset_shift,
+                                           const unsigned int* plan_passes,
+                                           unsigned int n_passes, hipStream_t stream) {
+  IMP_REQUIRE(log_len <= kNoAllocMaxLog, "log_len above the allocation-free maximum (31)");
+  IMP_REQUIRE(x && (log_len == 0 || twiddle), "null pointer");
+  PassPlan plan;
+  IMP_REQUIRE(plan_from(plan_passes, n_passes, log_len, &plan), "pass plan");
+  char* ws = nullptr;
+  IMP_CHECK(noalloc_scratch(&ws));
+  // the factor tables only (no room for a second buffer: a fused-reversal
+  // plan runs its passes in place)
+  const NttScratch sc = carve_scratch(ws, kNttFixedBytes, false, log_len);
+  NttJob J;
+  IMP_CHECK(prepare_single(x, twiddle, log_len, inverse, coset_shift, plan, sc, stream, &J));
+  return launch_stages(J, stream);
 }
 
 // ------------------------------------------------------------ batched NTT

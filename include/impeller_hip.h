@@ -188,6 +188,22 @@ hipError_t impeller_ntt_ws(uint32_t *x, const uint32_t *twiddle, unsigned int lo
                            const unsigned int *plan_passes, unsigned int n_passes,
                            void *workspace, size_t workspace_bytes, hipStream_t stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): impeller_ntt with the same arguments and results, whose
+ * scratch (the coset / inverse factor tables and n^-1, 1.5 MiB + 256 B) is a
+ * static device array that each GPU the library runs on holds until the
+ * process ends; it calls no hipMalloc / hipFree.  Plans that fuse the bit
+ * reversal run the same passes in place (there is no second buffer).
+ * log_len above 31 is rejected (hipErrorInvalidValue).  Calls on one GPU
+ * share that GPU's array: calls that overlap on one GPU (different streams)
+ * race on it and must be serialised by the caller; each call rewrites the
+ * parts it reads before reading them. */
+hipError_t impeller_ntt_noalloc(uint32_t *x, const uint32_t *twiddle, unsigned int log_len,
+                                int inverse, const uint32_t *coset_shift,
+                                const unsigned int *plan_passes, unsigned int n_passes,
+                                hipStream_t stream);
+
 /* Batched NTT (no reference analogue): count independent transforms run as
  * one launch per stage (bit reversal, then each LDS pass), work-groups
  * striding over the concatenated tiles of all transforms.  Per item: data

@@ -123,6 +123,9 @@ hipError_t hipEventCreate(hipEvent_t* e);
 #define hipEventDisableTiming 0x2
 hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned flags);
 hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned flags);
+// a __device__ variable is host memory here: its address is the symbol's
+#define HIP_SYMBOL(x) (reinterpret_cast<const void*>(&(x)))
+hipError_t hipGetSymbolAddress(void** p, const void* symbol);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s);
 hipError_t hipEventSynchronize(hipEvent_t e);
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b);

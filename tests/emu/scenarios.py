@@ -546,6 +546,25 @@ def ref_ntt_abi(E):
             assert (d.reshape(-1, 8) == e).all(), (bytes_, inverse, shift is not None)
 
 
+def ntt_noalloc(E):
+    """impeller_ntt_noalloc (factor tables in the library's static scratch)
+    against impeller_ntt, two calls in a row, the second smaller."""
+    for k, inverse, shift, plan in ((13, 0, 7, [1, 10, 7, 6]), (9, 1, 5, None), (6, 1, None, None),
+                                    (11, 0, None, [4, 10, 7])):
+        n = 1 << k
+        x = O.gen_scalars(n, 960 + k)
+        w = mont(omega(k))
+        tw = flat(O.powers(w, n // 2))
+        sh = flat(mont(shift)) if shift else None
+        arr = (ctypes.c_uint * len(plan))(*plan) if plan else None
+        pl = ctypes.cast(arr, ctypes.c_void_p) if plan else None
+        a, b = flat(x).copy(), flat(x).copy()
+        E.call("impeller_ntt", ptr(a), ptr(tw), k, inverse, ptr(sh), pl, len(plan or []), None)
+        E.call("impeller_ntt_noalloc", ptr(b), ptr(tw), k, inverse, ptr(sh), pl,
+               len(plan or []), None)
+        assert (a == b).all(), (k, inverse, shift, plan)
+
+
 def gate_ref_abi(E):
     """fused_gate_abc_d_qe through the generated-kernel convention: the views
     (rotated, sliced) sit in a device array that the kernel reads."""
@@ -628,7 +647,8 @@ ALL = [("single_ntt", single_ntt), ("ntt_plans", ntt_plans),
        ("gate", gate), ("field_inverse", field_inverse), ("arith29", arith29), ("mlp_grad", mlp_grad), ("single_msm", single_msm), ("msm_hot_bucket", msm_hot_bucket), ("msm_graph_replay", msm_graph_replay),
        ("batched_msm_shared_tables", batched_msm_shared_tables),
        ("batched_msm_degenerate", batched_msm_degenerate), ("ref_ntt_abi", ref_ntt_abi),
-       ("gate_ref_abi", gate_ref_abi), ("msm_host_pipeline", msm_host_pipeline)]
+       ("gate_ref_abi", gate_ref_abi), ("msm_host_pipeline", msm_host_pipeline),
+       ("ntt_noalloc", ntt_noalloc)]
 
 
 def main(argv):
