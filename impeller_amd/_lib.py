@@ -80,6 +80,19 @@ SIGNATURES = {
     "msm_precompute_cfg": (_ERR, [_U32, _U32, _U64, _P, _U32]),
     "fused_gate_abc_d_qe": (_ERR, [_P, _P, _U64, ctypes.POINTER(PolySlice), _P, _P]),
     "impeller_gate": (_ERR, [_P, _P, _U64, _P]),
+    "poly_add": (_ERR, [PolyPtr, PolyPtr, PolyPtr, _P]),
+    "poly_sub": (_ERR, [PolyPtr, PolyPtr, PolyPtr, _P]),
+    "poly_mul": (_ERR, [PolyPtr, PolyPtr, PolyPtr, _P]),
+    "poly_one_lagrange": (_ERR, [PolyPtr, _P]),
+    "poly_one_coef": (_ERR, [PolyPtr, _P, ctypes.POINTER(PolySlice)]),
+    "poly_zero": (_ERR, [PolyPtr, _P]),
+    "poly_eval": (_ERR, [_P, ctypes.POINTER(ctypes.c_ulong), PolyPtr, _P, _P, _P]),
+    "kate_division": (_ERR, [_P, ctypes.POINTER(ctypes.c_ulong), _U32, PolyPtr, _P, PolyPtr,
+                             _P]),
+    "scan_mul": (_ERR, [_P, ctypes.POINTER(ctypes.c_ulong), PolyPtr, _P]),
+    "batched_invert": (_ERR, [_P, ctypes.POINTER(ctypes.c_ulong), PolyPtr, _P]),
+    "inv_scalar": (_ERR, [_P, _P]),
+    "scalar_pow": (_ERR, [_P, _U64, _P]),
     "impeller_ntt": (_ERR, [_P, _P, _U32, _INT, _P, _P, _U32, _P]),
     "impeller_ntt_workspace_size": (ctypes.c_size_t, [_U32, _P, _U32]),
     "impeller_ntt_ws": (_ERR, [_P, _P, _U32, _INT, _P, _P, _U32, _P, ctypes.c_size_t, _P]),

@@ -1,0 +1,611 @@
+// poly.hip -- the reference's polynomial library over BN254 Fr
+// (core/src/poly/src/poly.h:6-30): mixed-length add / sub / mul, constant
+// fills, scalar inverse and power, and the scan-shaped operations scan_mul,
+// batched_invert, poly_eval and kate_division.
+//
+// The scan-shaped operations share one two-pass tree.  A work-group takes a
+// tile of STILE = SB * SK consecutive elements; thread t owns the run
+// [t SK, t SK + SK).  The up pass writes each run's summary and the tile's
+// summary (an element of the next level); levels repeat until one tile holds
+// the level.  The down pass, from the top level back to level 0, combines a
+// tile's carry with its run summaries (wave 0, through LDS) into each run's
+// carry, from which every thread finishes its run.  Summaries:
+//   product scans (scan_mul, batched_invert): the product of the run;
+//   suffix Horner (kate_division, poly_eval): h = sum_j x_j b^j over the run,
+//     combined as h_left + b^len(left) h_right.
+// No work-group waits on another one; the levels are separate launches.
+#include "common.h"
+
+namespace imp {
+
+__device__ __forceinline__ Fr fmul(const Fr& a, const Fr& b) { return mul<FrParams>(a, b); }
+__device__ __forceinline__ Fr fadd(const Fr& a, const Fr& b) { return add<FrParams>(a, b); }
+
+__device__ __forceinline__ Fr vld(const View& v, u64 i) {
+  return load_fe<Fr>(v.base + 8 * view_phys(v, i));
+}
+__device__ __forceinline__ void vst(const View& v, u64 i, const Fr& x) {
+  store_fe<Fr>(v.base + 8 * view_phys(v, i), x);
+}
+
+static inline bool make_cview(const ConstPolyPtr& c, View* v) {
+  PolyPtr p;
+  p.ptr = const_cast<u32*>(c.ptr);
+  p.len = c.len;
+  p.rotate = c.rotate;
+  p.offset = c.offset;
+  p.whole_len = c.whole_len;
+  return make_view(p, v);
+}
+
+static inline View flat_view(u32* p, u64 n) {
+  View v;
+  v.base = p;
+  v.off = 0;
+  v.wl = n ? n : 1;
+  return v;
+}
+
+static hipError_t ew_grid(u64 len, u32* blocks) {
+  int dev = 0, cus = 256;
+  IMP_CHECK(hipGetDevice(&dev));
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus = 256;
+  u64 b = (len + 255) / 256, cap = (u64)cus * 16;
+  *blocks = (u32)(b == 0 ? 1 : (b > cap ? cap : b));
+  return hipSuccess;
+}
+
+// ------------------------------------------------------------ element-wise
+// poly_basic.cuh:4-56: r[i] = a[i] op b[i] for i < len, and for
+// len <= i < upper the longer operand alone (negated when it is the
+// subtrahend).  a is the longer operand (the entry points swap).
+enum { EW_ADD = 0, EW_SUB = 1, EW_RSUB = 2, EW_MUL = 3 };
+
+__global__ __launch_bounds__(256) void k_poly_binary(int op, View a, View b, View r, u64 len,
+                                                     u64 upper) {
+  const u64 stride = (u64)gridDim.x * blockDim.x;
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < upper; i += stride) {
+    const Fr x = vld(a, i);
+    Fr z;
+    if (i < len) {
+      const Fr y = vld(b, i);
+      switch (op) {
+        case EW_ADD: z = fadd(x, y); break;
+        case EW_SUB: z = sub<FrParams>(x, y); break;
+        case EW_RSUB: z = sub<FrParams>(y, x); break;
+        default: z = fmul(x, y);
+      }
+    } else {
+      z = op == EW_RSUB ? neg<FrParams>(x) : x;
+    }
+    vst(r, i, z);
+  }
+}
+
+// poly_basic.cuh:58-72: every element := value, or (Coef form) value where
+// slice.to_origin(i) == 0 and zero elsewhere.  mode 0: zero, 1: one
+// everywhere, 2: one at the origin.
+__global__ __launch_bounds__(256) void k_poly_fill(int mode, View r, u64 len, u64 s_begin,
+                                                   u64 s_len) {
+  const u64 stride = (u64)gridDim.x * blockDim.x;
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
+    bool one = mode == 1;
+    if (mode == 2) one = s_len == 0 ? i == 0 : (s_begin + i) % s_len == 0;
+    vst(r, i, one ? Fr::one() : Fr::zero());
+  }
+}
+
+// poly_basic.cuh:74-82 (one thread)
+__global__ void k_inv_scalar(u32* t) { store_fe<Fr>(t, inv<FrParams>(load_fe<Fr>(t))); }
+
+// field.cuh:1144-1159: t^e, square-and-multiply from the top bit
+__global__ void k_scalar_pow(u32* t, u64 e) {
+  const Fr a = load_fe<Fr>(t);
+  Fr acc = Fr::one();
+  for (int i = 63; i >= 0; i--) {
+    acc = fmul(acc, acc);
+    if ((e >> i) & 1) acc = fmul(acc, a);
+  }
+  store_fe<Fr>(t, acc);
+}
+
+// ------------------------------------------------------------ scan machinery
+constexpr u32 SB = 256;          // threads per work-group
+constexpr u32 SK = 8;            // elements per thread run
+constexpr u32 STILE = SB * SK;   // elements per tile
+constexpr u32 SLOG = 11;         // log2(STILE)
+constexpr u32 SPW = SLOG + 1;    // powers per level: b^(2^e), e = 0..SLOG
+constexpr u32 SMAXL = 6;         // levels (n < 2^64)
+
+enum ScanOp { OP_PROD = 0, OP_INV = 1, OP_HORNER = 2 };
+
+__device__ __forceinline__ Fr shfl_fe(const Fr& a, int src) {
+  Fr r;
+#pragma unroll
+  for (int i = 0; i < 8; i++) r.v[i] = (u32)__shfl((int)a.v[i], src, 64);
+  return r;
+}
+
+__device__ __forceinline__ Fr sel(bool c, const Fr& a, const Fr& b) {
+  Fr r;
+#pragma unroll
+  for (int i = 0; i < 8; i++) r.v[i] = c ? a.v[i] : b.v[i];
+  return r;
+}
+
+// thread run [first, first + SK) of a level; elements past n read as the
+// operator's neutral padding (one for products, zero for the suffix Horner,
+// where trailing zeros change no suffix value)
+template <int OP>
+__device__ __forceinline__ void load_run(const View& v, u64 n, u64 first, Fr x[SK]) {
+#pragma unroll
+  for (u32 j = 0; j < SK; j++) {
+    const u64 e = first + j;
+    x[j] = e < n ? vld(v, e) : (OP == OP_HORNER ? Fr::zero() : Fr::one());
+  }
+}
+
+// batched_invert multiplies with zeros replaced by one (batched_invert.cuh:14-30)
+__device__ __forceinline__ Fr nz(const Fr& x) { return x.is_zero() ? Fr::one() : x; }
+
+template <int OP>
+__device__ __forceinline__ Fr run_summary(const Fr x[SK], const Fr& beta) {
+  Fr h;
+  if (OP == OP_HORNER) {
+    h = x[SK - 1];
+#pragma unroll
+    for (int j = SK - 2; j >= 0; j--) h = fadd(x[j], fmul(beta, h));
+  } else {
+    h = OP == OP_INV ? nz(x[0]) : x[0];
+#pragma unroll
+    for (u32 j = 1; j < SK; j++) h = fmul(h, OP == OP_INV ? nz(x[j]) : x[j]);
+  }
+  return h;
+}
+
+// up pass: run summaries -> lane_sum[tile * SB + t]; tile summary -> tile_sum[tile].
+// pw: this level's powers b^(2^e) (suffix Horner only).
+template <int OP>
+__global__ __launch_bounds__(SB) void k_scan_up(View in, u64 n, u32* lane_sum, u32* tile_sum,
+                                                const u32* pw) {
+  __shared__ uint4 sh[SB * 2];
+  u32* shw = reinterpret_cast<u32*>(sh);
+  const u32 t = threadIdx.x;
+  const u64 tile = blockIdx.x;
+  Fr beta = Fr::zero();
+  if (OP == OP_HORNER) beta = load_fe<Fr>(pw);
+  Fr x[SK];
+  load_run<OP>(in, n, tile * STILE + (u64)t * SK, x);
+  const Fr h = run_summary<OP>(x, beta);
+  store_fe<Fr>(lane_sum + 8 * (tile * SB + t), h);
+  store_fe<Fr>(shw + 8 * t, h);
+  __syncthreads();
+  if (t >= 64) return;
+  const u32 l = t;
+  const Fr s0 = load_fe<Fr>(shw + 8 * (4 * l)), s1 = load_fe<Fr>(shw + 8 * (4 * l + 1));
+  const Fr s2 = load_fe<Fr>(shw + 8 * (4 * l + 2)), s3 = load_fe<Fr>(shw + 8 * (4 * l + 3));
+  Fr T;
+  if (OP == OP_HORNER) {
+    const Fr bk = load_fe<Fr>(pw + 8 * 3);  // b^SK
+    T = fadd(s0, fmul(bk, fadd(s1, fmul(bk, fadd(s2, fmul(bk, s3))))));
+    // lane l (a multiple of 2^(s+1)) accumulates sum_i T_(l+i) (b^(4 SK))^i
+#pragma unroll
+    for (u32 s = 0; s < 6; s++) {
+      const Fr o = shfl_fe(T, (int)((l + (1u << s)) & 63));
+      T = fadd(T, fmul(load_fe<Fr>(pw + 8 * (5 + s)), o));
+    }
+  } else {
+    T = fmul(fmul(s0, s1), fmul(s2, s3));
+#pragma unroll
+    for (u32 s = 0; s < 6; s++) T = fmul(T, shfl_fe(T, (int)(l ^ (1u << s))));
+  }
+  if (l == 0) store_fe<Fr>(tile_sum + 8 * tile, T);
+}
+
+// down pass.  carry (carry2): this tile's carry from the level above, NULL at
+// the top level, where the carry is the neutral one (products: one; suffix
+// Horner: zero; batched_invert: prefix one and, for the inverse of the
+// inclusive prefix, the inverse of the tile's own total).  LEVEL0 writes the
+// operation's result (scan_mul: inclusive prefix products; batched_invert:
+// x^-1, zero kept; kate: exclusive suffix Horner values) through out;
+// otherwise out (out2) receive the carries of the level below: exclusive
+// prefix products (and inverses of inclusive prefix products) or exclusive
+// suffix Horner values.
+template <int OP, bool LEVEL0>
+__global__ __launch_bounds__(SB) void k_scan_down(View in, u64 n, const u32* lane_sum,
+                                                  const u32* carry, const u32* carry2, View out,
+                                                  View out2, const u32* pw) {
+  __shared__ uint4 sh[SB * 2 * (OP == OP_INV ? 2 : 1)];
+  u32* shw = reinterpret_cast<u32*>(sh);
+  const u32 t = threadIdx.x;
+  const u64 tile = blockIdx.x;
+  const u64 first = tile * STILE + (u64)t * SK;
+  Fr x[SK];
+  load_run<OP>(in, n, first, x);
+  store_fe<Fr>(shw + 8 * t, load_fe<Fr>(lane_sum + 8 * (tile * SB + t)));
+  __syncthreads();
+  if (t < 64) {
+    const u32 l = t;
+    const Fr s0 = load_fe<Fr>(shw + 8 * (4 * l)), s1 = load_fe<Fr>(shw + 8 * (4 * l + 1));
+    const Fr s2 = load_fe<Fr>(shw + 8 * (4 * l + 2)), s3 = load_fe<Fr>(shw + 8 * (4 * l + 3));
+    if (OP == OP_HORNER) {
+      const Fr C = carry ? load_fe<Fr>(carry + 8 * tile) : Fr::zero();
+      const Fr bk = load_fe<Fr>(pw + 8 * 3);
+      const Fr b2 = fadd(s2, fmul(bk, s3));
+      const Fr b1 = fadd(s1, fmul(bk, b2));
+      Fr T = fadd(s0, fmul(bk, b1));
+      // the carry enters after the last lane's span of 4 SK elements
+      const Fr Tc = fadd(T, fmul(load_fe<Fr>(pw + 8 * 5), C));
+      T = sel(l == 63, Tc, T);
+#pragma unroll
+      for (u32 s = 0; s < 6; s++) {
+        const Fr o = shfl_fe(T, (int)((l + (1u << s)) & 63));
+        const Fr Ts = fadd(T, fmul(load_fe<Fr>(pw + 8 * (5 + s)), o));
+        T = sel(l + (1u << s) < 64, Ts, T);
+      }
+      Fr X = shfl_fe(T, (int)((l + 1) & 63));
+      X = sel(l == 63, C, X);
+      const Fr e2 = fadd(s3, fmul(bk, X));
+      const Fr e1 = fadd(s2, fmul(bk, e2));
+      const Fr e0 = fadd(s1, fmul(bk, e1));
+      store_fe<Fr>(shw + 8 * (4 * l), e0);
+      store_fe<Fr>(shw + 8 * (4 * l + 1), e1);
+      store_fe<Fr>(shw + 8 * (4 * l + 2), e2);
+      store_fe<Fr>(shw + 8 * (4 * l + 3), X);
+    } else {
+      // forward exclusive products with the carry in front
+      const Fr C = carry ? load_fe<Fr>(carry + 8 * tile) : Fr::one();
+      const Fr a1 = fmul(s0, s1);
+      const Fr a2 = fmul(a1, s2);
+      Fr T = fmul(a2, s3);
+      T = sel(l == 0, fmul(C, T), T);
+#pragma unroll
+      for (u32 s = 0; s < 6; s++) {
+        const Fr o = shfl_fe(T, (int)((l - (1u << s)) & 63));
+        const Fr Ts = fmul(o, T);
+        T = sel(l >= (1u << s), Ts, T);
+      }
+      Fr X = shfl_fe(T, (int)((l - 1) & 63));
+      X = sel(l == 0, C, X);
+      const Fr p1 = fmul(X, s0), p2 = fmul(X, a1), p3 = fmul(X, a2);
+      if (OP == OP_INV) {
+        // suffix exclusive products, times the inverse of the inclusive
+        // prefix at the tile's end
+        Fr IC;
+        if (carry2) {
+          IC = load_fe<Fr>(carry2 + 8 * tile);
+        } else {
+          IC = inv<FrParams>(shfl_fe(T, 63));  // top level: the total's inverse
+        }
+        const Fr c1 = fmul(s2, s3);
+        const Fr c0 = fmul(s1, c1);
+        Fr U = fmul(s0, c0);
+#pragma unroll
+        for (u32 s = 0; s < 6; s++) {
+          const Fr o = shfl_fe(U, (int)((l + (1u << s)) & 63));
+          const Fr Us = fmul(U, o);
+          U = sel(l + (1u << s) < 64, Us, U);
+        }
+        Fr Y = shfl_fe(U, (int)((l + 1) & 63));
+        Y = sel(l == 63, IC, fmul(Y, IC));
+        const Fr q2 = fmul(Y, s3), q1 = fmul(Y, c1), q0 = fmul(Y, c0);
+        u32* sh2 = shw + 8 * SB;
+        store_fe<Fr>(sh2 + 8 * (4 * l), q0);
+        store_fe<Fr>(sh2 + 8 * (4 * l + 1), q1);
+        store_fe<Fr>(sh2 + 8 * (4 * l + 2), q2);
+        store_fe<Fr>(sh2 + 8 * (4 * l + 3), Y);
+      }
+      store_fe<Fr>(shw + 8 * (4 * l), X);
+      store_fe<Fr>(shw + 8 * (4 * l + 1), p1);
+      store_fe<Fr>(shw + 8 * (4 * l + 2), p2);
+      store_fe<Fr>(shw + 8 * (4 * l + 3), p3);
+    }
+  }
+  __syncthreads();
+  const Fr E = load_fe<Fr>(shw + 8 * t);
+  if (OP == OP_HORNER) {
+    const Fr beta = load_fe<Fr>(pw);
+    Fr Q = E;
+#pragma unroll
+    for (int j = SK - 1; j >= 0; j--) {
+      if (first + j < n) vst(out, first + j, Q);
+      if (j > 0) Q = fadd(x[j], fmul(beta, Q));
+    }
+  } else if (OP == OP_PROD) {
+    Fr acc = E;
+#pragma unroll
+    for (u32 j = 0; j < SK; j++) {
+      if (LEVEL0) {
+        acc = fmul(acc, x[j]);
+        if (first + j < n) vst(out, first + j, acc);
+      } else {
+        if (first + j < n) vst(out, first + j, acc);
+        if (j + 1 < SK) acc = fmul(acc, x[j]);
+      }
+    }
+  } else {
+    // p_j = prefix before j, i_j = inverse of the prefix through j
+    Fr p[SK];
+    p[0] = E;
+#pragma unroll
+    for (u32 j = 1; j < SK; j++) p[j] = fmul(p[j - 1], nz(x[j - 1]));
+    Fr I = load_fe<Fr>(shw + 8 * SB + 8 * t);
+#pragma unroll
+    for (int j = SK - 1; j >= 0; j--) {
+      if (first + j < n) {
+        if (LEVEL0) {
+          vst(out, first + j, x[j].is_zero() ? Fr::zero() : fmul(p[j], I));
+        } else {
+          vst(out, first + j, p[j]);
+          vst(out2, first + j, I);
+        }
+      }
+      if (j > 0) I = fmul(I, nz(x[j]));
+    }
+  }
+}
+
+// pw[SPW l + e] = b^(2^(SLOG l + e)) for l < levels, e <= SLOG (one lane)
+__global__ void k_pow2_chain(const u32* b, u32* pw, u32 levels) {
+  Fr v = load_fe<Fr>(b);
+  for (u32 l = 0; l < levels; l++) {
+    for (u32 e = 0; e < SLOG; e++) {
+      store_fe<Fr>(pw + 8 * (SPW * l + e), v);
+      v = fmul(v, v);
+    }
+    store_fe<Fr>(pw + 8 * (SPW * l + SLOG), v);
+  }
+}
+
+__global__ void k_store_zero(u32* r) { store_fe<Fr>(r, Fr::zero()); }
+
+// Temporary-buffer layout of a scan over n elements (n >= 1)
+struct ScanPlan {
+  u32 levels;
+  u64 n[SMAXL];
+  u64 tiles[SMAXL];
+  size_t lane[SMAXL], sum[SMAXL], carry[SMAXL], carry2[SMAXL], pw;
+  size_t bytes;
+};
+
+static void scan_plan(u64 n, int op, ScanPlan* s) {
+  size_t off = 0;
+  u32 l = 0;
+  u64 m = n;
+  for (;;) {
+    const u64 t = (m + STILE - 1) / STILE;
+    s->n[l] = m;
+    s->tiles[l] = t;
+    s->lane[l] = off;
+    off += (size_t)t * SB * 32;
+    s->sum[l] = off;
+    off += (size_t)t * 32;
+    s->carry[l] = s->carry2[l] = 0;
+    if (t > 1) {
+      s->carry[l] = off;
+      off += (size_t)t * 32;
+      if (op == OP_INV) {
+        s->carry2[l] = off;
+        off += (size_t)t * 32;
+      }
+    }
+    l++;
+    if (t == 1) break;
+    m = t;
+  }
+  s->levels = l;
+  s->pw = off;
+  if (op == OP_HORNER) off += (size_t)SPW * l * 32;
+  s->bytes = off;
+}
+
+static size_t scan_bytes(u64 n, int op) {
+  if (n == 0) return 0;
+  ScanPlan s;
+  scan_plan(n, op, &s);
+  return s.bytes;
+}
+
+// Runs the up passes and (down = true) the down passes of a scan of the
+// view `in` (n elements) in the caller's temporary buffer.  base: the
+// Horner base (device, suffix Horner only); top_sum (nullable) receives the
+// whole input's summary.
+template <int OP>
+static hipError_t run_scan(const View& in, u64 n, const View& out, void* temp, size_t temp_bytes,
+                           const u32* base, bool down, u32* top_sum, hipStream_t stream) {
+  ScanPlan s;
+  scan_plan(n, OP, &s);
+  IMP_REQUIRE(temp != nullptr && temp_bytes >= s.bytes, "temporary buffer too small");
+  IMP_REQUIRE(((uintptr_t)temp & 15) == 0, "temporary buffer must be 16-byte aligned");
+  char* T = static_cast<char*>(temp);
+  auto at = [&](size_t off) { return reinterpret_cast<u32*>(T + off); };
+  const u32* pw = nullptr;
+  if (OP == OP_HORNER) {
+    hipLaunchKernelGGL(k_pow2_chain, dim3(1), dim3(1), 0, stream, base, at(s.pw), s.levels);
+    IMP_CHECK(hipGetLastError());
+  }
+  for (u32 l = 0; l < s.levels; l++) {
+    const View vin = l == 0 ? in : flat_view(at(s.sum[l - 1]), s.n[l]);
+    if (OP == OP_HORNER) pw = at(s.pw) + 8 * SPW * l;
+    u32* tsum = (l + 1 == s.levels && top_sum) ? top_sum : at(s.sum[l]);
+    hipLaunchKernelGGL(k_scan_up<OP>, dim3((u32)s.tiles[l]), dim3(SB), 0, stream, vin, s.n[l],
+                       at(s.lane[l]), tsum, pw);
+    IMP_CHECK(hipGetLastError());
+  }
+  if (!down) return hipSuccess;
+  for (int l = (int)s.levels - 1; l >= 0; l--) {
+    const View vin = l == 0 ? in : flat_view(at(s.sum[l - 1]), s.n[l]);
+    if (OP == OP_HORNER) pw = at(s.pw) + 8 * SPW * l;
+    const bool top = l + 1 == (int)s.levels;
+    const u32* c1 = top ? nullptr : at(s.carry[l]);
+    const u32* c2 = (top || OP != OP_INV) ? nullptr : at(s.carry2[l]);
+    if (l == 0) {
+      hipLaunchKernelGGL((k_scan_down<OP, true>), dim3((u32)s.tiles[0]), dim3(SB), 0, stream,
+                         vin, s.n[0], at(s.lane[0]), c1, c2, out, out, pw);
+    } else {
+      const View o1 = flat_view(at(s.carry[l - 1]), s.n[l]);
+      const View o2 = OP == OP_INV ? flat_view(at(s.carry2[l - 1]), s.n[l]) : o1;
+      hipLaunchKernelGGL((k_scan_down<OP, false>), dim3((u32)s.tiles[l]), dim3(SB), 0, stream,
+                         vin, s.n[l], at(s.lane[l]), c1, c2, o1, o2, pw);
+    }
+    IMP_CHECK(hipGetLastError());
+  }
+  return hipSuccess;
+}
+
+// size query / size check of the reference's temp-buffer convention: with
+// temp == NULL the size goes to *size and nothing runs
+static bool size_query(void* temp, unsigned long* size, size_t need, hipError_t* err) {
+  if (temp == nullptr) {
+    if (size == nullptr) {
+      *err = hipErrorInvalidValue;
+      return true;
+    }
+    *size = (unsigned long)need;
+    *err = hipSuccess;
+    return true;
+  }
+  if (size != nullptr && *size < need) {
+    fprintf(stderr, "[impeller_hip] invalid argument: temporary buffer of %lu bytes, need %zu\n",
+            *size, need);
+    *err = hipErrorInvalidValue;
+    return true;
+  }
+  return false;
+}
+
+}  // namespace imp
+
+using namespace imp;
+
+static hipError_t binary_entry(int op, PolyPtr r, ConstPolyPtr a, ConstPolyPtr b,
+                               hipStream_t stream) {
+  if (a.len < b.len) {
+    ConstPolyPtr t = a;
+    a = b;
+    b = t;
+    if (op == EW_SUB) op = EW_RSUB;
+  }
+  const u64 len = b.len, upper = a.len;
+  if (upper == 0) return hipSuccess;
+  View va, vb, vr;
+  IMP_REQUIRE(make_cview(a, &va) && make_view(r, &vr), "bad PolyPtr view");
+  IMP_REQUIRE(len == 0 || make_cview(b, &vb), "bad PolyPtr view");
+  if (len == 0) vb = va;
+  IMP_REQUIRE(r.len >= upper, "result shorter than the longer operand");
+  u32 blocks;
+  IMP_CHECK(ew_grid(upper, &blocks));
+  hipLaunchKernelGGL(k_poly_binary, dim3(blocks), dim3(256), 0, stream, op, va, vb, vr, len,
+                     upper);
+  return hipGetLastError();
+}
+
+// core/src/poly/src/poly.h:6-10 (poly_basic.cu:4-54)
+extern "C" hipError_t poly_add(PolyPtr r, ConstPolyPtr a, ConstPolyPtr b, hipStream_t stream) {
+  return binary_entry(EW_ADD, r, a, b, stream);
+}
+extern "C" hipError_t poly_sub(PolyPtr r, ConstPolyPtr a, ConstPolyPtr b, hipStream_t stream) {
+  return binary_entry(EW_SUB, r, a, b, stream);
+}
+extern "C" hipError_t poly_mul(PolyPtr r, ConstPolyPtr a, ConstPolyPtr b, hipStream_t stream) {
+  return binary_entry(EW_MUL, r, a, b, stream);
+}
+
+static hipError_t fill_entry(int mode, PolyPtr t, u64 s_begin, u64 s_len, hipStream_t stream) {
+  if (t.len == 0) return hipSuccess;
+  View v;
+  IMP_REQUIRE(make_view(t, &v), "bad PolyPtr view");
+  u32 blocks;
+  IMP_CHECK(ew_grid(t.len, &blocks));
+  hipLaunchKernelGGL(k_poly_fill, dim3(blocks), dim3(256), 0, stream, mode, v, (u64)t.len,
+                     s_begin, s_len);
+  return hipGetLastError();
+}
+
+// core/src/poly/src/poly.h:12-16 (poly_basic.cuh:84-120)
+extern "C" hipError_t poly_one_lagrange(PolyPtr target, hipStream_t stream) {
+  return fill_entry(1, target, 0, 0, stream);
+}
+extern "C" hipError_t poly_one_coef(PolyPtr target, hipStream_t stream,
+                                   const PolySlice* slice_info) {
+  if (slice_info == nullptr) return fill_entry(2, target, 0, target.len, stream);
+  return fill_entry(2, target, slice_info->begin, slice_info->len, stream);
+}
+extern "C" hipError_t poly_zero(PolyPtr target, hipStream_t stream) {
+  return fill_entry(0, target, 0, 0, stream);
+}
+
+// core/src/poly/src/poly.h:26-28 (poly_basic.cu:72-80)
+extern "C" hipError_t inv_scalar(unsigned int* target, hipStream_t stream) {
+  IMP_REQUIRE(target != nullptr, "null scalar");
+  hipLaunchKernelGGL(k_inv_scalar, dim3(1), dim3(1), 0, stream, target);
+  return hipGetLastError();
+}
+extern "C" hipError_t scalar_pow(unsigned int* target, unsigned long long exp,
+                                 hipStream_t stream) {
+  IMP_REQUIRE(target != nullptr, "null scalar");
+  hipLaunchKernelGGL(k_scalar_pow, dim3(1), dim3(1), 0, stream, target, (u64)exp);
+  return hipGetLastError();
+}
+
+// core/src/poly/src/poly.h:22 (scan_mul.cuh:6-26): in-place inclusive
+// prefix products of target
+extern "C" hipError_t scan_mul(void* temp_buffer, unsigned long* buffer_size, PolyPtr target,
+                               hipStream_t stream) {
+  const size_t need = scan_bytes(target.len, OP_PROD);
+  hipError_t e;
+  if (size_query(temp_buffer, buffer_size, need, &e)) return e;
+  if (target.len == 0) return hipSuccess;
+  View v;
+  IMP_REQUIRE(make_view(target, &v), "bad PolyPtr view");
+  return run_scan<OP_PROD>(v, target.len, v, temp_buffer, need, nullptr, true, nullptr, stream);
+}
+
+// core/src/poly/src/poly.h:24 (batched_invert.cuh:50-101): every non-zero
+// element replaced by its inverse, zeros kept, in place
+extern "C" hipError_t batched_invert(void* temp_buffer, unsigned long* buffer_size, PolyPtr poly,
+                                     hipStream_t stream) {
+  const size_t need = scan_bytes(poly.len, OP_INV);
+  hipError_t e;
+  if (size_query(temp_buffer, buffer_size, need, &e)) return e;
+  if (poly.len == 0) return hipSuccess;
+  View v;
+  IMP_REQUIRE(make_view(poly, &v), "bad PolyPtr view");
+  return run_scan<OP_INV>(v, poly.len, v, temp_buffer, need, nullptr, true, nullptr, stream);
+}
+
+// core/src/poly/src/poly.h:18 (poly_eval.cuh:7-71): *res = sum_i poly[i] x^i
+// (x and res: one device element each)
+extern "C" hipError_t poly_eval(void* temp_buf, unsigned long* temp_buf_size, ConstPolyPtr poly,
+                                unsigned int* res, const unsigned int* x, hipStream_t stream) {
+  const size_t need = scan_bytes(poly.len, OP_HORNER);
+  hipError_t e;
+  if (size_query(temp_buf, temp_buf_size, need, &e)) return e;
+  IMP_REQUIRE(res != nullptr && x != nullptr, "null scalar");
+  if (poly.len == 0) {
+    hipLaunchKernelGGL(k_store_zero, dim3(1), dim3(1), 0, stream, res);
+    return hipGetLastError();
+  }
+  View v;
+  IMP_REQUIRE(make_cview(poly, &v), "bad PolyPtr view");
+  return run_scan<OP_HORNER>(v, poly.len, v, temp_buf, need, x, false, res, stream);
+}
+
+// core/src/poly/src/poly.h:20 (kate_division.cuh:48-88): q = (p(X) - p(b)) /
+// (X - b) for p of 2^log_p coefficients: q[i] = sum_{j > i} p[j] b^(j-i-1),
+// q[2^log_p - 1] = 0 (halo2 kate_division padded with one zero)
+extern "C" hipError_t kate_division(void* temp_buf, unsigned long* temp_buf_size,
+                                    unsigned int log_p, ConstPolyPtr p, const unsigned int* b,
+                                    PolyPtr q, hipStream_t stream) {
+  IMP_REQUIRE(log_p < 63, "log_p out of range");
+  const u64 len = 1ull << log_p;
+  const size_t need = scan_bytes(len, OP_HORNER);
+  hipError_t e;
+  if (size_query(temp_buf, temp_buf_size, need, &e)) return e;
+  IMP_REQUIRE(p.len == len && q.len == len, "p and q must hold 2^log_p elements");
+  IMP_REQUIRE(b != nullptr, "null scalar");
+  View vp, vq;
+  IMP_REQUIRE(make_cview(p, &vp) && make_view(q, &vq), "bad PolyPtr view");
+  return run_scan<OP_HORNER>(vp, len, vq, temp_buf, need, b, true, nullptr, stream);
+}

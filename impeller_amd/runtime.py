@@ -174,3 +174,94 @@ def to_device(arr, device_id: int = 0):
 
 def to_host(t, cols=FIELD_LIMBS):
     return t.cpu().numpy().view(np.uint32).reshape(-1, cols)
+
+
+class Scalar:
+    """One Fr element on the device (runtime/src/scalar.rs:27-30 Scalar with
+    device = GPU): `value` is a torch tensor of 8 words."""
+
+    def __init__(self, values=None, device_id: int = 0):
+        if values is None:
+            torch = _torch()
+            values = torch.zeros(FIELD_LIMBS, dtype=torch.int32, device=f"cuda:{device_id}")
+        self.values = values
+        self.device = "cuda"
+
+    @classmethod
+    def from_numpy(cls, arr, device_id: int = 0):
+        torch = _torch()
+        a = np.ascontiguousarray(arr, dtype=np.uint32).reshape(FIELD_LIMBS)
+        return cls(torch.from_numpy(a.view(np.int32).copy()).to(f"cuda:{device_id}"))
+
+    def data_ptr(self):
+        return self.values.data_ptr()
+
+    def to_numpy(self):
+        return self.values.cpu().numpy().view(np.uint32).reshape(FIELD_LIMBS)
+
+
+class GpuBuffer:
+    """A device byte buffer (runtime/src/gpu_buffer.rs GpuBuffer)."""
+
+    def __init__(self, size: int, device_id: int = 0):
+        torch = _torch()
+        self.size = size
+        # 256-byte aligned: a torch allocation of at least one word
+        self.values = torch.empty(max(size, 1), dtype=torch.uint8, device=f"cuda:{device_id}")
+
+    def data_ptr(self):
+        return self.values.data_ptr()
+
+
+def _phys_segments(arr: "ScalarArray", start: int, count: int):
+    """Physical (first, length) pieces holding logical elements
+    [start, start + count) of a view (iter.cuh:309-317 index map)."""
+    if arr.slice_info is None:
+        wl = arr.len
+        off = (arr.len - arr.rotate_) % arr.len if arr.rotate_ else 0
+    else:
+        wl, off = arr.slice_info.whole_len, arr.slice_info.offset
+    pieces, p = [], (off + start) % wl if wl else 0
+    while count > 0:
+        k = min(count, wl - p)
+        pieces.append((p, k))
+        count -= k
+        p = 0
+    return pieces
+
+
+def copy_view(src: "ScalarArray", s0: int, count: int, dst: "ScalarArray", d0: int,
+              stream: "Stream"):
+    """Device-to-device copy of logical elements src[s0 : s0 + count] to
+    dst[d0 : d0 + count] on stream (ScalarArray::gpu2gpu of slices,
+    runtime/src/scalar.rs), split where either view wraps."""
+    torch = _torch()
+    a, b = _phys_segments(src, s0, count), _phys_segments(dst, d0, count)
+    with torch.cuda.stream(stream.torch_stream):
+        ia = ib = 0
+        pa, ka = a[0] if a else (0, 0)
+        pb, kb = b[0] if b else (0, 0)
+        while count > 0:
+            k = min(ka, kb)
+            dst.values[8 * pb: 8 * (pb + k)].copy_(src.values[8 * pa: 8 * (pa + k)],
+                                                   non_blocking=True)
+            count -= k
+            pa, ka, pb, kb = pa + k, ka - k, pb + k, kb - k
+            if ka == 0 and count > 0:
+                ia += 1
+                pa, ka = a[ia]
+            if kb == 0 and count > 0:
+                ib += 1
+                pb, kb = b[ib]
+
+
+def copy_elements(values, dst: "ScalarArray", d0: int, stream: "Stream"):
+    """Copies a device tensor of whole elements into logical dst[d0 : ...]."""
+    torch = _torch()
+    n = values.numel() // FIELD_LIMBS
+    with torch.cuda.stream(stream.torch_stream):
+        done = 0
+        for p, k in _phys_segments(dst, d0, n):
+            dst.values[8 * p: 8 * (p + k)].copy_(values[8 * done: 8 * (done + k)],
+                                                 non_blocking=True)
+            done += k

@@ -145,6 +145,55 @@ hipError_t fused_gate_abc_d_qe(const ConstPolyPtr *vars, const PolyPtr *mut_vars
                                unsigned long long len, const PolySlice *slice_info,
                                void *local_buffer, hipStream_t stream);
 
+/* ------- polynomial library: core/src/poly/src/poly.h:6-30 (libpoly.so) -------
+ * Every entry is asynchronous on stream (kernel launches only, so they can
+ * be captured into a HIP graph) and allocates nothing.  Element-wise views
+ * follow PolyPtr / ConstPolyPtr above.  The four with a temporary buffer
+ * keep the reference's convention: with temp == NULL the needed bytes go to
+ * *size and nothing runs; with a buffer (16-byte aligned), size may be NULL
+ * (the runtime passes NULL, core/src/poly.rs:395-410) or hold the buffer's
+ * bytes, which are then checked. */
+
+/* poly.h:6-10 (poly_basic.cu:4-54): r[i] = a[i] op b[i] for i < min(len);
+ * for min(len) <= i < max(len), the longer operand alone (negated when it
+ * is the subtrahend of poly_sub; poly_mul copies it).  r.len >= max(len). */
+hipError_t poly_add(PolyPtr r, ConstPolyPtr a, ConstPolyPtr b, hipStream_t stream);
+hipError_t poly_sub(PolyPtr r, ConstPolyPtr a, ConstPolyPtr b, hipStream_t stream);
+hipError_t poly_mul(PolyPtr r, ConstPolyPtr a, ConstPolyPtr b, hipStream_t stream);
+
+/* poly.h:12-16 (poly_basic.cuh:58-120): one everywhere; one where
+ * slice_info->to_origin(i) == 0 and zero elsewhere (slice_info NULL: the
+ * whole array, i.e. index 0); zero everywhere. */
+hipError_t poly_one_lagrange(PolyPtr target, hipStream_t stream);
+hipError_t poly_one_coef(PolyPtr target, hipStream_t stream, const PolySlice *slice_info);
+hipError_t poly_zero(PolyPtr target, hipStream_t stream);
+
+/* poly.h:18 (poly_eval.cuh): *res = sum_i poly[i] * x^i; x and res are one
+ * device element each; 0 for an empty poly. */
+hipError_t poly_eval(void *temp_buf, unsigned long *temp_buf_size, ConstPolyPtr poly,
+                     unsigned int *res, const unsigned int *x, hipStream_t stream);
+
+/* poly.h:20 (kate_division.cuh:48-88): q = (p(X) - p(b)) / (X - b) for p of
+ * 2^log_p coefficients (p.len == q.len == 2^log_p): q[i] = sum_{j > i}
+ * p[j] b^(j-i-1), q[2^log_p - 1] = 0.  b: one device element.  The size
+ * query reads only log_p. */
+hipError_t kate_division(void *temp_buf, unsigned long *temp_buf_size, unsigned int log_p,
+                         ConstPolyPtr p, const unsigned int *b, PolyPtr q, hipStream_t stream);
+
+/* poly.h:22 (scan_mul.cuh): target[i] := target[0] * ... * target[i], in place. */
+hipError_t scan_mul(void *temp_buffer, unsigned long *buffer_size, PolyPtr target,
+                    hipStream_t stream);
+
+/* poly.h:24 (batched_invert.cuh:50-101): every non-zero element replaced by
+ * its inverse, zeros kept, in place. */
+hipError_t batched_invert(void *temp_buffer, unsigned long *buffer_size, PolyPtr poly,
+                          hipStream_t stream);
+
+/* poly.h:26-28 (poly_basic.cu:72-80): *target := target^-1 (0 stays 0);
+ * *target := target^exp.  One device element, one thread. */
+hipError_t inv_scalar(unsigned int *target, hipStream_t stream);
+hipError_t scalar_pow(unsigned int *target, unsigned long long exp, hipStream_t stream);
+
 /* ======================= device-resident extensions ====================== */
 
 /* The same gate over contiguous device arrays, without the view round trip

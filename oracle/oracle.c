@@ -904,3 +904,162 @@ void orc_gate(uint32_t *out, const uint32_t *a, const uint32_t *b, const uint32_
         store(out + 8 * i, &t1);
     }
 }
+
+/* ------------------------------------------------- polynomial library
+ * core/src/poly/src/poly.h:6-30.  Each result below is unique given its
+ * inputs (field arithmetic is exact), so the restatements follow the
+ * published CPU algorithms the reference's tests compare against
+ * (core/tests/poly.rs:29-470): halo2_proofs kate_division and Horner
+ * evaluation, ff::BatchInvert, and the tests' own prefix-product and
+ * lookup-permutation truths. */
+
+/* poly_basic.cu:4-54: r[i] = a[i] op b[i] for i < min(na, nb); past it the
+ * longer operand alone, negated when it is the subtrahend.  op 0 add,
+ * 1 sub, 2 mul. */
+void orc_poly_binary(int op, uint32_t *r, const uint32_t *a, uint64_t na, const uint32_t *b,
+                     uint64_t nb) {
+    params_init();
+    const fparams *P = &FP[0];
+    uint64_t len = na < nb ? na : nb, upper = na < nb ? nb : na;
+    for (uint64_t i = 0; i < upper; i++) {
+        fe z;
+        if (i < len) {
+            fe x = load(a + 8 * i), y = load(b + 8 * i);
+            if (op == 0) fadd(P, &z, &x, &y);
+            else if (op == 1) fsub(P, &z, &x, &y);
+            else fmul(P, &z, &x, &y);
+        } else if (na >= nb) {
+            z = load(a + 8 * i);
+        } else {
+            fe y = load(b + 8 * i);
+            if (op == 1) fneg(P, &z, &y);
+            else z = y;
+        }
+        store(r + 8 * i, &z);
+    }
+}
+
+/* halo2_proofs::arithmetic::kate_division (called at core/tests/poly.rs:244):
+ * with b' = -b, walking the coefficients from the top, q_i = a_(i+1) - tmp,
+ * tmp = q_i * b'.  The n - 1 quotient coefficients go to q[0..n-1) and
+ * q[n-1] = 0, the reference kernel's length (kate_division.cuh:48-88). */
+void orc_kate_division(const uint32_t *p, uint64_t n, const uint32_t *b, uint32_t *q) {
+    params_init();
+    const fparams *P = &FP[0];
+    if (n == 0) return;
+    fe nb, tmp = {{0, 0, 0, 0}}, bb = load(b);
+    fneg(P, &nb, &bb);
+    fe zero = {{0, 0, 0, 0}};
+    store(q + 8 * (n - 1), &zero);
+    for (uint64_t i = n - 1; i-- > 0;) {
+        fe lead = load(p + 8 * (i + 1));
+        fsub(P, &lead, &lead, &tmp);
+        store(q + 8 * i, &lead);
+        fmul(P, &tmp, &lead, &nb);
+    }
+}
+
+/* ff::BatchInvert::batch_invert (ff 0.13.1, used at core/tests/poly.rs:466):
+ * Montgomery's trick over the non-zero elements; zeros stay zero. */
+void orc_batch_invert(uint32_t *x, uint64_t n) {
+    params_init();
+    const fparams *P = &FP[0];
+    fe *tmp = (fe *)malloc(sizeof(fe) * (n ? n : 1));
+    fe acc = P->r1;
+    for (uint64_t i = 0; i < n; i++) {
+        fe v = load(x + 8 * i);
+        tmp[i] = acc;
+        if (!is_zero(&v)) fmul(P, &acc, &acc, &v);
+    }
+    finv(P, &acc, &acc);
+    for (uint64_t i = n; i-- > 0;) {
+        fe v = load(x + 8 * i);
+        if (is_zero(&v)) continue;
+        fe t;
+        fmul(P, &t, &tmp[i], &acc);
+        fmul(P, &acc, &acc, &v);
+        store(x + 8 * i, &t);
+    }
+    free(tmp);
+}
+
+/* scan_mul.cuh:6-26: inclusive prefix products in place (the truth of
+ * core/tests/poly.rs:412-415 after the wrapper's shift) */
+void orc_scan_mul(uint32_t *x, uint64_t n) {
+    params_init();
+    const fparams *P = &FP[0];
+    if (n == 0) return;
+    fe acc = load(x);
+    for (uint64_t i = 1; i < n; i++) {
+        fe v = load(x + 8 * i);
+        fmul(P, &acc, &acc, &v);
+        store(x + 8 * i, &acc);
+    }
+}
+
+static int cmp_num(const void *pa, const void *pb) {
+    const fe *a = (const fe *)pa, *b = (const fe *)pb;
+    for (int k = 3; k >= 0; k--) {
+        if (a->v[k] != b->v[k]) return a->v[k] < b->v[k] ? -1 : 1;
+    }
+    return 0;
+}
+
+/* The lookup permutation of core/tests/poly.rs:556-640 (the truth the
+ * reference's permute kernel is checked against, permute.cuh): the first
+ * `usable` inputs sorted ascending by canonical value; at the first row of
+ * each run of equal inputs the table takes that value (one copy removed
+ * from the sorted table multiset), and the remaining rows take the leftover
+ * table values in ascending order.  Returns the number of input runs whose
+ * value is missing from the table (such rows then take the input value, no
+ * copy removed); 0 for a valid lookup. */
+uint64_t orc_permute(const uint32_t *input, const uint32_t *table, uint64_t usable,
+                     uint32_t *res_input, uint32_t *res_table) {
+    params_init();
+    const fparams *P = &FP[0];
+    fe one = {{1, 0, 0, 0}};
+    fe *in = (fe *)malloc(sizeof(fe) * (usable ? usable : 1));
+    fe *tb = (fe *)malloc(sizeof(fe) * (usable ? usable : 1));
+    unsigned char *used = (unsigned char *)calloc(usable ? usable : 1, 1);
+    fe *out_t = (fe *)malloc(sizeof(fe) * (usable ? usable : 1));
+    for (uint64_t i = 0; i < usable; i++) {
+        fe a = load(input + 8 * i), t = load(table + 8 * i);
+        fmul(P, &in[i], &a, &one); /* to_number */
+        fmul(P, &tb[i], &t, &one);
+    }
+    qsort(in, usable, sizeof(fe), cmp_num);
+    qsort(tb, usable, sizeof(fe), cmp_num);
+    uint64_t missing = 0, j = 0;
+    for (uint64_t i = 0; i < usable; i++) {
+        if (i > 0 && cmp_num(&in[i], &in[i - 1]) == 0) continue;
+        while (j < usable && cmp_num(&tb[j], &in[i]) < 0) j++;
+        out_t[i] = in[i];
+        if (j < usable && cmp_num(&tb[j], &in[i]) == 0) {
+            /* first unused copy of the value (earlier copies serve earlier runs of
+             * the same value, which cannot exist: runs are distinct values) */
+            used[j] = 1;
+        } else {
+            missing++;
+        }
+    }
+    uint64_t k = 0;
+    for (uint64_t i = 0; i < usable; i++) {
+        if (i == 0 || cmp_num(&in[i], &in[i - 1]) != 0) continue;
+        while (k < usable && used[k]) k++;
+        out_t[i] = k < usable ? tb[k] : in[i];
+        k++;
+    }
+    fe r2 = P->r2;
+    for (uint64_t i = 0; i < usable; i++) {
+        fe a, t;
+        fmul(P, &a, &in[i], &r2); /* from_number */
+        fmul(P, &t, &out_t[i], &r2);
+        store(res_input + 8 * i, &a);
+        store(res_table + 8 * i, &t);
+    }
+    free(in);
+    free(tb);
+    free(used);
+    free(out_t);
+    return missing;
+}

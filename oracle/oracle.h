@@ -85,6 +85,20 @@ void orc_msm_closed_form(const uint32_t *scalars, const uint32_t *k, uint64_t n,
 void orc_gate(uint32_t *out, const uint32_t *a, const uint32_t *b, const uint32_t *c,
               const uint32_t *d, const uint32_t *q, const uint32_t *e, uint64_t n);
 
+/* ---- polynomial library (core/src/poly/src/poly.h:6-30) ---- */
+/* op 0 add, 1 sub, 2 mul over mixed lengths (poly_basic.cu:4-54); r holds max(na, nb) */
+void orc_poly_binary(int op, uint32_t *r, const uint32_t *a, uint64_t na, const uint32_t *b,
+                     uint64_t nb);
+/* halo2 kate_division, padded with q[n-1] = 0 */
+void orc_kate_division(const uint32_t *p, uint64_t n, const uint32_t *b, uint32_t *q);
+/* ff::BatchInvert in place (zeros stay zero) */
+void orc_batch_invert(uint32_t *x, uint64_t n);
+/* inclusive prefix products in place */
+void orc_scan_mul(uint32_t *x, uint64_t n);
+/* lookup permutation (permute.cuh); returns the number of input values missing from the table */
+uint64_t orc_permute(const uint32_t *input, const uint32_t *table, uint64_t usable,
+                     uint32_t *res_input, uint32_t *res_table);
+
 #ifdef __cplusplus
 }
 #endif

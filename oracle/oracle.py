@@ -73,6 +73,11 @@ def _declare(L):
         "orc_best_fft": [_P, _P, ctypes.c_uint32, _INT],
         "orc_powers": [_P, _U64, _P],
         "orc_poly_eval": [_P, _U64, _P, _P],
+        "orc_poly_binary": [_INT, _P, _P, _U64, _P, _U64],
+        "orc_kate_division": [_P, _U64, _P, _P],
+        "orc_batch_invert": [_P, _U64],
+        "orc_scan_mul": [_P, _U64],
+        "orc_permute": [_P, _P, _U64, _P, _P],
         "orc_distribute_powers": [_P, _U64, _U64, _U64, _U64, _P, _U64, _U64, _U64, _U64],
         "orc_xyzz_madd": [_P, _P, _P],
         "orc_xyzz_add": [_P, _P, _P],
@@ -92,6 +97,7 @@ def _declare(L):
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = _INT if name.endswith("on_curve") else None
+    L.orc_permute.restype = _U64
 
 
 def _ptr(a):
@@ -225,6 +231,46 @@ def poly_eval(coeffs, point):
     r = np.zeros(8, dtype=np.uint32)
     lib().orc_poly_eval(_ptr(c), c.shape[0], _ptr(p), _ptr(r))
     return r
+
+
+def poly_binary(op, a, b):
+    """op "add" / "sub" / "mul" over mixed lengths (poly_basic.cu:4-54)."""
+    a = np.ascontiguousarray(a, dtype=np.uint32).reshape(-1, 8)
+    b = np.ascontiguousarray(b, dtype=np.uint32).reshape(-1, 8)
+    r = np.zeros((max(a.shape[0], b.shape[0]), 8), dtype=np.uint32)
+    lib().orc_poly_binary({"add": 0, "sub": 1, "mul": 2}[op], _ptr(r), _ptr(a), a.shape[0],
+                          _ptr(b), b.shape[0])
+    return r
+
+
+def kate_division(p, b):
+    p = np.ascontiguousarray(p, dtype=np.uint32).reshape(-1, 8)
+    b = np.ascontiguousarray(b, dtype=np.uint32)
+    q = np.zeros_like(p)
+    lib().orc_kate_division(_ptr(p), p.shape[0], _ptr(b), _ptr(q))
+    return q
+
+
+def batch_invert(x):
+    x = np.ascontiguousarray(x, dtype=np.uint32).reshape(-1, 8).copy()
+    lib().orc_batch_invert(_ptr(x), x.shape[0])
+    return x
+
+
+def scan_mul(x):
+    x = np.ascontiguousarray(x, dtype=np.uint32).reshape(-1, 8).copy()
+    lib().orc_scan_mul(_ptr(x), x.shape[0])
+    return x
+
+
+def permute(inp, table, usable):
+    """(res_input, res_table, missing) of the lookup permutation."""
+    inp = np.ascontiguousarray(inp, dtype=np.uint32).reshape(-1, 8)
+    table = np.ascontiguousarray(table, dtype=np.uint32).reshape(-1, 8)
+    ri = np.zeros((usable, 8), dtype=np.uint32)
+    rt = np.zeros((usable, 8), dtype=np.uint32)
+    missing = lib().orc_permute(_ptr(inp), _ptr(table), usable, _ptr(ri), _ptr(rt))
+    return ri, rt, missing
 
 
 def powers(base, n):

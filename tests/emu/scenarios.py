@@ -641,6 +641,132 @@ def gate(E):
         assert (out.reshape(-1, 8) == O.gate(*ins)).all(), n
 
 
+def _temp(E, name, *args):
+    size = ctypes.c_ulong(0)
+    E.call(name, None, ctypes.byref(size), *args)
+    return garbage_u32((size.value + 3) // 4 + 4)
+
+
+def _placed(x, rotate=0):
+    """Physical buffer holding logical x under a rotation (iter.cuh:309-317)."""
+    n = x.shape[0]
+    phys = np.zeros_like(x)
+    phys[_logical(None, n, rotate)] = x
+    return flat(phys)
+
+
+POLY_SIZES = (1, 2, 9, 2048, 2049, 5000)
+
+
+def poly_scans(E):
+    """scan_mul, batched_invert and poly_eval on rotated views, one and two
+    levels of tiles, against the oracle."""
+    for n in POLY_SIZES:
+        x = O.gen_scalars(n, 700 + n)
+        rot = 3 % n
+        # scan_mul in place
+        d = _placed(x, rot)
+        tmp = _temp(E, "scan_mul", _view(d, n), None)
+        E.call("scan_mul", ptr(tmp), None, _view(d, n, rot), None)
+        got = d.reshape(-1, 8)[_logical(None, n, rot)]
+        assert (got == O.scan_mul(x)).all(), ("scan_mul", n)
+        # batched_invert with zeros (core/tests/poly.rs:434-440)
+        z = x.copy()
+        z[::17] = 0
+        d = _placed(z, rot)
+        tmp = _temp(E, "batched_invert", _view(d, n), None)
+        E.call("batched_invert", ptr(tmp), None, _view(d, n, rot), None)
+        got = d.reshape(-1, 8)[_logical(None, n, rot)]
+        assert (got == O.batch_invert(z)).all(), ("batched_invert", n)
+        # poly_eval
+        pt = O.gen_scalars(1, 900 + n)[0]
+        d = _placed(x, rot)
+        res = garbage_u32(8)
+        tmp = _temp(E, "poly_eval", _view(d, n), None, None, None)
+        E.call("poly_eval", ptr(tmp), None, _view(d, n, rot), ptr(res), ptr(flat(pt)), None)
+        assert (res == O.poly_eval(x, pt)).all(), ("poly_eval", n)
+    # all zeros and all ones for the inverse
+    for n in (5, 3000):
+        for fill in ("zero", "one"):
+            z = np.zeros((n, 8), np.uint32) if fill == "zero" else np.tile(O.one(), (n, 1))
+            d = flat(z).copy()
+            tmp = _temp(E, "batched_invert", _view(d, n), None)
+            E.call("batched_invert", ptr(tmp), None, _view(d, n), None)
+            assert (d.reshape(-1, 8) == z).all(), ("batched_invert", fill, n)
+
+
+def poly_kate(E):
+    """kate_division on rotated views (core/tests/poly.rs:170-246 rotates p by
+    2 and q by -2) against the oracle."""
+    for k in (0, 1, 3, 11, 12, 13):
+        n = 1 << k
+        p = O.gen_scalars(n, 800 + k)
+        b = O.gen_scalars(1, 850 + k)[0]
+        rp, rq = 2 % n, (n - 2) % n
+        dp = _placed(p, rp)
+        dq = garbage_u32(n * 8)
+        tmp = _temp(E, "kate_division", k, _view(dp, n), None, _view(dq, n), None)
+        E.call("kate_division", ptr(tmp), None, k, _view(dp, n, rp), ptr(flat(b)),
+               _view(dq, n, rq), None)
+        got = dq.reshape(-1, 8)[_logical(None, n, rq)]
+        assert (got == O.kate_division(p, b)).all(), ("kate", k)
+    # a buffer smaller than the query is rejected when its size is given
+    n = 4096
+    size = ctypes.c_ulong(0)
+    E.call("kate_division", None, ctypes.byref(size), 12, _view(u32(8), n), None,
+           _view(u32(8), n), None)
+    small = ctypes.c_ulong(size.value - 32)
+    try:
+        E.call("kate_division", ptr(garbage_u32(size.value // 4)), ctypes.byref(small), 12,
+               _view(u32(n * 8), n), ptr(u32(8)), _view(u32(n * 8), n), None)
+        raise AssertionError("small buffer accepted")
+    except _lib.HipError:
+        pass
+
+
+def poly_elementwise(E):
+    """Mixed-length add / sub / mul on rotated and sliced views, the three
+    constant fills and the scalar inverse / power."""
+    for na, nb in ((1000, 1000), (700, 1000), (1000, 300), (5, 1), (1, 3)):
+        a, b = O.gen_scalars(na, 11 + na), O.gen_scalars(nb, 12 + nb)
+        ra, rb = 10 % na, (nb - 10 % nb) % nb
+        up = max(na, nb)
+        for op in ("add", "sub", "mul"):
+            da, db = _placed(a, ra), _placed(b, rb)
+            whole = up + 7
+            dr = garbage_u32(whole * 8)
+            E.call("poly_" + op, _lib.PolyPtr(dr.ctypes.data + 32 * 0, up, 0, 5, whole),
+                   _view(da, na, ra), _view(db, nb, rb), None)
+            got = dr.reshape(-1, 8)[(5 + np.arange(up)) % whole]
+            assert (got == O.poly_binary(op, a, b)).all(), (op, na, nb)
+    n = 300
+    d = garbage_u32(n * 8)
+    E.call("poly_zero", _view(d, n), None)
+    assert (d == 0).all()
+    E.call("poly_one_lagrange", _view(d, n, 7), None)
+    assert (d.reshape(-1, 8) == O.one()).all()
+    for begin, slen in ((0, n), (37, n), (0, 1000), (650, 1000)):
+        d = garbage_u32(n * 8)
+        sl = _lib.PolySlice(begin, n, slen)
+        E.call("poly_one_coef", _view(d, n), None, ctypes.byref(sl))
+        exp = np.zeros((n, 8), np.uint32)
+        for i in range(n):
+            if (begin + i) % slen == 0:
+                exp[i] = O.one()
+        assert (d.reshape(-1, 8) == exp).all(), (begin, slen)
+    for v in (O.gen_scalars(3, 5)):
+        t = flat(v).copy()
+        E.call("inv_scalar", ptr(t), None)
+        assert (t == O.inv(v)).all()
+        for e in (0, 1, 2, 0xFFFFFFFFFFFFFFFF, 0x123456789ABCDEF):
+            t = flat(v).copy()
+            E.call("scalar_pow", ptr(t), e, None)
+            assert (t == O.pow_(v, L(e))).all(), e
+    t = u32(8)
+    E.call("inv_scalar", ptr(t), None)
+    assert (t == 0).all()
+
+
 ALL = [("single_ntt", single_ntt), ("ntt_plans", ntt_plans),
        ("ntt_fused_reversal", ntt_fused_reversal), ("batched_ntt_mixed", batched_ntt_mixed),
        ("batched_ntt_plans", batched_ntt_plans), ("ntt_graph_replay", ntt_graph_replay),
@@ -648,7 +774,8 @@ ALL = [("single_ntt", single_ntt), ("ntt_plans", ntt_plans),
        ("batched_msm_shared_tables", batched_msm_shared_tables),
        ("batched_msm_degenerate", batched_msm_degenerate), ("ref_ntt_abi", ref_ntt_abi),
        ("gate_ref_abi", gate_ref_abi), ("msm_host_pipeline", msm_host_pipeline),
-       ("ntt_noalloc", ntt_noalloc)]
+       ("ntt_noalloc", ntt_noalloc), ("poly_scans", poly_scans), ("poly_kate", poly_kate),
+       ("poly_elementwise", poly_elementwise)]
 
 
 def main(argv):
