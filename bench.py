@@ -12,7 +12,13 @@ as batched launches replayed from a HIP graph with the cost model's picks
 bit.  Then the fused proving chunk at k = 22 under the same three schedules
 (--no-chunk skips it).
 
+Last, the polynomial library (poly_add / sub / mul, scan_mul,
+batched_invert, poly_eval, kate_division, permute) at 2^24 elements, each
+against its HBM and Montgomery-product rooflines, with sampled or
+whole-array parity checks (--no-poly skips it).
+
   python bench.py [--steps K] [--warmup W] [--window C] [--no-cpu] [--no-batch] [--no-chunk]
+                  [--no-poly]
 """
 import argparse
 import json
@@ -39,6 +45,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched comparisons")
     ap.add_argument("--no-chunk", action="store_true", help="skip the k=22 proving chunk")
+    ap.add_argument("--no-poly", action="store_true", help="skip the polynomial-library ops")
+    ap.add_argument("--log-poly", type=int, default=24, help="log2 size of the poly-op leg")
     return ap.parse_args()
 
 
@@ -65,6 +73,129 @@ def load_picks():
         with open(TUNED) as f:
             return json.load(f)
     return {}
+
+
+def poly_leg(args, stream, timed, mul_peak, with_cpu):
+    """Each op of the polynomial library at 2^log_poly elements: median of 3
+    timed calls, algorithmic bytes (operands read once, results written
+    once) against 8 TB/s and the Fr products its math needs against the
+    live Montgomery-product peak; parity on the same data."""
+    import ctypes
+
+    import torch
+    from impeller_amd import _lib
+    from impeller_amd.runtime import GpuBuffer, ScalarArray, device_u32, to_host
+
+    lg = args.log_poly
+    n = 1 << lg
+
+    def gen(seed):
+        t = device_u32(n * 8)
+        _lib.call("impeller_gen_scalars", t.data_ptr(), n, 0, seed, 0, stream.raw())
+        return ScalarArray(n, t, "cuda")
+
+    a, b, r = gen(501), gen(502), gen(503)
+    pt = device_u32(8)
+    _lib.call("impeller_gen_scalars", pt.data_ptr(), 1, 0, 504, 0, stream.raw())
+    res = device_u32(8)
+    sizes = {}
+
+    def tmp(name, *args_):
+        sz = ctypes.c_ulong(0)
+        _lib.call(name, None, ctypes.byref(sz), *args_)
+        sizes[name] = sz.value
+        return GpuBuffer(sz.value)
+
+    pv = a.poly_ptr()
+    t_scan = tmp("scan_mul", pv, None)
+    t_inv = tmp("batched_invert", pv, None)
+    t_eval = tmp("poly_eval", pv, None, None, None)
+    t_kate = tmp("kate_division", lg, pv, None, pv, None)
+    usable = n - 25
+    t_perm = tmp("permute", usable, pv, pv, pv, pv, None)
+    r_u = ScalarArray(usable, device="cuda")
+    t_u = ScalarArray(usable, device="cuda")
+    src = a.values.clone()
+
+    def restore():
+        r.values.copy_(src)
+
+    t_restore = timed(restore, 3)
+    ops = {
+        "poly_add": (lambda: _lib.call("poly_add", r.poly_ptr(), a.poly_ptr(), b.poly_ptr(),
+                                       stream.raw()), 3, 1, 0),
+        "poly_mul": (lambda: _lib.call("poly_mul", r.poly_ptr(), a.poly_ptr(), b.poly_ptr(),
+                                       stream.raw()), 3, 1, 0),
+        "poly_eval": (lambda: _lib.call("poly_eval", t_eval.data_ptr(), None, a.poly_ptr(),
+                                        res.data_ptr(), pt.data_ptr(), stream.raw()), 1, 1, 0),
+        "kate_division": (lambda: _lib.call("kate_division", t_kate.data_ptr(), None, lg,
+                                            a.poly_ptr(), pt.data_ptr(), b.poly_ptr(),
+                                            stream.raw()), 2, 1, 0),
+        "scan_mul": (lambda: (restore(), _lib.call("scan_mul", t_scan.data_ptr(), None,
+                                                   r.poly_ptr(), stream.raw())), 2, 1, 1),
+        "batched_invert": (lambda: (restore(), _lib.call("batched_invert", t_inv.data_ptr(),
+                                                         None, r.poly_ptr(), stream.raw())),
+                           2, 3, 1),
+        "permute": (lambda: _lib.call("permute", t_perm.data_ptr(), None, usable, a.poly_ptr(),
+                                      b.poly_ptr(), r_u.poly_ptr(), t_u.poly_ptr(),
+                                      stream.raw()), 4, 0, 0),
+    }
+    out = {}
+    for name, (fn, nbytes, nmul, restores) in ops.items():
+        ms = timed(fn, 3) - (t_restore if restores else 0.0)
+        gbs = nbytes * n * 32 / (ms * 1e-3) / 1e9
+        e = {"ms": round(ms, 4), "algorithmic_bytes": nbytes * n * 32, "gb_s": round(gbs, 1),
+             "hbm_frac": round(gbs * 1e9 / (HBM_PEAK_GBS * 1e9), 4)}
+        if nmul:
+            rate = nmul * n / (ms * 1e-3)
+            e.update({"fr_muls": nmul * n, "fr_mul_per_s": round(rate, 1),
+                      "valu_frac": round(rate / mul_peak, 4)})
+        out[name] = e
+    info = {"log_n": lg, "ops": out, "temp_bytes": sizes, "mont_mul_peak": round(mul_peak, 1)}
+    if with_cpu:
+        # the checker leg: parity on the benchmarked data, and the oracle's
+        # single-thread times at 2^20 beside the GPU ones
+        from oracle import oracle as O
+        ah = to_host(a.values)
+        ph = to_host(pt)[0]
+        parity = {}
+        _lib.call("poly_eval", t_eval.data_ptr(), None, a.poly_ptr(), res.data_ptr(),
+                  pt.data_ptr(), stream.raw())
+        stream.sync()
+        parity["poly_eval_full"] = bool((to_host(res)[0] == O.poly_eval(ah, ph)).all())
+        _lib.call("kate_division", t_kate.data_ptr(), None, lg, a.poly_ptr(), pt.data_ptr(),
+                  b.poly_ptr(), stream.raw())
+        stream.sync()
+        qh = to_host(b.values)
+        ok = bool((qh[n - 1] == 0).all())
+        for i in (0, n // 2, n - 2):
+            ok &= bool((qh[i] == O.poly_eval(ah[i + 1:], ph)).all())
+        parity["kate_sampled_quotients"] = ok
+        restore()
+        _lib.call("scan_mul", t_scan.data_ptr(), None, r.poly_ptr(), stream.raw())
+        stream.sync()
+        parity["scan_mul_full"] = bool((to_host(r.values) == O.scan_mul(ah)).all())
+        restore()
+        _lib.call("batched_invert", t_inv.data_ptr(), None, r.poly_ptr(), stream.raw())
+        _lib.call("batched_invert", t_inv.data_ptr(), None, r.poly_ptr(), stream.raw())
+        stream.sync()
+        parity["batched_invert_involution"] = bool(torch.equal(r.values, src))
+        info["parity"] = parity
+        m = 1 << 20
+        xs = O.gen_scalars(m, 9)
+        bb = O.gen_scalars(1, 10)[0]
+        cpu = {}
+        for name, f in (("scan_mul", lambda: O.scan_mul(xs)),
+                        ("batched_invert", lambda: O.batch_invert(xs)),
+                        ("poly_eval", lambda: O.poly_eval(xs, bb)),
+                        ("kate_division", lambda: O.kate_division(xs, bb))):
+            t = time.perf_counter()
+            f()
+            cpu[name] = round((time.perf_counter() - t) * 1e3, 2)
+        info["cpu_port_ms_2^20_1_thread"] = cpu
+    del a, b, r, r_u, t_u, src, t_scan, t_inv, t_eval, t_kate, t_perm
+    torch.cuda.empty_cache()
+    return info
 
 
 def main():
@@ -404,6 +535,11 @@ def main():
         chunk["commitments_equal"] = equal
         del pc
 
+    # ------------------------------------------------- polynomial library
+    poly = None
+    if not args.no_poly:
+        poly = poly_leg(args, stream, timed, rate_fr32, not args.no_cpu)
+
     # ------------------------------------------------ parity + CPU baseline
     parity = dict(batch_parity)
     cpu = None
@@ -478,6 +614,7 @@ def main():
         "batched_vs_plain_speedup": batch["speedup"] if batch else None,
         "batched": batch,
         "proving_chunk": chunk,
+        "poly_ops": poly,
         "kernels_ms": {name: {"total": round(v[0], 3), "launches": v[1]}
                        for name, v in kt.items() if v[1]},
         "roofline": {

@@ -92,6 +92,8 @@ SIGNATURES = {
     "scan_mul": (_ERR, [_P, ctypes.POINTER(ctypes.c_ulong), PolyPtr, _P]),
     "batched_invert": (_ERR, [_P, ctypes.POINTER(ctypes.c_ulong), PolyPtr, _P]),
     "inv_scalar": (_ERR, [_P, _P]),
+    "permute": (_ERR, [_P, ctypes.POINTER(ctypes.c_ulong), ctypes.c_ulong, PolyPtr, PolyPtr,
+                       PolyPtr, PolyPtr, _P]),
     "scalar_pow": (_ERR, [_P, _U64, _P]),
     "impeller_ntt": (_ERR, [_P, _P, _U32, _INT, _P, _P, _U32, _P]),
     "impeller_ntt_workspace_size": (ctypes.c_size_t, [_U32, _P, _U32]),

@@ -1,7 +1,8 @@
 // poly.hip -- the reference's polynomial library over BN254 Fr
 // (core/src/poly/src/poly.h:6-30): mixed-length add / sub / mul, constant
-// fills, scalar inverse and power, and the scan-shaped operations scan_mul,
-// batched_invert, poly_eval and kate_division.
+// fills, scalar inverse and power, the scan-shaped operations scan_mul,
+// batched_invert, poly_eval and kate_division, and the lookup permutation
+// (permute) over a 256-bit-key radix sort.
 //
 // The scan-shaped operations share one two-pass tree.  A work-group takes a
 // tile of STILE = SB * SK consecutive elements; thread t owns the run
@@ -14,6 +15,8 @@
 //   suffix Horner (kate_division, poly_eval): h = sum_j x_j b^j over the run,
 //     combined as h_left + b^len(left) h_right.
 // No work-group waits on another one; the levels are separate launches.
+#include <algorithm>
+
 #include "common.h"
 
 namespace imp {
@@ -360,6 +363,301 @@ __global__ void k_pow2_chain(const u32* b, u32* pw, u32 levels) {
 
 __global__ void k_store_zero(u32* r) { store_fe<Fr>(r, Fr::zero()); }
 
+// ------------------------------------------------ u32 exclusive scan
+// Device-wide exclusive sums of u32 counts, tiles of STILE values (8 per
+// thread), with the same up / down level structure as the Fr scans: the up
+// pass writes tile sums (the next level's values), the down pass scans a
+// tile in place from its carry (the scanned tile sum of the level above).
+__device__ __forceinline__ u32 block_excl_u32(u32 v, u32* sh, u32* total) {
+  const u32 t = threadIdx.x, lane = t & 63, w = t >> 6;
+  u32 inc = v;
+#pragma unroll
+  for (u32 s = 1; s < 64; s <<= 1) {
+    const u32 o = (u32)__shfl((int)inc, (int)((lane - s) & 63), 64);
+    if (lane >= s) inc += o;
+  }
+  if (lane == 63) sh[w] = inc;
+  __syncthreads();
+  u32 before = 0, all = 0;
+#pragma unroll
+  for (u32 k = 0; k < SB / 64; k++) {
+    const u32 x = sh[k];
+    if (k < w) before += x;
+    all += x;
+  }
+  __syncthreads();
+  *total = all;
+  return before + inc - v;
+}
+
+__global__ __launch_bounds__(SB) void k_u32_up(const u32* in, u64 n, u32* tile_sum) {
+  __shared__ u32 sh[SB / 64];
+  const u32 t = threadIdx.x;
+  const u64 first = (u64)blockIdx.x * STILE + (u64)t * SK;
+  u32 s = 0;
+#pragma unroll
+  for (u32 j = 0; j < SK; j++)
+    if (first + j < n) s += in[first + j];
+  u32 total;
+  block_excl_u32(s, sh, &total);
+  if (t == 0) tile_sum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(SB) void k_u32_down(u32* data, u64 n, const u32* carry) {
+  __shared__ u32 sh[SB / 64];
+  const u32 t = threadIdx.x;
+  const u64 first = (u64)blockIdx.x * STILE + (u64)t * SK;
+  u32 x[SK], s = 0;
+#pragma unroll
+  for (u32 j = 0; j < SK; j++) {
+    x[j] = first + j < n ? data[first + j] : 0u;
+    s += x[j];
+  }
+  u32 total;
+  u32 acc = block_excl_u32(s, sh, &total) + (carry ? carry[blockIdx.x] : 0u);
+#pragma unroll
+  for (u32 j = 0; j < SK; j++) {
+    if (first + j < n) data[first + j] = acc;
+    acc += x[j];
+  }
+}
+
+static size_t u32scan_bytes(u64 n) {
+  size_t b = 0;
+  for (u64 m = n; m > STILE;) {
+    m = (m + STILE - 1) / STILE;
+    b += align_up((size_t)m * 4, 256);
+  }
+  return b;
+}
+
+static hipError_t u32scan(u32* data, u64 n, void* temp, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  u32* lv[SMAXL + 1];
+  u64 ln[SMAXL + 1];
+  u32 L = 0;
+  lv[0] = data;
+  ln[0] = n;
+  char* T = static_cast<char*>(temp);
+  while (ln[L] > STILE) {
+    const u64 tiles = (ln[L] + STILE - 1) / STILE;
+    lv[L + 1] = reinterpret_cast<u32*>(T);
+    T += align_up((size_t)tiles * 4, 256);
+    ln[L + 1] = tiles;
+    hipLaunchKernelGGL(k_u32_up, dim3((u32)tiles), dim3(SB), 0, stream, lv[L], ln[L],
+                       lv[L + 1]);
+    IMP_CHECK(hipGetLastError());
+    L++;
+  }
+  for (int l = (int)L; l >= 0; l--) {
+    const u64 tiles = (ln[l] + STILE - 1) / STILE;
+    hipLaunchKernelGGL(k_u32_down, dim3((u32)tiles), dim3(SB), 0, stream, lv[l], ln[l],
+                       l == (int)L ? (const u32*)nullptr : lv[l + 1]);
+    IMP_CHECK(hipGetLastError());
+  }
+  return hipSuccess;
+}
+
+// ------------------------------------------------- 256-bit key radix sort
+// LSD over the 32 bytes of canonical 256-bit numbers (32-byte records),
+// ascending.  Per pass: a digit histogram per tile (digit-major, so one
+// exclusive scan gives every tile's stable output offsets), then a stable
+// scatter ranking each tile's keys round by round: within a wave the keys
+// with the same digit are found by ballots over the digit's bits, across
+// the work-group's waves by per-wave digit counts in LDS.
+constexpr u32 RPASSES = 32;
+
+__device__ __forceinline__ u32 key_digit(const u32* keys, u64 i, u32 pass) {
+  return (keys[8 * i + (pass >> 2)] >> ((pass & 3) * 8)) & 255u;
+}
+
+__global__ __launch_bounds__(SB) void k_rs_hist(const u32* keys, u64 n, u32 pass, u32* cnt,
+                                                u32 tiles) {
+  __shared__ u32 h[256];
+  const u32 t = threadIdx.x;
+  h[t] = 0;
+  __syncthreads();
+#pragma unroll
+  for (u32 j = 0; j < SK; j++) {
+    const u64 i = (u64)blockIdx.x * STILE + (u64)j * SB + t;
+    if (i < n) atomicAdd(&h[key_digit(keys, i, pass)], 1u);
+  }
+  __syncthreads();
+  cnt[(u64)t * tiles + blockIdx.x] = h[t];
+}
+
+__global__ __launch_bounds__(SB) void k_rs_scatter(const u32* kin, u32* kout, u64 n, u32 pass,
+                                                   const u32* off, u32 tiles) {
+  __shared__ u32 base[256];
+  __shared__ u32 wcnt[SB / 64][256];
+  const u32 t = threadIdx.x, lane = t & 63, w = t >> 6;
+  base[t] = off[(u64)t * tiles + blockIdx.x];
+#pragma unroll
+  for (u32 k = 0; k < SB / 64; k++) wcnt[k][t] = 0;
+  __syncthreads();
+  const u64 below = lane ? ((1ull << lane) - 1) : 0ull;
+  for (u32 j = 0; j < SK; j++) {
+    const u64 i = (u64)blockIdx.x * STILE + (u64)j * SB + t;
+    const bool valid = i < n;
+    Fr key = Fr::zero();
+    u32 d = 256;  // lanes past the end form a group of their own
+    if (valid) {
+      key = load_fe<Fr>(kin + 8 * i);
+      d = (key.v[pass >> 2] >> ((pass & 3) * 8)) & 255u;
+    }
+    unsigned long long m = ~0ull;
+#pragma unroll
+    for (u32 b = 0; b < 9; b++) {
+      const unsigned long long bal = __ballot((d >> b) & 1);
+      m &= ((d >> b) & 1) ? bal : ~bal;
+    }
+    const u32 rank = __popcll(m & below);
+    if (valid && rank == 0) wcnt[w][d] = __popcll(m);
+    __syncthreads();
+    if (valid) {
+      u32 pos = base[d] + rank;
+      for (u32 k = 0; k < w; k++) pos += wcnt[k][d];
+      store_fe<Fr>(kout + 8 * (u64)pos, key);
+    }
+    __syncthreads();
+    u32 add = 0;
+#pragma unroll
+    for (u32 k = 0; k < SB / 64; k++) {
+      add += wcnt[k][t];
+      wcnt[k][t] = 0;
+    }
+    base[t] += add;
+    __syncthreads();
+  }
+}
+
+struct SortPlan {
+  u32 tiles;
+  size_t cnt, scan, bytes;  // offsets within the sort's own area
+};
+
+static void sort_plan(u64 n, SortPlan* p) {
+  p->tiles = (u32)((n + STILE - 1) / STILE);
+  p->cnt = 0;
+  p->scan = align_up((size_t)p->tiles * 256 * 4, 256);
+  p->bytes = p->scan + u32scan_bytes((u64)p->tiles * 256);
+}
+
+// sorts n keys starting in a, ping-ponging with b; an even pass count
+// leaves the result in a
+static hipError_t sort256(u32* a, u32* b, u64 n, void* area, hipStream_t stream) {
+  SortPlan p;
+  sort_plan(n, &p);
+  char* A = static_cast<char*>(area);
+  u32* cnt = reinterpret_cast<u32*>(A + p.cnt);
+  for (u32 pass = 0; pass < RPASSES; pass++) {
+    const u32* src = (pass & 1) ? b : a;
+    u32* dst = (pass & 1) ? a : b;
+    hipLaunchKernelGGL(k_rs_hist, dim3(p.tiles), dim3(SB), 0, stream, src, n, pass, cnt,
+                       p.tiles);
+    IMP_CHECK(hipGetLastError());
+    IMP_CHECK(u32scan(cnt, (u64)p.tiles * 256, A + p.scan, stream));
+    hipLaunchKernelGGL(k_rs_scatter, dim3(p.tiles), dim3(SB), 0, stream, src, dst, n, pass, cnt,
+                       p.tiles);
+    IMP_CHECK(hipGetLastError());
+  }
+  return hipSuccess;
+}
+
+// ---------------------------------------------------------- permute
+// permute.cuh semantics: res_input = the usable inputs sorted by canonical
+// value; res_table holds at each run start of res_input that value (one
+// copy taken from the table) and at the other rows the table's leftover
+// values in ascending order.  An input value missing from the table takes
+// its own value at its run start (the reference asserts instead).
+__global__ void k_to_number(View in, u64 n, u32* out) {
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) store_fe<Fr>(out + 8 * i, to_number<FrParams>(vld(in, i)));
+}
+
+__global__ void k_fill_u32(u32* p, u64 n, u32 v) {
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__device__ __forceinline__ int cmp_num(const Fr& a, const Fr& b) {
+#pragma unroll
+  for (int k = 7; k >= 0; k--) {
+    if (a.v[k] != b.v[k]) return a.v[k] < b.v[k] ? -1 : 1;
+  }
+  return 0;
+}
+
+// run starts of the sorted inputs take their value and mark one table copy used
+__global__ void k_perm_lookup(const u32* sa, const u32* st, u64 n, u32* start, u32* avail,
+                              View res_table) {
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Fr a = load_fe<Fr>(sa + 8 * i);
+  const bool s = i == 0 || cmp_num(a, load_fe<Fr>(sa + 8 * (i - 1))) != 0;
+  start[i] = s ? 1u : 0u;
+  if (!s) return;
+  u64 lo = 0, hi = n;
+  while (lo < hi) {
+    const u64 mid = lo + (hi - lo) / 2;
+    const int c = cmp_num(load_fe<Fr>(st + 8 * mid), a);
+    if (c < 0) {
+      lo = mid + 1;
+    } else if (c > 0) {
+      hi = mid;
+    } else {
+      avail[mid] = 0;
+      break;
+    }
+  }
+  vst(res_table, i, from_number<FrParams>(a));
+}
+
+__global__ void k_perm_compact(const u32* st, const u32* avail, const u32* pos, u64 n,
+                               u32* left) {
+  const u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n && avail[j]) store_fe<Fr>(left + 8 * (u64)pos[j], load_fe<Fr>(st + 8 * j));
+}
+
+__global__ void k_perm_fill(const u32* sa, const u32* left, const u32* nstart, u64 n,
+                            View res_input, View res_table) {
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Fr a = load_fe<Fr>(sa + 8 * i);
+  vst(res_input, i, from_number<FrParams>(a));
+  const bool s = i == 0 || cmp_num(a, load_fe<Fr>(sa + 8 * (i - 1))) != 0;
+  // nstart[i]: run starts before row i (exclusive sum), so the row is the
+  // (i - nstart[i])-th repeated row
+  if (!s) vst(res_table, i, from_number<FrParams>(load_fe<Fr>(left + 8 * (i - nstart[i]))));
+}
+
+struct PermPlan {
+  size_t k0, k1, k2, start, avail, pos, sort, scan, bytes;
+};
+
+static void perm_plan(u64 n, PermPlan* p) {
+  size_t off = 0;
+  const size_t keys = align_up((size_t)n * 32, 256), flags = align_up((size_t)n * 4, 256);
+  p->k0 = off;
+  off += keys;
+  p->k1 = off;
+  off += keys;
+  p->k2 = off;
+  off += keys;
+  p->start = off;
+  off += flags;
+  p->avail = off;
+  off += flags;
+  p->pos = off;
+  off += flags;
+  SortPlan sp;
+  sort_plan(n, &sp);
+  p->sort = off;
+  p->scan = off;  // the flag scans reuse the sort's area after both sorts
+  off += std::max(sp.bytes, u32scan_bytes(n));
+  p->bytes = off;
+}
+
 // Temporary-buffer layout of a scan over n elements (n >= 1)
 struct ScanPlan {
   u32 levels;
@@ -608,4 +906,50 @@ extern "C" hipError_t kate_division(void* temp_buf, unsigned long* temp_buf_size
   View vp, vq;
   IMP_REQUIRE(make_cview(p, &vp) && make_view(q, &vq), "bad PolyPtr view");
   return run_scan<OP_HORNER>(vp, len, vq, temp_buf, need, b, true, nullptr, stream);
+}
+
+// core/src/poly/src/poly.h:30 (permute.cuh): the lookup permutation of the
+// first `usable` rows of input and table.  res_input / res_table hold usable
+// elements.  The temp-buffer size depends on usable only.
+extern "C" hipError_t permute(void* temp_buffer, unsigned long* buffer_size, unsigned long usable,
+                              ConstPolyPtr input, ConstPolyPtr table, PolyPtr res_input,
+                              PolyPtr res_table, hipStream_t stream) {
+  PermPlan pp;
+  perm_plan(usable, &pp);
+  hipError_t e;
+  if (size_query(temp_buffer, buffer_size, pp.bytes, &e)) return e;
+  IMP_REQUIRE(input.len == table.len && usable <= input.len, "input and table lengths");
+  IMP_REQUIRE(res_input.len == usable && res_table.len == usable, "result lengths");
+  IMP_REQUIRE(((uintptr_t)temp_buffer & 15) == 0, "temporary buffer must be 16-byte aligned");
+  if (usable == 0) return hipSuccess;
+  View vi, vt, ri, rt;
+  IMP_REQUIRE(make_cview(input, &vi) && make_cview(table, &vt) && make_view(res_input, &ri) &&
+                  make_view(res_table, &rt),
+              "bad PolyPtr view");
+  char* T = static_cast<char*>(temp_buffer);
+  auto at = [&](size_t off) { return reinterpret_cast<u32*>(T + off); };
+  const u64 n = usable;
+  const u32 g = (u32)((n + 255) / 256);
+  hipLaunchKernelGGL(k_to_number, dim3(g), dim3(256), 0, stream, vi, n, at(pp.k0));
+  IMP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_to_number, dim3(g), dim3(256), 0, stream, vt, n, at(pp.k1));
+  IMP_CHECK(hipGetLastError());
+  // sorted inputs end in k0 (ping-pong with k2), sorted table in k1 (with k2)
+  IMP_CHECK(sort256(at(pp.k0), at(pp.k2), n, T + pp.sort, stream));
+  IMP_CHECK(sort256(at(pp.k1), at(pp.k2), n, T + pp.sort, stream));
+  hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(256), 0, stream, at(pp.avail), n, 1u);
+  IMP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_perm_lookup, dim3(g), dim3(256), 0, stream, at(pp.k0), at(pp.k1), n,
+                     at(pp.start), at(pp.avail), rt);
+  IMP_CHECK(hipGetLastError());
+  // positions of the leftover table values, then the run starts before each row
+  IMP_CHECK(hipMemcpyAsync(at(pp.pos), at(pp.avail), n * 4, hipMemcpyDeviceToDevice, stream));
+  IMP_CHECK(u32scan(at(pp.pos), n, T + pp.scan, stream));
+  hipLaunchKernelGGL(k_perm_compact, dim3(g), dim3(256), 0, stream, at(pp.k1), at(pp.avail),
+                     at(pp.pos), n, at(pp.k2));
+  IMP_CHECK(hipGetLastError());
+  IMP_CHECK(u32scan(at(pp.start), n, T + pp.scan, stream));
+  hipLaunchKernelGGL(k_perm_fill, dim3(g), dim3(256), 0, stream, at(pp.k0), at(pp.k2),
+                     at(pp.start), n, ri, rt);
+  return hipGetLastError();
 }

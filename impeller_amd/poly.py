@@ -171,6 +171,25 @@ class PolyInvert:
         return f
 
 
+class PolyPermute:
+    """poly.rs:227-282: mut_var=[res_input, res_table, temp_buf],
+    var=[input, table, stream]; usable = res_input.len."""
+
+    def get_buffer_size(self, usable_len):
+        v = _lib.PolyPtr(None, usable_len, 0, 0, usable_len)
+        return _size_query("permute", usable_len, v, v, v, v, None)
+
+    def get_fn(self):
+        def f(mut_var, var, gpu_mapping=None, poly_slice=None):
+            assert len(mut_var) == 3 and len(var) == 3
+            res_input, res_table, temp = mut_var
+            inp, table, stream = var
+            _gpu(res_input, res_table, inp, table)
+            _lib.call("permute", temp.data_ptr(), None, res_input.len, inp.poly_ptr(),
+                      table.poly_ptr(), res_input.poly_ptr(), res_table.poly_ptr(), stream.raw())
+        return f
+
+
 class ScalarInv:
     """poly.rs:343-378: mut_var=[scalar], var=[stream]."""
 

@@ -194,6 +194,17 @@ hipError_t batched_invert(void *temp_buffer, unsigned long *buffer_size, PolyPtr
 hipError_t inv_scalar(unsigned int *target, hipStream_t stream);
 hipError_t scalar_pow(unsigned int *target, unsigned long long exp, hipStream_t stream);
 
+/* poly.h:30 (permute.cuh): the lookup permutation of the first `usable`
+ * rows: res_input = those inputs sorted ascending by canonical value;
+ * res_table = at each run start of res_input that value (one copy taken
+ * from the table's first `usable` rows), at every other row the table's
+ * leftover values in ascending order.  input.len == table.len >= usable,
+ * res_input.len == res_table.len == usable.  An input value absent from
+ * the table keeps its own value at its run start (the reference asserts). */
+hipError_t permute(void *temp_buffer, unsigned long *buffer_size, unsigned long usable,
+                   ConstPolyPtr input, ConstPolyPtr table, PolyPtr res_input, PolyPtr res_table,
+                   hipStream_t stream);
+
 /* ======================= device-resident extensions ====================== */
 
 /* The same gate over contiguous device arrays, without the view round trip

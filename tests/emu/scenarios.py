@@ -724,6 +724,42 @@ def poly_kate(E):
         pass
 
 
+def poly_permute(E):
+    """permute on lookup-shaped data (core/tests/poly.rs:556-640: table values
+    with duplicates, inputs drawn from the usable table rows), a rotated
+    input view, one and two sort tiles, and a missing input value."""
+    rng = np.random.default_rng(5)
+    for n, usable, spread in ((40, 37, 30), (3000, 2975, 5000), (2100, 2100, 7)):
+        table_ints = rng.integers(0, spread, n)
+        tab = O.ints_to_array([O.to_mont_int(int(v)) for v in table_ints])
+        # large canonical values too: the top limbs take part in the sort
+        tab[::5] = O.gen_scalars(n, 77)[::5]
+        # inputs drawn from the usable table rows
+        inp = tab[rng.integers(0, usable, n)]
+        ei, et, missing = O.permute(inp, tab, usable)
+        assert missing == 0
+        rot = 9 % n
+        di = _placed(inp, rot)
+        dt = flat(tab).copy()
+        ri, rt = garbage_u32(usable * 8), garbage_u32(usable * 8)
+        tmp = _temp(E, "permute", usable, _view(di, n), _view(dt, n), _view(ri, usable),
+                    _view(rt, usable), None)
+        E.call("permute", ptr(tmp), None, usable, _view(di, n, rot), _view(dt, n),
+               _view(ri, usable), _view(rt, usable), None)
+        assert (ri.reshape(-1, 8) == ei).all(), ("permute input", n)
+        assert (rt.reshape(-1, 8) == et).all(), ("permute table", n)
+    inp = O.ints_to_array([O.to_mont_int(v) for v in (9, 2, 2, 9, 4, 4)])
+    tab = O.ints_to_array([O.to_mont_int(v) for v in (1, 2, 3, 4, 5, 6)])
+    ei, et, missing = O.permute(inp, tab, 6)
+    assert missing == 1
+    ri, rt = garbage_u32(48), garbage_u32(48)
+    tmp = _temp(E, "permute", 6, _view(u32(48), 6), _view(u32(48), 6), _view(ri, 6),
+                _view(rt, 6), None)
+    E.call("permute", ptr(tmp), None, 6, _view(flat(inp), 6), _view(flat(tab), 6),
+           _view(ri, 6), _view(rt, 6), None)
+    assert (ri.reshape(-1, 8) == ei).all() and (rt.reshape(-1, 8) == et).all()
+
+
 def poly_elementwise(E):
     """Mixed-length add / sub / mul on rotated and sliced views, the three
     constant fills and the scalar inverse / power."""
@@ -775,7 +811,7 @@ ALL = [("single_ntt", single_ntt), ("ntt_plans", ntt_plans),
        ("batched_msm_degenerate", batched_msm_degenerate), ("ref_ntt_abi", ref_ntt_abi),
        ("gate_ref_abi", gate_ref_abi), ("msm_host_pipeline", msm_host_pipeline),
        ("ntt_noalloc", ntt_noalloc), ("poly_scans", poly_scans), ("poly_kate", poly_kate),
-       ("poly_elementwise", poly_elementwise)]
+       ("poly_elementwise", poly_elementwise), ("poly_permute", poly_permute)]
 
 
 def main(argv):

@@ -257,3 +257,46 @@ def test_graph_capture_replay():
             assert (qd.logical_numpy() == O.kate_division(pk, b)).all(), rep
             assert (idd.logical_numpy() == O.batch_invert(xi)).all(), rep
             assert (ed.to_numpy() == O.poly_eval(pk, pv)).all(), rep
+
+
+def lookup_data(n, usable, seed):
+    """core/tests/poly.rs:556-600: table values rng % (2 len) (duplicates),
+    inputs sampled from the distinct values of the usable table rows; here
+    every fifth table value is a full-width field element as well."""
+    rng = np.random.default_rng(seed)
+    tab = O.ints_to_array([O.to_mont_int(int(v)) for v in rng.integers(0, 2 * n, n)])
+    tab[::5] = O.gen_scalars(n, seed)[::5]
+    inp = tab[rng.integers(0, usable, n)]
+    return inp, tab
+
+
+@pytest.mark.parametrize("k,blinding", [(K, 24), (12, 0), (16, 5)])
+def test_permute(stream, k, blinding):
+    """core/tests/poly.rs:556-680 (K = 20, 24 blinding rows): sorted inputs;
+    each run start matched by one table copy; the other rows take the
+    leftover table values in ascending order."""
+    n = 1 << k
+    usable = n - (blinding + 1) if blinding else n
+    inp, tab = lookup_data(n, usable, k)
+    ei, et, missing = O.permute(inp, tab, usable)
+    assert missing == 0
+    f = P.PolyPermute()
+    tmp = GpuBuffer(f.get_buffer_size(usable))
+    ri, rt = empty(usable), empty(usable)
+    f.get_fn()([ri, rt, tmp], [dev(inp, 3), dev(tab), stream])
+    stream.sync()
+    assert (ri.logical_numpy() == ei).all()
+    assert (rt.logical_numpy() == et).all()
+
+
+def test_permute_missing_value(stream):
+    inp = O.ints_to_array([O.to_mont_int(v) for v in (9, 2, 2, 9, 4, 4)])
+    tab = O.ints_to_array([O.to_mont_int(v) for v in (1, 2, 3, 4, 5, 6)])
+    ei, et, missing = O.permute(inp, tab, 6)
+    assert missing == 1
+    f = P.PolyPermute()
+    tmp = GpuBuffer(f.get_buffer_size(6))
+    ri, rt = empty(6), empty(6)
+    f.get_fn()([ri, rt, tmp], [dev(inp), dev(tab), stream])
+    stream.sync()
+    assert (ri.logical_numpy() == ei).all() and (rt.logical_numpy() == et).all()
