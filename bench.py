@@ -75,6 +75,19 @@ def load_picks():
     return {}
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform string."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
+
+
 def poly_leg(args, stream, timed, mul_peak, with_cpu):
     """Each op of the polynomial library at 2^log_poly elements: median of 3
     timed calls, algorithmic bytes (operands read once, results written
@@ -388,17 +401,28 @@ def main():
     }
 
     # ---------------------------------------------------- batched vs plain
-    # plain: sequential single calls with the configurations used without a
-    # tuner (default_ntt_plan, default_msm); batched: one launch per stage,
-    # replayed from a HIP graph, with the cost model's picks, and for
-    # reference with the exhaustive tuner's picks.  Outputs compared bit for bit.
+    # Configurations come from this run: the reduced candidate space of
+    # every benched problem is timed, and each problem's model pick comes
+    # from a cost model trained (on the GPU) without that problem's rows.
+    # Legs: plain / batched x default / model / exhaustive configurations,
+    # so that the batching gain and the configuration gain show apart.
     from impeller_amd import autotune as A
 
+    run_picks, cm = {}, None
+    if not (args.no_batch and args.no_chunk):
+        ntt_probs = {("ntt_inv", 22), ("ntt_ext", 25), ("ntt_inv", 25)}
+        for kind, lg in WL.ntt_batch_spec():
+            ntt_probs.add({"fwd": ("ntt_fwd", lg), "inv": ("ntt_inv", lg),
+                           "ext": ("ntt_ext", lg + 3)}[kind])
+        msm_probs = set(WL.msm_batch_spec()) | {(22, 0)}
+        del ws
+        torch.cuda.empty_cache()
+        _, run_picks, cm = A.inrun_tune(stream, sorted(ntt_probs), sorted(msm_probs),
+                                        log=lambda m: print(m, file=sys.stderr, flush=True))
+        torch.cuda.empty_cache()
+
     def picked(prob, key):
-        pk = picks.get(prob)
-        if pk is None and prob.startswith("ntt_ext:"):
-            # no zero-extension timings: the plain coset problem's pick
-            pk = picks.get("ntt_coset:" + prob.split(":")[1])
+        pk = run_picks.get(prob)
         return pk.get(key) if pk else None
 
     def ntt_plans(key):
@@ -414,76 +438,88 @@ def main():
     def msm_configs(key):
         out = []
         for lg, kind in WL.msm_batch_spec():
-            cfg = picked(f"msm:{lg}:{kind}", key) if key else None
+            cfg = picked(f"msm:{lg}:{kind}", key)
             out.append(tuple(cfg) if cfg else A.default_msm(lg))
         return out
 
     batch = None
     batch_parity = {}
     if not args.no_batch:
-        del ws
-        torch.cuda.empty_cache()
         # 64 uneven NTTs; every schedule restores its in-place inputs first and
         # the restore time is measured on its own and subtracted
         spec = WL.ntt_batch_spec()
-        ntt_wl = WL.NttWorkload(spec, stream)
-        t_restore = timed(ntt_wl.restore, 2)
-        t_plain_ntt = timed(lambda: (ntt_wl.restore(), ntt_wl.run_plain()), 2) - t_restore
-        plain_out = [it["data"].clone() for it in ntt_wl.items]
-        del ntt_wl
-        t_batch_ntt = {}
+        t_ntt, t_msm = {}, {}
+        ref_out = None
         ntt_equal = True
-        for key in ("model", "exhaustive"):
+        for key in ("default", "model", "exhaustive"):
             wl = WL.NttWorkload(spec, stream, plans=ntt_plans(key))
+            t_restore = timed(wl.restore, 2)
+            if key != "exhaustive":
+                t_ntt[("plain", key)] = timed(lambda: (wl.restore(), wl.run_plain()), 2) - t_restore
+                if ref_out is None:
+                    ref_out = [it["data"].clone() for it in wl.items]
+                ntt_equal &= all(bool(torch.equal(o, it["data"]))
+                                 for o, it in zip(ref_out, wl.items))
             nb_ = wl.batch()
             g = N.Graph(stream, lambda: nb_.run(stream))
-            t_batch_ntt[key] = timed(lambda: (wl.restore(), g.launch(stream)), 2) - t_restore
-            ntt_equal &= all(bool(torch.equal(o, it["data"])) for o, it in zip(plain_out, wl.items))
+            t_ntt[("batched", key)] = timed(lambda: (wl.restore(), g.launch(stream)), 2) - t_restore
+            ntt_equal &= all(bool(torch.equal(o, it["data"])) for o, it in zip(ref_out, wl.items))
             del g, nb_, wl
             torch.cuda.empty_cache()
         batch_parity["ntt_batched_equals_plain"] = ntt_equal
-        del plain_out
+        del ref_out
         torch.cuda.empty_cache()
 
         # 40 commitment MSMs on a shared 2^22 base table
         mspec = WL.msm_batch_spec()
         msm_wl = WL.MsmWorkload(mspec, stream)
-        cfg_plain = msm_configs(None)
-        wss = msm_wl.workspaces(cfg_plain)
-        plain_res = device_u32(16 * len(mspec))
-        msm_wl.run_plain(cfg_plain, plain_res, wss)     # builds the table sets
-        t_plain_msm = timed(lambda: msm_wl.run_plain(cfg_plain, plain_res, wss), 2)
-        del wss
-        torch.cuda.empty_cache()
-        t_batch_msm = {}
+        ref_res = None
         msm_equal = True
-        for key in ("model", "exhaustive"):
-            mb = M.MsmBatch(msm_wl.items(msm_configs(key)), stream)
+        for key in ("default", "model", "exhaustive"):
+            cfgs = msm_configs(key)
+            if key != "exhaustive":
+                wss = msm_wl.workspaces(cfgs)
+                res_p = device_u32(16 * len(mspec))
+                msm_wl.run_plain(cfgs, res_p, wss)     # builds the table sets
+                t_msm[("plain", key)] = timed(lambda: msm_wl.run_plain(cfgs, res_p, wss), 2)
+                if ref_res is None:
+                    ref_res = res_p.clone()
+                msm_equal &= bool(torch.equal(ref_res, res_p))
+                del wss, res_p
+                torch.cuda.empty_cache()
+            mb = M.MsmBatch(msm_wl.items(cfgs), stream)
             g = N.Graph(stream, lambda: mb.run(stream))
-            t_batch_msm[key] = timed(lambda: g.launch(stream), 2)
-            msm_equal &= bool(torch.equal(plain_res, mb.results))
+            t_msm[("batched", key)] = timed(lambda: g.launch(stream), 2)
+            msm_equal &= bool(torch.equal(ref_res, mb.results))
             del g, mb
             torch.cuda.empty_cache()
         batch_parity["msm_batched_equals_plain"] = msm_equal
-        batch = {
-            "ntt64_plain_ms": round(t_plain_ntt, 3),
-            "ntt64_batched_graph_ms": round(t_batch_ntt["model"], 3),
-            "ntt64_batched_exhaustive_ms": round(t_batch_ntt["exhaustive"], 3),
-            "msm40_plain_ms": round(t_plain_msm, 3),
-            "msm40_batched_graph_ms": round(t_batch_msm["model"], 3),
-            "msm40_batched_exhaustive_ms": round(t_batch_msm["exhaustive"], 3),
-            "plain_configs": "default (no tuner)",
-            "batched_configs": "cost model picks" if picks else "default (no picks file)",
-        }
-        batch["ntt64_speedup"] = round(t_plain_ntt / t_batch_ntt["model"], 3)
-        batch["msm40_speedup"] = round(t_plain_msm / t_batch_msm["model"], 3)
-        batch["speedup"] = round((t_plain_ntt + t_plain_msm)
-                                 / (t_batch_ntt["model"] + t_batch_msm["model"]), 3)
-        batch["model_vs_exhaustive"] = round(
-            (t_batch_ntt["model"] + t_batch_msm["model"])
-            / (t_batch_ntt["exhaustive"] + t_batch_msm["exhaustive"]), 3)
-        del msm_wl, plain_res
+
+        def tot(leg, key):
+            return t_ntt[(leg, key)] + t_msm[(leg, key)]
+
+        batch = {"legs_ms": {f"{leg}_{key}": {"ntt64": round(t_ntt[(leg, key)], 3),
+                                              "msm40": round(t_msm[(leg, key)], 3),
+                                              "total": round(tot(leg, key), 3)}
+                             for (leg, key) in t_ntt},
+                 "plain_configs": "default (no tuner)",
+                 "batched_configs": "cost model picks (in-run, held out)"}
+        batch["ntt64_plain_ms"] = round(t_ntt[("plain", "default")], 3)
+        batch["ntt64_batched_graph_ms"] = round(t_ntt[("batched", "model")], 3)
+        batch["msm40_plain_ms"] = round(t_msm[("plain", "default")], 3)
+        batch["msm40_batched_graph_ms"] = round(t_msm[("batched", "model")], 3)
+        batch["ntt64_speedup"] = round(t_ntt[("plain", "default")] / t_ntt[("batched", "model")], 3)
+        batch["msm40_speedup"] = round(t_msm[("plain", "default")] / t_msm[("batched", "model")], 3)
+        # plain-default -> batched-model, and its two factors
+        batch["speedup"] = round(tot("plain", "default") / tot("batched", "model"), 3)
+        batch["batching_gain"] = round(tot("plain", "model") / tot("batched", "model"), 3)
+        batch["config_gain"] = round(tot("batched", "default") / tot("batched", "model"), 3)
+        batch["model_vs_exhaustive"] = round(tot("batched", "model")
+                                             / tot("batched", "exhaustive"), 3)
+        batch["model_training"] = cm["model_training"] if cm else None
+        del msm_wl
         torch.cuda.empty_cache()
+        ws = None
 
     chunk = None
     if not args.no_chunk:
@@ -519,7 +555,7 @@ def main():
         chunk = {"plain_ms": round(t_plain, 2), "plain_config": list(cfg_d)}
         equal = True
         for key in ("model", "exhaustive"):
-            cfg = tuple(picked(f"msm:{pc.k}:0", key) or cfg_d)
+            cfg = tuple(picked(f"msm:{pc.k}:0", key) or cfg_d)  # in-run picks
             tab, stride = chunk_tables(cfg)
             nbs = pc.batches(chunk_plans(key))
             mbc = pc.msm_batch(cfg, tables=tab, table_stride=stride)
@@ -557,28 +593,45 @@ def main():
         for j in (0, 1, n // 3, n - 1):
             ok &= bool((O.poly_eval(xh, mont(pow(w, j, r))) == yh[j]).all())
         parity["ntt_sampled_outputs"] = ok
-        threads = os.cpu_count() or 1
-        kc = min(k, 22)
-        xc = O.gen_scalars(1 << kc, 5)
+        # the checker's CPU path at the plain-path sizes: best_fft 2^k forward
+        # and coset (distribute_powers then best_fft), best_multiexp 2^log_msm
+        # on the same device-generated bases, all host threads
+        threads = O.host_threads()
+        xc = to_host(x_orig)
         t = time.perf_counter()
-        O.best_fft(xc, mont(WL.omega(kc)), kc, threads)
+        cpu_fft = O.best_fft(xc, mont(w), k, threads)
         t_fft = time.perf_counter() - t
-        mc = min(args.log_msm, 18)
-        bc = O.gen_bases(1 << mc, 9, threads=threads)
-        sc = O.gen_scalars(1 << mc, 10)
+        parity["ntt_full_vs_cpu"] = bool((cpu_fft == yh).all())
+        gpow = O.powers(mont(WL.GENERATOR), n)      # a precomputed table, untimed
         t = time.perf_counter()
-        O.best_multiexp(sc, bc, threads)
+        cpu_coset = O.best_fft(O.distribute_powers(xc, gpow), mont(w), k, threads)
+        t_coset = time.perf_counter() - t
+        x.copy_(x_orig)
+        N.ntt(x, tw, k, stream, coset_shift=shift, passes=coset_plan)
+        stream.sync()
+        parity["coset_ntt_full_vs_cpu"] = bool((cpu_coset == to_host(x)).all())
+        bh = to_host(bases, 16)
+        sh = to_host(scal)
+        t = time.perf_counter()
+        cpu_msm = O.best_multiexp(sh, bh, threads)
         t_msm = time.perf_counter() - t
+        parity["msm_cpu_best_multiexp"] = bool((cpu_msm.reshape(-1) == expect.reshape(-1)).all())
         cpu = {
-            "value": round(2 * (1 << kc) * 32 / t_fft / 1e9, 4),
+            "value": round(2 * n * 32 / t_fft / 1e9, 4),
             "unit": "GB/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"best_fft 2^{kc} ({t_fft * 1e3:.0f} ms) and best_multiexp 2^{mc} "
-                      f"({t_msm * 1e3:.0f} ms): C restatement of halo2's CPU path, "
-                      f"{threads} threads",
-            "ntt_ms": round(t_fft * 1e3, 2), "ntt_log": kc,
-            "msm_ms": round(t_msm * 1e3, 2), "msm_log": mc,
+            "cpu_model": cpu_model(),
+            "sample": f"the full plain-path workload on the host: best_fft 2^{k} "
+                      f"({t_fft * 1e3:.0f} ms), coset best_fft 2^{k} ({t_coset * 1e3:.0f} ms), "
+                      f"best_multiexp 2^{args.log_msm} ({t_msm * 1e3:.0f} ms); C restatement "
+                      f"of halo2's CPU path, {threads} threads (the coset's g^i multiply "
+                      f"is one thread)",
+            "ntt_ms": round(t_fft * 1e3, 2), "coset_ntt_ms": round(t_coset * 1e3, 2),
+            "ntt_log": k, "msm_ms": round(t_msm * 1e3, 2), "msm_log": args.log_msm,
+            "gpu_speedup": {"ntt": round(t_fft * 1e3 / ntt_ms, 1),
+                            "coset_ntt": round(t_coset * 1e3 / coset_ms, 1),
+                            "msm": round(t_msm * 1e3 / msm_ms, 1)},
         }
 
     line = {
@@ -613,6 +666,7 @@ def main():
                                    "points and scalars in, affine result out, PCIe included"},
         "batched_vs_plain_speedup": batch["speedup"] if batch else None,
         "batched": batch,
+        "cost_model": cm,
         "proving_chunk": chunk,
         "poly_ops": poly,
         "kernels_ms": {name: {"total": round(v[0], 3), "launches": v[1]}

@@ -344,3 +344,98 @@ def regret_report(db, stream, steps=1500, holdout=True, log=None):
             log(f"[regret] {p}: model {report[p]['model_regret']:.3f} "
                 f"default {report[p]['default_regret'] if d_ms else float('nan'):.3f}")
     return report
+
+
+# ------------------------------------------------- in-run, held-out tuning
+def reduced_ntt_plans(L):
+    """The in-run candidate plans of a 2^L transform: 1024-element LDS
+    tiles, bit reversal fused into the first pass or four levels fused into
+    the bit-reversal kernel, at most three passes, plus the default plan
+    (the shape of every exhaustive best of the full sweep at 2^18..2^26,
+    apart from one four-pass coset plan)."""
+    out = [p for p in ntt_plans(L, max_passes=3) if p[1] == 10 and p[0] in (1, 4)]
+    d = default_ntt_plan(L)
+    if d not in out:
+        out.append(d)
+    return out
+
+
+def reduced_msm_candidates():
+    """Precomputed window tables, windows 10..20, tasks 32 / 64."""
+    return msm_candidates(windows=range(10, 21), targets=(1,), tasks=(32, 64))
+
+
+def default_config(problem):
+    op, lg = problem.split(":")[0], int(problem.split(":")[1])
+    return list(default_msm(lg)) if op == "msm" else default_ntt_plan(lg)
+
+
+def inrun_tune(stream, ntt_problems, msm_problems, folds=5, steps=1500, reps=2, log=None):
+    """Times every reduced candidate of each problem in this run, then picks
+    each problem's configuration with a cost model trained (on the GPU) only
+    on the other problems' rows: the problems are split into `folds` groups
+    and the model for a group never sees that group's timings (a pick
+    without a sweep of the problem).  Returns (db, picks, summary); picks[p]
+    holds the model's, the in-run exhaustive best's and the default's
+    configuration and measured ms."""
+    import time
+
+    from . import _lib
+    from .runtime import device_u32
+    t0 = time.time()
+    db = TimingDB()
+    by_L = {}
+    for op, L in ntt_problems:
+        by_L.setdefault(L, []).append(op)
+    for L in sorted(by_L):
+        tune_ntt(db, L, stream, ops=tuple(sorted(set(by_L[L]))), reps=reps,
+                 plans=reduced_ntt_plans(L))
+        if log:
+            log(f"[inrun] ntt 2^{L} {sorted(set(by_L[L]))}, {time.time() - t0:.1f}s")
+    by_lg = {}
+    for lg, kind in msm_problems:
+        by_lg.setdefault(lg, set()).add(kind)
+    if by_lg:
+        nb = 1 << max(by_lg)
+        bases = device_u32(nb * 16)
+        _lib.call("impeller_gen_bases", bases.data_ptr(), nb, 77, 0, stream.raw())
+        stream.sync()
+        for lg in sorted(by_lg):
+            tune_msm(db, lg, bases, stream, kinds=tuple(sorted(by_lg[lg])),
+                     cands=reduced_msm_candidates(), reps=reps)
+            if log:
+                log(f"[inrun] msm 2^{lg}, {time.time() - t0:.1f}s")
+        del bases
+    t_sweep = time.time() - t0
+    probs = db.problems()
+    picks = {}
+    for f in range(folds):
+        held = [p for i, p in enumerate(probs) if i % folds == f]
+        if not held:
+            continue
+        model = fit_model(db, stream, [r for r in db.rows if r["problem"] not in held], steps)
+        for p in held:
+            rows = db.for_problem(p)
+            best = min(rows, key=lambda r: r["ms"])
+            choice = pick(model, rows, stream)
+            dflt = [r for r in rows if list(r["config"]) == default_config(p)]
+            picks[p] = {"model": list(choice["config"]), "model_ms": choice["ms"],
+                        "exhaustive": list(best["config"]), "exhaustive_ms": best["ms"],
+                        "default": default_config(p),
+                        "default_ms": dflt[0]["ms"] if dflt else None}
+        if log:
+            log(f"[inrun] fold {f}: {len(held)} problems held out, {time.time() - t0:.1f}s")
+    regrets = [v["model_ms"] / v["exhaustive_ms"] - 1.0 for v in picks.values()]
+    dregrets = [v["default_ms"] / v["exhaustive_ms"] - 1.0 for v in picks.values()
+                if v["default_ms"]]
+    summary = {
+        "model_training": f"in-run, held-out ({folds}-fold by problem)",
+        "rows": len(db.rows), "problems": len(probs),
+        "sweep_s": round(t_sweep, 1), "fit_and_pick_s": round(time.time() - t0 - t_sweep, 1),
+        "model_regret_mean": round(float(np.mean(regrets)), 4),
+        "model_regret_max": round(float(np.max(regrets)), 4),
+        "model_within_5pct": round(sum(1 for r in regrets if r <= 0.05) / len(regrets), 3),
+        "default_regret_mean": round(float(np.mean(dregrets)), 4) if dregrets else None,
+        "default_regret_max": round(float(np.max(dregrets)), 4) if dregrets else None,
+    }
+    return db, picks, summary

@@ -5,9 +5,9 @@ timings and evaluate the model's picks.
                                    [--reuse TIMINGS.json] [--sweep all|ntt|msm|none]
 
 Writes DIR/timings.json (every measured configuration), DIR/picks.json
-(exhaustive best and model pick per problem, used by the batched workloads)
-and DIR/report.json (leave-one-problem-out regret of the model vs the
-exhaustive best and vs the library default).  --reuse starts from the rows
+(exhaustive best and the model's leave-one-problem-out pick per problem)
+and DIR/report.json (the regret of those picks vs the exhaustive best and
+of the library default).  --reuse starts from the rows
 of an earlier timing file; --sweep re-measures only the named operation
 (rows of that operation in the reused file are dropped).
 """
@@ -79,19 +79,18 @@ def main():
               f"{time.time() - t0:.1f}s", flush=True)
     db.save(os.path.join(args.out, "timings.json"))
 
-    # cost model: trained on everything for the picks used downstream, and
-    # leave-one-problem-out for the regret report
+    # the model's pick for each problem comes from a model trained without
+    # that problem's rows (leave-one-problem-out): a pick without a sweep
     t1 = time.time()
-    model = A.fit_model(db, stream, steps=args.steps)
-    print(f"[tune] cost model trained on {len(db.rows)} rows, {time.time() - t1:.1f}s", flush=True)
-    for prob in db.problems():
-        choice = A.pick(model, db.for_problem(prob), stream)
-        picks[prob]["model"] = choice["config"]
-        picks[prob]["model_ms"] = choice["ms"]
-    with open(os.path.join(args.out, "picks.json"), "w") as f:
-        json.dump(picks, f, indent=1)
     report = A.regret_report(db, stream, steps=args.steps, holdout=True,
                              log=lambda m: print(m, flush=True))
+    print(f"[tune] leave-one-problem-out picks, {time.time() - t1:.1f}s", flush=True)
+    for prob, rep in report.items():
+        picks[prob]["model"] = rep["model_config"]
+        picks[prob]["model_ms"] = rep["model_ms"]
+        picks[prob]["model_training"] = "leave-one-problem-out"
+    with open(os.path.join(args.out, "picks.json"), "w") as f:
+        json.dump(picks, f, indent=1)
     regrets = [v["model_regret"] for v in report.values()]
     dregrets = [v["default_regret"] for v in report.values() if v["default_regret"] is not None]
     summary = {
