@@ -151,6 +151,12 @@ struct MsmJob {
   u32* result;
   u32 *kA, *vA, *kB, *vB, *hist, *bstart, *bend, *csum, *toff, *loff, *partA, *partB, *rparts,
       *sparts, *sparts1;
+  // bucket sums kept across the parts of one MSM (n_windows x nb XYZZ, or
+  // null): the accumulate stage stores (bmode 1, the first part) or adds
+  // (bmode 2) this part's bucket totals, and the reduce stage of the last
+  // part reads them instead of this part's partial-sum tree
+  u32* bacc;
+  u32 bmode;
 };
 
 static MsmJob make_msm_job(const MsmShape& s, u64 tstride, const u32* bases, const u32* scalars,
@@ -182,6 +188,8 @@ static MsmJob make_msm_job(const MsmShape& s, u64 tstride, const u32* bases, con
   J.rparts = (u32*)(ws + l.rparts);
   J.sparts = (u32*)(ws + l.sparts);
   J.sparts1 = (u32*)(ws + l.sparts1);
+  J.bacc = nullptr;
+  J.bmode = 0;
   return J;
 }
 
@@ -755,6 +763,27 @@ __device__ __forceinline__ void combine_body(const MsmJob& J, u32 level, u64 gi)
   store_xyzz29_g(dst + kPt * (base + k), acc);
 }
 
+// Bucket b (1..nb) of window pw: this part's total (identity for an empty
+// bucket) stored into (bmode 1) or added onto (bmode 2) the kept sums.
+__device__ __forceinline__ void accum_body(const MsmJob& J, u64 i) {
+  const MsmShape& s = J.s;
+  if (J.bacc == nullptr || i >= (u64)s.n_windows * s.nb) return;
+  const u32 pw = (u32)(i / s.nb), b = (u32)(i % s.nb) + 1;
+  const u32* to = J.toff + (u64)pw * (s.nb + 2);
+  const u32 nt = to[b + 1] - to[b];
+  u32* dst = J.bacc + kPt * i;
+  G1Xyzz29 v = G1Xyzz29::identity();
+  if (nt) {
+    const u32* pp = (bucket_levels(nt, s.g) & 1) ? J.partB : J.partA;
+    v = load_xyzz29_g(pp + kPt * ((u64)pw * s.task_cap + to[b]));
+  }
+  if (J.bmode == 1) {
+    store_xyzz29_g(dst, v);
+  } else if (nt) {
+    store_xyzz29_g(dst, xyzz29_add(load_xyzz29_g(dst), v));
+  }
+}
+
 // Window sum S = sum_b b * B_b (b = 1 .. nb) in three shallow stages.
 // Segment s holds buckets m s + 1 .. m s + m (m = seg = 2^sbits):
 //   T_s = sum_k (k + 1) B_(ms+k+1),  R_s = sum_k B_(ms+k+1),
@@ -775,10 +804,14 @@ __device__ __forceinline__ void reduce_body(const MsmJob& J, u64 t) {
   G1Xyzz29 run = G1Xyzz29::identity(), tot = G1Xyzz29::identity();
   for (int k = (int)s.seg - 1; k >= 0; k--) {
     const u32 b = lo + k + 1;
-    const u32 nt = to[b + 1] - to[b];
-    if (nt) {
-      const u32* pp = (bucket_levels(nt, s.g) & 1) ? J.partB : J.partA;
-      run = xyzz29_add(run, load_xyzz29_g(pp + kPt * (pbase + to[b])));
+    if (J.bacc) {
+      run = xyzz29_add(run, load_xyzz29_g(J.bacc + kPt * ((u64)pw * s.nb + b - 1)));
+    } else {
+      const u32 nt = to[b + 1] - to[b];
+      if (nt) {
+        const u32* pp = (bucket_levels(nt, s.g) & 1) ? J.partB : J.partA;
+        run = xyzz29_add(run, load_xyzz29_g(pp + kPt * (pbase + to[b])));
+      }
     }
     tot = xyzz29_add(tot, run);
   }
@@ -918,7 +951,7 @@ __device__ __forceinline__ u32 find_job(const u32* prefix, u32 njobs, u32 t) {
 
 enum MsmStage : u32 {
   MS_DIGITS = 0, MS_HIST, MS_SCAN_A, MS_SCAN_B, MS_SCAN_C, MS_SCATTER, MS_ZERO, MS_BOUNDS,
-  MS_TASKS, MS_COMBINE, MS_REDUCE, MS_SUM, MS_FINAL, MS_CONVERT
+  MS_TASKS, MS_COMBINE, MS_REDUCE, MS_SUM, MS_FINAL, MS_CONVERT, MS_ACCUM
 };
 // scan stage argument: ScanKind | (pass or level) << 8
 static inline u32 scan_arg(u32 kind, u32 x) { return kind | (x << 8); }
@@ -997,6 +1030,9 @@ __global__ __launch_bounds__(256) void kb_combine(const MsmJob* jobs, u32 njobs,
                                                   const u32* prefix, u32 level) {
   IMP_BATCH_LOOP(combine_body(J, level, lb * blockDim.x + threadIdx.x))
 }
+__global__ __launch_bounds__(256) void kb_accum(const MsmJob* jobs, u32 njobs, const u32* prefix) {
+  IMP_BATCH_LOOP(accum_body(J, lb * blockDim.x + threadIdx.x))
+}
 __global__ __launch_bounds__(256) void kb_reduce(const MsmJob* jobs, u32 njobs, const u32* prefix) {
   IMP_BATCH_LOOP(reduce_body(J, lb * blockDim.x + threadIdx.x))
 }
@@ -1067,6 +1103,7 @@ static u32 stage_blocks(const MsmShape& s, u32 kind, u32 arg) {
     case MS_COMBINE:
       return arg <= s.levels ? (u32)cdiv((u64)s.n_windows * combine_cap(s, arg), 256) : 0;
     case MS_REDUCE: return (u32)cdiv((u64)s.n_windows * s.nseg, 256);
+    case MS_ACCUM: return (u32)cdiv((u64)s.n_windows * s.nb, 256);
     case MS_SUM:
       if (arg == 0) return s.n_windows * (s.lbits + 1) * s.g0;
       return s.sum_levels == 2 ? s.n_windows * (s.lbits + 1) : 0u;
@@ -1074,8 +1111,10 @@ static u32 stage_blocks(const MsmShape& s, u32 kind, u32 arg) {
   }
 }
 
+// part: 0 a whole MSM; 1 a part that only accumulates its bucket totals;
+// 2 the last part: accumulates, then reduces the kept sums to the result
 static void plan_stages(const std::vector<MsmShape>& shapes, std::vector<StageLaunch>& out,
-                        std::vector<u32>& prefix) {
+                        std::vector<u32>& prefix, u32 part = 0) {
   u32 maxp = 0, maxl = 0;
   for (auto& s : shapes) {
     if (s.passes > maxp) maxp = s.passes;
@@ -1102,10 +1141,13 @@ static void plan_stages(const std::vector<MsmShape>& shapes, std::vector<StageLa
     scan3(SC_LEVEL, l);
     kinds.push_back({MS_COMBINE, l});
   }
-  kinds.push_back({MS_REDUCE, 0});
-  kinds.push_back({MS_SUM, 0});
-  kinds.push_back({MS_SUM, 1});
-  kinds.push_back({MS_FINAL, 0});
+  if (part) kinds.push_back({MS_ACCUM, 0});
+  if (part != 1) {
+    kinds.push_back({MS_REDUCE, 0});
+    kinds.push_back({MS_SUM, 0});
+    kinds.push_back({MS_SUM, 1});
+    kinds.push_back({MS_FINAL, 0});
+  }
   const u32 n = (u32)shapes.size();
   out.clear();
   prefix.assign(kinds.size() * (n + 1), 0);
@@ -1126,7 +1168,7 @@ static void plan_stages(const std::vector<MsmShape>& shapes, std::vector<StageLa
 static const u32 kTimerOf[] = {K_MSM_DIGITS,  K_MSM_SORT,   K_MSM_SORT, K_MSM_SORT, K_MSM_SORT,
                                K_MSM_SORT,    K_MSM_SCAN,   K_MSM_SCAN, K_MSM_BUCKETS,
                                K_MSM_COMBINE, K_MSM_REDUCE, K_MSM_SUM,  K_MSM_FINAL,
-                               K_MSM_DIGITS};
+                               K_MSM_DIGITS,  K_MSM_COMBINE};
 static u32 stage_timer(const StageLaunch& st) {
   if (st.kind == MS_SCAN_A || st.kind == MS_SCAN_B || st.kind == MS_SCAN_C) {
     const u32 sk = st.arg & 255;
@@ -1146,6 +1188,7 @@ static hipError_t launch_stages(const MsmJob* jobs, u32 njobs, const u32* prefix
     TimedScope ts(stage_timer(st[k]), stream);
     switch (st[k].kind) {
       case MS_CONVERT: hipLaunchKernelGGL(kb_convert, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
+      case MS_ACCUM: hipLaunchKernelGGL(kb_accum, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
       case MS_DIGITS: hipLaunchKernelGGL(kb_digits, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf); break;
       case MS_HIST: hipLaunchKernelGGL(kb_hist, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf, st[k].arg); break;
       case MS_SCAN_A: hipLaunchKernelGGL(kb_scan_a, dim3(grid), dim3(256), 0, stream, jobs, njobs, pf, st[k].arg); break;
@@ -1211,30 +1254,20 @@ static hipError_t precompute_device(u32* tables, u64 N, const MsmShape& s, hipSt
 // (part, batch) scalar range through one of `stage_scalers` buffers, copied
 // on a copy stream while the compute stream runs the MSM of the previous
 // range (events order the reuse of every buffer).  Batches run
-// `batch_per_run` at a time; each (part, batch) MSM leaves an affine partial
-// sum and a last kernel adds the parts of each batch.  Runs alternate the
-// order of the parts, so the part resident at the end of one run is not
-// copied again at the start of the next.  Everything runs on the first card
-// (this build drives one GPU; the other cards' buffer sizes are 0).
-
-// sum of count affine points per output (a batch's part partials) -> affine
-__global__ void k_sum_parts(const u32* partials, u32 count, u32 nout, u32* out) {
-  const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nout) return;
-  G1Xyzz acc = G1Xyzz::identity();
-  for (u32 p = 0; p < count; p++) {
-    const G1Aff a = load_aff(partials + 16 * ((u64)j * count + p));
-    if (!a.is_identity()) acc = xyzz_madd(acc, a);
-  }
-  store_aff(out + 16 * j, xyzz_to_aff(acc, true));
-}
-
+// `batch_per_run` at a time; each batch keeps one set of bucket sums across
+// the parts: every part sorts its digits and sums its buckets, then adds the
+// bucket totals onto the kept set, and only the last part of the run reduces
+// the kept buckets to the batch's result (the reduction tail runs once per
+// batch, not once per part).  Runs alternate the order of the parts, so the
+// part resident at the end of one run is not copied again at the start of
+// the next.  Everything runs on the first card (this build drives one GPU;
+// the other cards' buffer sizes are 0).
 struct HostPlan {
   MsmShape s;      // shape of a full part (part_len points)
   u64 len, part_len;
   u32 parts, bpr, sp, ss;
-  size_t pts_stage, scl_stage, ws_body, op_desc, descs, partials, finals, total;
-  size_t off_pts, off_scl, off_ws, off_desc, off_part, off_fin;
+  size_t pts_stage, scl_stage, ws_body, op_desc, descs, bacc, finals, total;
+  size_t off_pts, off_scl, off_ws, off_desc, off_bacc, off_fin;
 };
 
 static size_t op_desc_bytes() {
@@ -1257,14 +1290,15 @@ static HostPlan host_plan(u32 window, u32 target, u64 len, u32 bpr, u32 parts, u
   h.ws_body = align256(msm_layout(h.s).total);
   h.op_desc = op_desc_bytes();
   h.descs = (size_t)h.parts * h.bpr * h.op_desc;
-  h.partials = align256((size_t)h.bpr * h.parts * 64);
+  // each batch of a run keeps its bucket sums across the parts
+  h.bacc = h.parts > 1 ? align256((size_t)h.s.n_windows * h.s.nb * kPt * 4) : 0;
   h.finals = align256((size_t)h.bpr * 64);
   size_t o = 0;
   h.off_pts = o; o += h.sp * h.pts_stage;
   h.off_scl = o; o += h.ss * h.scl_stage;
   h.off_ws = o; o += h.ws_body;
   h.off_desc = o; o += h.descs;
-  h.off_part = o; o += h.partials;
+  h.off_bacc = o; o += h.bpr * h.bacc;
   h.off_fin = o; o += h.finals;
   h.total = o;
   return h;
@@ -1337,16 +1371,22 @@ static hipError_t msm_host(u32 window, u32 target, void* const* buffers,
   u32 next_pt = 0, next_sc = 0;
   const u32 runs = (batches + h.bpr - 1) / h.bpr;
   std::vector<MsmJob> jobs;
-  std::vector<StageLaunch> stages_full, stages_last;
-  std::vector<u32> pref_full, pref_last;
+  // stage lists by [the part's shape: full, or the shorter last range]
+  // [what it does: a whole MSM (one part), accumulate only, accumulate and
+  // finish] (the reference keeps its bucket sums across parts and reduces
+  // once, msm_impl.cuh:373-580)
+  std::vector<StageLaunch> stl[2][3];
+  std::vector<u32> pfl[2][3];
   {
     std::vector<MsmShape> one(1, h.s);
-    plan_stages(one, stages_full, pref_full);
     const u64 last_len = h.len - (u64)(h.parts - 1) * h.part_len;
-    one[0] = msm_shape(last_len, window, target);
-    plan_stages(one, stages_last, pref_last);
-    IMP_REQUIRE(pref_full.size() <= kMaxStages * 2 && pref_last.size() <= kMaxStages * 2,
-                "stage plan");
+    for (u32 sh = 0; sh < 2; sh++) {
+      one[0] = sh ? msm_shape(last_len, window, target) : h.s;
+      for (u32 m = 0; m < 3; m++) {
+        plan_stages(one, stl[sh][m], pfl[sh][m], m);
+        IMP_REQUIRE(pfl[sh][m].size() <= kMaxStages * 2, "stage plan");
+      }
+    }
   }
   for (u32 r = 0; r < runs; r++) {
     const u32 b0 = r * h.bpr, nb = batches - b0 < h.bpr ? batches - b0 : h.bpr;
@@ -1376,17 +1416,24 @@ static hipError_t msm_host(u32 window, u32 target, void* const* buffers,
       }
     }
     jobs.assign(ops.size(), MsmJob());
+    std::vector<u32> mode(ops.size(), 0);
     char* desc = buf + h.off_desc;
+    const u32 first_part = ops.empty() ? 0 : ops.front().part;
+    const u32 last_part = ops.empty() ? 0 : ops.back().part;
     for (size_t o = 0; o < ops.size(); o++) {
       const Op& op = ops[o];
       const u64 cur = op.part + 1 == h.parts ? h.len - (u64)op.part * h.part_len : h.part_len;
       const MsmShape s = msm_shape(cur, window, target);
-      u32* result = h.parts == 1 ? (u32*)(buf + h.off_fin) + 16 * op.batch
-                                 : (u32*)(buf + h.off_part) + 16 * ((u64)op.batch * h.parts + op.part);
+      u32* result = (u32*)(buf + h.off_fin) + 16 * op.batch;
       // scalar stage pointers are patched below (the stage is picked when
       // the copy is issued, in the same order)
       jobs[o] = make_msm_job(s, h.part_len, (const u32*)(buf + h.off_pts + op.pstage * h.pts_stage),
                              nullptr, result, buf + h.off_ws);
+      if (h.parts > 1) {
+        jobs[o].bacc = (u32*)(buf + h.off_bacc + op.batch * h.bacc);
+        jobs[o].bmode = op.part == first_part ? 1u : 2u;
+        mode[o] = op.part == last_part ? 2u : 1u;
+      }
     }
     {
       u32 ns = next_sc;
@@ -1398,7 +1445,7 @@ static hipError_t msm_host(u32 window, u32 target, void* const* buffers,
     for (size_t o = 0; o < ops.size(); o++) {
       char* d = desc + o * h.op_desc;
       IMP_CHECK(hipMemcpyAsync(d, &jobs[o], sizeof(MsmJob), hipMemcpyHostToDevice, g.st));
-      const std::vector<u32>& pf = ops[o].part + 1 == h.parts ? pref_last : pref_full;
+      const std::vector<u32>& pf = pfl[ops[o].part + 1 == h.parts ? 1 : 0][mode[o]];
       IMP_CHECK(hipMemcpyAsync(d + align256(sizeof(MsmJob)), pf.data(), pf.size() * 4,
                                hipMemcpyHostToDevice, g.st));
     }
@@ -1440,19 +1487,14 @@ static hipError_t msm_host(u32 window, u32 target, void* const* buffers,
       IMP_CHECK(hipEventRecord(sc_ready[sk], g.cp));
       IMP_CHECK(hipStreamWaitEvent(g.st, sc_ready[sk], 0));
       char* d = desc + o * h.op_desc;
-      const std::vector<StageLaunch>& stl = op.part + 1 == h.parts ? stages_last : stages_full;
+      const std::vector<StageLaunch>& sl = stl[op.part + 1 == h.parts ? 1 : 0][mode[o]];
       IMP_CHECK(launch_stages((const MsmJob*)d, 1, (const u32*)(d + align256(sizeof(MsmJob))),
-                              stl.data(), (u32)stl.size(), 0, g.st));
+                              sl.data(), (u32)sl.size(), 0, g.st));
       IMP_CHECK(hipEventRecord(sc_free[sk], g.st));
       const bool last_of_part = o + 1 == ops.size() || ops[o + 1].part != op.part;
       if (last_of_part) IMP_CHECK(hipEventRecord(pt_free[k], g.st));
     }
     u32* fin = (u32*)(buf + h.off_fin);
-    if (h.parts > 1) {
-      hipLaunchKernelGGL(k_sum_parts, dim3((nb + 63) / 64), dim3(64), 0, g.st,
-                         (const u32*)(buf + h.off_part), h.parts, nb, fin);
-      IMP_CHECK(hipGetLastError());
-    }
     for (u32 j = 0; j < nb; j++)
       IMP_CHECK(hipMemcpyAsync(h_result[b0 + j], fin + 16 * j, 64, hipMemcpyDeviceToHost, g.st));
     IMP_CHECK(hipStreamSynchronize(g.st));
