@@ -211,6 +211,58 @@ def poly_leg(args, stream, timed, mul_peak, with_cpu):
     return info
 
 
+def fused_leg(stream, timed, log_n=24):
+    """The C5 gate emitted by impeller_amd.kernel_gen (built beforehand into
+    impeller_amd/_fused) against the hand-written gate kernel, through the
+    same generated-kernel signature, at 2^log_n elements: 7 x 32 B per
+    element of algorithmic traffic."""
+    import ctypes
+
+    import numpy as np
+    import torch
+    from impeller_amd import _lib
+    from impeller_amd import kernel_gen as KG
+    from impeller_amd import workloads as WL
+    from impeller_amd.runtime import ScalarArray, device_u32
+
+    op = WL.c5_gate_op()
+    kern = KG.FusedKernel(op.meta(WL.FUSED_DIR))
+    n = 1 << log_n
+    arrs = []
+    for i in range(7):
+        t = device_u32(n * 8)
+        _lib.call("impeller_gen_scalars", t.data_ptr(), n, 0, 900 + i, 0, stream.raw())
+        arrs.append(ScalarArray(n, t, "cuda"))
+    one = torch.from_numpy(WL.mont_limbs(1).view(np.int32).copy()).cuda()  # the scalar 1
+    ptrs = [a.poly_ptr() for a in arrs[:6]] + [_lib.PolyPtr(one.data_ptr(), 1, 0, 0, 1),
+                                               arrs[6].poly_ptr()]
+    host = (_lib.PolyPtr * 8)(*ptrs)
+    argb = torch.from_numpy(np.frombuffer(bytes(host), dtype=np.uint8).copy()).cuda()
+    sl = _lib.PolySlice(0, n, n)
+    base = argb.data_ptr()
+    emitted = lambda: _lib.check(op.name, kern.fn(base, base + 7 * 40, n, ctypes.byref(sl),  # noqa
+                                                  None, stream.raw()))
+    # hand-written: vars = the six arrays, mut_vars = the output
+    host2 = (_lib.PolyPtr * 7)(*([a.poly_ptr() for a in arrs[:6]] + [arrs[6].poly_ptr()]))
+    argb2 = torch.from_numpy(np.frombuffer(bytes(host2), dtype=np.uint8).copy()).cuda()
+    b2 = argb2.data_ptr()
+    hand = lambda: _lib.call("fused_gate_abc_d_qe", b2, b2 + 6 * 40, n, ctypes.byref(sl),  # noqa
+                             None, stream.raw())
+    t_e = timed(emitted, 5)
+    out_e = arrs[6].values.clone()
+    t_h = timed(hand, 5)
+    same = bool(torch.equal(out_e, arrs[6].values))
+    nbytes = 7 * n * 32
+    res = {"log_n": log_n, "emitted_ms": round(t_e, 4), "handwritten_ms": round(t_h, 4),
+           "emitted_gb_s": round(nbytes / (t_e * 1e-3) / 1e9, 1),
+           "emitted_hbm_frac": round(nbytes / (t_e * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "handwritten_gb_s": round(nbytes / (t_h * 1e-3) / 1e9, 1),
+           "outputs_equal": same}
+    del arrs, argb, argb2
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -582,8 +634,10 @@ def main():
 
     # ------------------------------------------------- polynomial library
     poly = None
+    fused = None
     if not args.no_poly:
         poly = poly_leg(args, stream, timed, rate_fr32, not args.no_cpu)
+        fused = fused_leg(stream, timed)
 
     # ------------------------------------------------ parity + CPU baseline
     parity = dict(batch_parity)
@@ -685,6 +739,7 @@ def main():
         "cost_model": cm,
         "proving_chunk": chunk,
         "poly_ops": poly,
+        "fused_emitter": fused,
         "kernels_ms": {name: {"total": round(v[0], 3), "launches": v[1]}
                        for name, v in kt.items() if v[1]},
         "roofline": {

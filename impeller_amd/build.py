@@ -46,15 +46,18 @@ if not os.path.exists(READELF):
     READELF = shutil.which("llvm-readelf") or "/opt/rocm/llvm/bin/llvm-readelf"
 
 
-def _device_functions(src):
+def _device_functions(src, include_dirs=()):
     """Out-of-line device functions (FUNC symbols without a kernel descriptor)
     in the gfx950 code object of one source.  There must be none: in a callee
     the compiler's long-branch expansion has been seen to reuse the return
     address registers s[30:31] without saving them, so a wave that takes such
     a branch never returns.  Every device function here is force-inlined and
     this check keeps it that way."""
-    obj = os.path.join(OBJDIR, os.path.basename(src) + ".dev.o")
-    cmd = [HIPCC, *COMMON, "--cuda-device-only", "--no-gpu-bundle-output", "-c", src, "-o", obj]
+    obj = os.path.join(OBJDIR if not include_dirs else os.path.dirname(src),
+                       os.path.basename(src) + ".dev.o")
+    inc = [a for d in include_dirs for a in ("-I", d)]
+    cmd = [HIPCC, *COMMON, *inc, "--cuda-device-only", "--no-gpu-bundle-output", "-c", src,
+           "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc (device object) failed for {src}:\n{r.stderr}")

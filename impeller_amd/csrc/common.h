@@ -81,6 +81,18 @@ __device__ __forceinline__ void gst(T* p, T v) {
 }
 #endif
 
+// A PolyPtr read from device memory as a view (make_slice_iter: a non-zero
+// rotate means offset = len - rotate on an unsliced array)
+__device__ __forceinline__ View dev_view(const PolyPtr* p) {
+  const PolyPtr v = gld(p);
+  View r;
+  r.base = v.ptr;
+  const u64 wl = v.whole_len ? v.whole_len : v.len;
+  r.wl = wl ? wl : 1;
+  r.off = v.rotate != 0 ? v.len - v.rotate : v.offset;
+  return r;
+}
+
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // The launch's dynamic LDS as a u32 pointer (one definition per kernel).

@@ -35,18 +35,6 @@ __global__ __launch_bounds__(256) void k_gate(GateViews g, u64 len) {
   for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) gate_at(g, i);
 }
 
-// A PolyPtr read from device memory as a view (make_slice_iter: a non-zero
-// rotate means offset = len - rotate on an unsliced array)
-__device__ __forceinline__ View dev_view(const PolyPtr* p) {
-  const PolyPtr v = gld(p);
-  View r;
-  r.base = v.ptr;
-  const u64 wl = v.whole_len ? v.whole_len : v.len;
-  r.wl = wl ? wl : 1;
-  r.off = v.rotate != 0 ? v.len - v.rotate : v.offset;
-  return r;
-}
-
 // The reference calling convention: the views live in device memory
 // (fused_kernels.rs:172-188 copies them there), so each thread reads them.
 __global__ __launch_bounds__(256) void k_gate_dev(const ConstPolyPtr* vars, const PolyPtr* mut_vars,

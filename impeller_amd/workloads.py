@@ -5,6 +5,7 @@ batched NTT: 64 polynomials with seeded log-sizes in 18..23 and seeded kinds
   as 64 sequential single calls or as one batched launch per stage (optionally
   replayed from a HIP graph).
 """
+import os
 import random
 
 import numpy as np
@@ -294,3 +295,33 @@ class ProvingChunk:
     def _gate_batched(self):
         from .fused import gate_device
         gate_device(self.ext[:6], self.h, 1 << self.K, self.stream)
+
+
+# ----------------------------------------------------- emitted C5 gate
+FUSED_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_fused")
+
+
+def c5_gate_graph():
+    """The proving chunk's gate a*b*c - d + q*e*(1 - q) as an ArithGraph
+    (Lagrange form, the constant one a scalar input), for the emitter."""
+    from . import kernel_gen as KG
+    g = KG.ArithGraph(KG.LAGRANGE)
+    a, b, c, d, q, e = (g.input(KG.ARRAY) for _ in range(6))
+    one = g.input(KG.SCALAR)
+    t = g.binop("pp", "mul", g.binop("pp", "mul", a, b), c)
+    t = g.binop("pp", "sub", t, d)
+    u = g.binop("pp", "mul", g.binop("pp", "mul", q, e), g.binop("sp", "sub", one, q))
+    g.output(KG.ARRAY, g.binop("pp", "add", t, u))
+    return g
+
+
+def c5_gate_op():
+    from . import kernel_gen as KG
+    return KG.FusedOp(c5_gate_graph(), "emitted_gate_abc_d_qe")
+
+
+def build_fused():
+    """Emits and builds the workloads' fused kernels (run beforehand by the
+    library build; the bench loads the result)."""
+    from . import kernel_gen as KG
+    return KG.build([c5_gate_op()], FUSED_DIR)

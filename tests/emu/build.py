@@ -65,6 +65,30 @@ def build(asan=False, verbose=False):
     return out
 
 
+def build_extra(srcs, out, include_dirs=()):
+    """Compiles extra HIP sources (e.g. emitted fused kernels) with the
+    emulation runtime into their own shared object (its own copy of the
+    launch emulation; kernels launched from it run on the host)."""
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    flags = ["-std=c++17", "-O1", "-fPIC", "-I", HERE, "-DIMP_EMU", "-Wno-unused-result",
+             "-Wno-unknown-attributes", "-I", CSRC]
+    for d in include_dirs:
+        flags += ["-I", d]
+    objs = []
+    for src in list(srcs) + [os.path.join(HERE, "emu.cpp")]:
+        obj = os.path.join(os.path.dirname(out), os.path.basename(src) + ".emu.o")
+        r = subprocess.run([CLANG, *flags, "-x", "c++", "-c", src, "-o", obj],
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"emu compile failed for {src}:\n{r.stderr[-4000:]}")
+        objs.append(obj)
+    r = subprocess.run([CLANG, "-shared", "-fPIC", *objs, "-o", out], capture_output=True,
+                       text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"emu link failed:\n{r.stderr[-4000:]}")
+    return out
+
+
 def asan_runtime():
     r = subprocess.run([CLANG, "-print-file-name=libclang_rt.asan-x86_64.so"],
                        capture_output=True, text=True)
