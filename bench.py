@@ -19,6 +19,8 @@ whole-array parity checks (--no-poly skips it).
 
   python bench.py [--steps K] [--warmup W] [--window C] [--no-cpu] [--no-batch] [--no-chunk]
                   [--no-poly]
+  python bench.py --profile ntt|msm|poly   (the rocprofv3 harness: a calibration
+                                            copy, then a few calls of one path)
 """
 import argparse
 import json
@@ -47,6 +49,10 @@ def parse():
     ap.add_argument("--no-chunk", action="store_true", help="skip the k=22 proving chunk")
     ap.add_argument("--no-poly", action="store_true", help="skip the polynomial-library ops")
     ap.add_argument("--log-poly", type=int, default=24, help="log2 size of the poly-op leg")
+    ap.add_argument("--profile", choices=("ntt", "msm", "poly"), default=None,
+                    help="profiling harness for rocprofv3: one calibration copy of a known "
+                         "size, then a few calls of the named path, nothing else")
+    ap.add_argument("--plan", default="", help="--profile ntt: pass plan, e.g. 1,10,8,8,8")
     return ap.parse_args()
 
 
@@ -73,6 +79,75 @@ def load_picks():
         with open(TUNED) as f:
             return json.load(f)
     return {}
+
+
+def profile_main(args):
+    """Runs under rocprofv3 (e.g. `rocprofv3 --pmc FETCH_SIZE -- python3 bench.py
+    --profile msm`): a device copy of a known byte count (the calibration of
+    the HBM counters FETCH_SIZE / WRITE_SIZE), then the named path a few
+    times -- NTT 2^log_ntt forward and coset with the tuned plans, MSM
+    2^log_msm with precomputed tables (c = --window or 20, task 64), or every
+    polynomial-library op at 2^log_poly.  Prints one JSON line of times."""
+    import torch
+    from impeller_amd import _lib
+    from impeller_amd import msm as M
+    from impeller_amd import ntt as N
+    from impeller_amd import workloads as WL
+    from impeller_amd.runtime import Stream, device_u32, to_device
+
+    stream = Stream.current(0)
+
+    def timed1(fn):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        b.synchronize()
+        return round(a.elapsed_time(b), 4)
+
+    out = {"profile": args.profile}
+    if args.profile == "ntt":
+        k = args.log_ntt
+        n = 1 << k
+        picks = load_picks()
+        plan = [int(v) for v in args.plan.split(",")] if args.plan else \
+            (picks.get(f"ntt_fwd:{k}") or {}).get("exhaustive")
+        coset_plan = plan if args.plan else (picks.get(f"ntt_coset:{k}") or {}).get("exhaustive")
+        x = device_u32(n * 8)
+        _lib.call("impeller_gen_scalars", x.data_ptr(), n, 0, 2024, 0, stream.raw())
+        tw = N.twiddles(WL.mont_limbs(WL.omega(k)), n // 2, stream)
+        shift = to_device(WL.mont_limbs(WL.GENERATOR))
+        y = torch.empty_like(x)
+        stream.sync()
+        out.update({"log": k, "plan": plan, "coset_plan": coset_plan,
+                    "copy_bytes": 2 * n * 32, "copy_ms": timed1(lambda: y.copy_(x)),
+                    "ntt_ms": [timed1(lambda: N.ntt(x, tw, k, stream, passes=plan))
+                               for _ in range(3)],
+                    "coset_ntt_ms": [timed1(lambda: N.ntt(x, tw, k, stream, coset_shift=shift,
+                                                          passes=coset_plan))
+                                     for _ in range(3)]})
+    elif args.profile == "msm":
+        nm, c, task = 1 << args.log_msm, args.window or 20, 64
+        bases = device_u32(nm * 16)
+        _lib.call("impeller_gen_bases", bases.data_ptr(), nm, 77, 0, stream.raw())
+        scal = device_u32(nm * 8)
+        _lib.call("impeller_gen_scalars", scal.data_ptr(), nm, 0, 78, 0, stream.raw())
+        tables = device_u32(M.n_tables(c, 1) * nm * 16)
+        tables[: nm * 16].copy_(bases)
+        M.precompute_device(tables, nm, c, 1, stream)
+        ws = device_u32((M.cfg_workspace_size(nm, c, 1, task) + 3) // 4)
+        res = device_u32(16)
+        y = torch.empty_like(bases)
+        stream.sync()
+        out.update({"log": args.log_msm, "window": c, "copy_bytes": 2 * nm * 64,
+                    "copy_ms": timed1(lambda: y.copy_(bases)),
+                    "msm_ms": [timed1(lambda: M.msm_device_cfg(
+                        tables, scal, nm, c, 1, stream, task=task, workspace=ws, out=res,
+                        sync=False)) for _ in range(2)]})
+    else:
+        info = poly_leg(args, stream, lambda fn, reps: timed1(fn), 1.0, False)
+        out.update({"log": args.log_poly, "ms": {k: v["ms"] for k, v in info["ops"].items()}})
+    print(json.dumps(out), flush=True)
 
 
 def cpu_model():
@@ -269,6 +344,9 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.profile:
+        profile_main(args)
+        return
 
     import numpy as np
     import torch
