@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "common.h"
@@ -1214,8 +1215,97 @@ static size_t desc_bytes(u32 njobs) {
   return align256(sizeof(MsmJob) * njobs) + align256(kMaxStages * (njobs + 1) * 4);
 }
 
-// Single MSM: descriptor + prefix uploaded into the workspace, then the stage
-// launches.  The upload is synchronised so the host staging can go away.
+// A single MSM is some 45 launches, many of them a few microseconds long, so
+// issuing them one by one leaves the GPU waiting on the host.  Their
+// sequence depends only on the shape and on the workspace (every kernel
+// reads its descriptor from the workspace, which k_msm_upload rewrites
+// before each call), so it is captured once into a HIP graph per (device,
+// workspace, stage list) and replayed.  Direct launches are kept while the
+// kernel timing registry is on (its events would be captured) and when the
+// caller's stream is itself being captured.  At most kGraphCache graphs are
+// kept; the oldest is destroyed after a device synchronisation.  The cache
+// is never torn down (graphs destroyed after the runtime's own teardown at
+// exit would be unsafe).
+static const size_t kGraphCache = 32;
+
+struct StageGraphCache {
+  std::mutex mu;
+  std::vector<std::pair<std::string, hipGraphExec_t>> entries;  // oldest first
+};
+
+static StageGraphCache& stage_graphs() {
+  static StageGraphCache* c = new StageGraphCache();
+  return *c;
+}
+
+static hipError_t capture_stages(const MsmJob* jobs_d, const u32* prefix_d,
+                                 const std::vector<StageLaunch>& st, hipGraphExec_t* out) {
+  hipStream_t cap = nullptr;
+  IMP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  hipError_t le = hipSuccess;
+  if (e == hipSuccess) {
+    le = launch_stages(jobs_d, 1, prefix_d, st.data(), (u32)st.size(), 0, cap);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(cap, &g);
+    if (e == hipSuccess && le == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+  }
+  (void)hipStreamDestroy(cap);
+  return e != hipSuccess ? e : le;
+}
+
+static hipError_t launch_cached(const MsmJob* jobs_d, const u32* prefix_d,
+                                const std::vector<StageLaunch>& st, hipStream_t stream) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (timing_on() || hipStreamIsCapturing(stream, &cs) != hipSuccess ||
+      cs != hipStreamCaptureStatusNone)
+    return launch_stages(jobs_d, 1, prefix_d, st.data(), (u32)st.size(), 0, stream);
+  int dev = 0;
+  IMP_CHECK(hipGetDevice(&dev));
+  std::string key((const char*)&dev, sizeof(dev));
+  key.append((const char*)&jobs_d, sizeof(jobs_d));
+  key.append((const char*)st.data(), st.size() * sizeof(StageLaunch));
+  StageGraphCache& c = stage_graphs();
+  std::lock_guard<std::mutex> lk(c.mu);
+  for (size_t i = 0; i < c.entries.size(); i++) {
+    if (c.entries[i].first == key) {
+      auto hit = c.entries[i];
+      c.entries.erase(c.entries.begin() + i);
+      c.entries.push_back(hit);
+      return hipGraphLaunch(hit.second, stream);
+    }
+  }
+  hipGraphExec_t exec = nullptr;
+  if (capture_stages(jobs_d, prefix_d, st, &exec) != hipSuccess || exec == nullptr) {
+    (void)hipGetLastError();
+    return launch_stages(jobs_d, 1, prefix_d, st.data(), (u32)st.size(), 0, stream);
+  }
+  if (c.entries.size() >= kGraphCache) {
+    IMP_CHECK(hipDeviceSynchronize());
+    (void)hipGraphExecDestroy(c.entries.front().second);
+    c.entries.erase(c.entries.begin());
+  }
+  c.entries.push_back({key, exec});
+  return hipGraphLaunch(exec, stream);
+}
+
+// A single MSM's descriptor and stage prefixes travel as kernel arguments to
+// one small kernel that writes them into the workspace, in stream order: no
+// host staging to keep alive, no synchronisation, and the call stays a pure
+// sequence of launches (graph-capturable).
+struct PrefixBlock {
+  u32 v[kMaxStages * 2];
+};
+
+__global__ __launch_bounds__(256) void k_msm_upload(MsmJob J, PrefixBlock P, u32 np,
+                                                   MsmJob* jd, u32* pd) {
+  const u32* src = reinterpret_cast<const u32*>(&J);
+  u32* dst = reinterpret_cast<u32*>(jd);
+  for (u32 i = threadIdx.x; i < sizeof(MsmJob) / 4; i += blockDim.x) dst[i] = src[i];
+  for (u32 i = threadIdx.x; i < np; i += blockDim.x) pd[i] = P.v[i];
+}
+
 static hipError_t msm_run(const u32* bases, u64 tstride, const u32* scalars, const MsmShape& s,
                           u32* result, char* ws, hipStream_t stream) {
   char* body = ws + desc_bytes(1);
@@ -1224,13 +1314,16 @@ static hipError_t msm_run(const u32* bases, u64 tstride, const u32* scalars, con
   std::vector<StageLaunch> st;
   std::vector<u32> prefix;
   plan_stages(shapes, st, prefix);
+  IMP_REQUIRE(prefix.size() <= kMaxStages * 2, "stage plan");
   MsmJob* jobs_d = (MsmJob*)ws;
   u32* prefix_d = (u32*)(ws + align256(sizeof(MsmJob)));
-  IMP_CHECK(hipMemcpyAsync(jobs_d, &J, sizeof(MsmJob), hipMemcpyHostToDevice, stream));
-  IMP_CHECK(hipMemcpyAsync(prefix_d, prefix.data(), prefix.size() * 4, hipMemcpyHostToDevice,
-                           stream));
-  IMP_CHECK(hipStreamSynchronize(stream));
-  return launch_stages(jobs_d, 1, prefix_d, st.data(), (u32)st.size(), 0, stream);
+  PrefixBlock P;
+  memset(&P, 0, sizeof(P));
+  memcpy(P.v, prefix.data(), prefix.size() * 4);
+  hipLaunchKernelGGL(k_msm_upload, dim3(1), dim3(256), 0, stream, J, P, (u32)prefix.size(),
+                     jobs_d, prefix_d);
+  IMP_CHECK(hipGetLastError());
+  return launch_cached(jobs_d, prefix_d, st, stream);
 }
 
 static size_t msm_ws_total(const MsmShape& s) { return desc_bytes(1) + msm_layout(s).total; }

@@ -317,6 +317,10 @@ hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
   g_capture = new std::vector<std::function<void()>>();
   return hipSuccess;
 }
+hipError_t hipStreamIsCapturing(hipStream_t, hipStreamCaptureStatus* status) {
+  *status = g_capturing ? hipStreamCaptureStatusActive : hipStreamCaptureStatusNone;
+  return hipSuccess;
+}
 hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) {
   if (!g_capturing) return hipErrorInvalidValue;
   g_capturing = false;

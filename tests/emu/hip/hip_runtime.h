@@ -49,6 +49,7 @@ enum hipMemcpyKind {
 };
 enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 1 };
 enum hipStreamCaptureMode { hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal = 1 };
+enum hipStreamCaptureStatus { hipStreamCaptureStatusNone = 0, hipStreamCaptureStatusActive = 1 };
 #define hipStreamNonBlocking 0x1
 
 typedef struct ihipStream_t* hipStream_t;
@@ -131,6 +132,7 @@ hipError_t hipEventSynchronize(hipEvent_t e);
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b);
 hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode m);
+hipError_t hipStreamIsCapturing(hipStream_t s, hipStreamCaptureStatus* status);
 hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t* g);
 hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, hipGraphNode_t* n, char* log,
                                size_t log_size);
