@@ -306,47 +306,45 @@ __device__ __forceinline__ void hist_body(const MsmJob& J, u32 pass, u64 lb, u32
   for (u32 b = t; b < kRadix; b += 256) gst(J.hist + ((u64)pw * kRadix + b) * s.tiles + tile, cnt[b]);
 }
 
-// block-wide exclusive scan of len entries (in place allowed); with
-// write_total, dst[len] = total
+__device__ __forceinline__ u32 block_excl_scan(u32 v, u32* part, u32* total);
+
+// block-wide exclusive scan of len entries (in place allowed; blockDim.x ==
+// 256); with write_total, dst[len] = total
 __device__ __forceinline__ void scan_body(const u32* src, u32* dst, u64 len, bool write_total, u32* part) {
-  const u32 T = blockDim.x, t = threadIdx.x;
+  const u32 T = 256, t = threadIdx.x;
   const u64 per = (len + T - 1) / T;
   u64 lo = (u64)t * per, hi = lo + per < len ? lo + per : len;
   u32 sum = 0;
-  __syncthreads();
   for (u64 b = lo; b < hi; b++) sum += src[b];
-  part[t] = sum;
-  __syncthreads();
-  for (u32 d = 1; d < T; d <<= 1) {
-    u32 v = t >= d ? part[t - d] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  u32 run = t ? part[t - 1] : 0;
+  u32 total;
+  u32 run = block_excl_scan(sum, part, &total);
   for (u64 b = lo; b < hi; b++) {
     u32 x = src[b];
     dst[b] = run;
     run += x;
   }
-  if (write_total && t == T - 1) dst[len] = part[T - 1];
+  if (write_total && t == T - 1) dst[len] = total;
 }
 
-// exclusive scan of one value per thread (blockDim.x == 256); returns the
-// thread's prefix, *total the block sum
+// exclusive scan of one value per thread (blockDim.x == 256, four waves of
+// 64); returns the thread's prefix, *total the block sum.  Each wave scans
+// its lanes with cross-lane reads (six steps, no barrier); the four wave
+// totals meet in part[0..3] behind one barrier.
 __device__ __forceinline__ u32 block_excl_scan(u32 v, u32* part, u32* total) {
-  const u32 t = threadIdx.x;
-  __syncthreads();
-  part[t] = v;
-  __syncthreads();
-  for (u32 d = 1; d < 256; d <<= 1) {
-    u32 w = t >= d ? part[t - d] : 0;
-    __syncthreads();
-    part[t] += w;
-    __syncthreads();
+  const u32 t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  u32 inc = v;
+#pragma unroll
+  for (u32 d = 1; d < 64; d <<= 1) {
+    const u32 w = __shfl(inc, (int)(lane >= d ? lane - d : lane), 64);
+    if (lane >= d) inc += w;
   }
-  *total = part[255];
-  return part[t] - v;
+  __syncthreads();  // the caller's previous reads of part are done
+  if (lane == 63) part[wv] = inc;
+  __syncthreads();
+  const u32 p0 = part[0], p1 = part[1], p2 = part[2], p3 = part[3];
+  *total = p0 + p1 + p2 + p3;
+  const u32 off = (wv > 0 ? p0 : 0u) + (wv > 1 ? p1 : 0u) + (wv > 2 ? p2 : 0u);
+  return off + inc - v;
 }
 
 struct ScatterLds {
@@ -653,13 +651,19 @@ __device__ __forceinline__ u32 find_bucket(const u32* toff, u32 nb, u32 t) {
 
 // A sorted entry's table point as stored (x then y, 8 words each) and its
 // sign: loaded one entry ahead, unpacked to limbs only when it is added.
+typedef u32 u32x4 __attribute__((ext_vector_type(4)));
 struct RawEntry {
-  uint4 q[4];
+  u32x4 q[4];
   u32 neg;
 };
 
+// Four aligned 16-byte global loads of one 64-byte point.  Left to itself
+// the compiler narrows them to the words each 29-bit limb needs (seven
+// overlapping flat loads at byte offsets 0/4/8/24/36/40/56: 46 % more L1
+// accesses); unpack_entry therefore passes the four vectors through an empty
+// asm statement, which uses every bit of them.
 __device__ __forceinline__ RawEntry load_entry(const MsmJob& J, u32 e) {
-  const uint4* q = reinterpret_cast<const uint4*>(J.bases + 16 * (u64)(e & 0x7fffffffu));
+  const u32x4* q = reinterpret_cast<const u32x4*>(J.bases + 16 * (u64)(e & 0x7fffffffu));
   RawEntry r;
 #pragma unroll
   for (int k = 0; k < 4; k++) r.q[k] = gld(q + k);
@@ -667,7 +671,10 @@ __device__ __forceinline__ RawEntry load_entry(const MsmJob& J, u32 e) {
   return r;
 }
 
-__device__ __forceinline__ G1Aff29 unpack_entry(const RawEntry& r) {
+__device__ __forceinline__ G1Aff29 unpack_entry(RawEntry r) {
+#if !defined(IMP_EMU) && defined(__HIP_DEVICE_COMPILE__)
+  asm("" : "+v"(r.q[0]), "+v"(r.q[1]), "+v"(r.q[2]), "+v"(r.q[3]));
+#endif
   const u32 x[8] = {r.q[0].x, r.q[0].y, r.q[0].z, r.q[0].w, r.q[1].x, r.q[1].y, r.q[1].z, r.q[1].w};
   const u32 y[8] = {r.q[2].x, r.q[2].y, r.q[2].z, r.q[2].w, r.q[3].x, r.q[3].y, r.q[3].z, r.q[3].w};
   G1Aff29 p;
