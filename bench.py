@@ -126,6 +126,20 @@ def profile_main(args):
                     "coset_ntt_ms": [timed1(lambda: N.ntt(x, tw, k, stream, coset_shift=shift,
                                                           passes=coset_plan))
                                      for _ in range(3)]})
+        # the reference-ABI entry points on the same array: ssip_ntt (in
+        # place, twiddle table) and recompute_ntt (no twiddle table: each
+        # pass rebuilds its twiddles from pq / omegas)
+        from impeller_amd.runtime import ScalarArray
+        xs, tws = ScalarArray(n, x, "cuda"), ScalarArray(n // 2, tw, "cuda")
+        g = N.GenPqOmegas()
+        pq, om = ScalarArray(g.get_pq_len(k), None, "cpu"), ScalarArray(32, None, "cpu")
+        g.get_fn()(pq, om, n, WL.mont_limbs(WL.omega(k)))
+        pq_d, om_d = ScalarArray.from_numpy(pq.to_numpy()), ScalarArray.from_numpy(om.to_numpy())
+        stream.sync()
+        out["ssip_ntt_ms"] = [timed1(lambda: N.SsipNtt().get_fn()([xs], [tws, stream]))
+                              for _ in range(3)]
+        out["recompute_ntt_ms"] = [timed1(lambda: N.RecomputeNtt().get_fn()(
+            [xs], [pq_d, om_d, stream])) for _ in range(3)]
     elif args.profile == "msm":
         nm, c, task = 1 << args.log_msm, args.window or 20, 64
         bases = device_u32(nm * 16)

@@ -390,7 +390,11 @@ __device__ __forceinline__ u32 block_excl_u32(u32 v, u32* sh, u32* total) {
   return before + inc - v;
 }
 
-__global__ __launch_bounds__(SB) void k_u32_up(const u32* in, u64 n, u32* tile_sum) {
+// guard (optional): the launch does nothing when *guard == 0 (a radix pass
+// the sort's device-side schedule skips)
+__global__ __launch_bounds__(SB) void k_u32_up(const u32* in, u64 n, u32* tile_sum,
+                                               const u32* guard) {
+  if (guard && *guard == 0) return;
   __shared__ u32 sh[SB / 64];
   const u32 t = threadIdx.x;
   const u64 first = (u64)blockIdx.x * STILE + (u64)t * SK;
@@ -403,7 +407,9 @@ __global__ __launch_bounds__(SB) void k_u32_up(const u32* in, u64 n, u32* tile_s
   if (t == 0) tile_sum[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(SB) void k_u32_down(u32* data, u64 n, const u32* carry) {
+__global__ __launch_bounds__(SB) void k_u32_down(u32* data, u64 n, const u32* carry,
+                                                 const u32* guard) {
+  if (guard && *guard == 0) return;
   __shared__ u32 sh[SB / 64];
   const u32 t = threadIdx.x;
   const u64 first = (u64)blockIdx.x * STILE + (u64)t * SK;
@@ -431,7 +437,8 @@ static size_t u32scan_bytes(u64 n) {
   return b;
 }
 
-static hipError_t u32scan(u32* data, u64 n, void* temp, hipStream_t stream) {
+static hipError_t u32scan(u32* data, u64 n, void* temp, hipStream_t stream,
+                          const u32* guard = nullptr) {
   if (n == 0) return hipSuccess;
   u32* lv[SMAXL + 1];
   u64 ln[SMAXL + 1];
@@ -445,14 +452,14 @@ static hipError_t u32scan(u32* data, u64 n, void* temp, hipStream_t stream) {
     T += align_up((size_t)tiles * 4, 256);
     ln[L + 1] = tiles;
     hipLaunchKernelGGL(k_u32_up, dim3((u32)tiles), dim3(SB), 0, stream, lv[L], ln[L],
-                       lv[L + 1]);
+                       lv[L + 1], guard);
     IMP_CHECK(hipGetLastError());
     L++;
   }
   for (int l = (int)L; l >= 0; l--) {
     const u64 tiles = (ln[l] + STILE - 1) / STILE;
     hipLaunchKernelGGL(k_u32_down, dim3((u32)tiles), dim3(SB), 0, stream, lv[l], ln[l],
-                       l == (int)L ? (const u32*)nullptr : lv[l + 1]);
+                       l == (int)L ? (const u32*)nullptr : lv[l + 1], guard);
     IMP_CHECK(hipGetLastError());
   }
   return hipSuccess;
@@ -460,108 +467,243 @@ static hipError_t u32scan(u32* data, u64 n, void* temp, hipStream_t stream) {
 
 // ------------------------------------------------- 256-bit key radix sort
 // LSD over the 32 bytes of canonical 256-bit numbers (32-byte records),
-// ascending.  Per pass: a digit histogram per tile (digit-major, so one
-// exclusive scan gives every tile's stable output offsets), then a stable
-// scatter ranking each tile's keys round by round: within a wave the keys
-// with the same digit are found by ballots over the digit's bits, across
-// the work-group's waves by per-wave digit counts in LDS.
+// ascending, stable.  Per pass: a digit histogram per tile (digit-major, so
+// one exclusive scan gives every tile's stable output offsets), then a
+// stable scatter ranking each tile's keys round by round: within a wave the
+// keys with the same digit are found by ballots over the digit's bits,
+// across the work-group's waves by per-wave digit counts in LDS.
+//
+// Fewer passes where the keys allow, decided on the device (no host
+// synchronisation): the keys are first sorted by their top 8 bytes (passes
+// 24..31, slots 0..7).  When no two different keys share those bytes -- the
+// rule for values spread over the field -- that order is already the full
+// order (equal keys keep their input order, as the full sort keeps it), and
+// the 32 remaining slots (passes 0..31 over the prefix-sorted keys) do
+// nothing.  Otherwise they run, an LSD sort of the whole key.  In both phases
+// a byte equal in every key (the AND and the OR of all keys agree there,
+// e.g. the high bytes of small integers) needs no pass.  A slot's kernels
+// read the schedule word (run, source buffer) the plan kernel wrote and
+// return at once for a skipped pass; a last kernel moves the result to the
+// first buffer when an odd number of passes ran.
 constexpr u32 RPASSES = 32;
+constexpr u32 RPREFIX = 8;                   // slots of the prefix phase (bytes 24..31)
+constexpr u32 RSLOTS = RPREFIX + RPASSES;
+constexpr u32 RS_PAR_PREFIX = 2 * RSLOTS;    // buffer (0: a, 1: b) after the prefix phase
+constexpr u32 RS_TIES = RS_PAR_PREFIX + 1;   // two different keys share the top 8 bytes
+constexpr u32 RS_PAR_END = RS_TIES + 1;      // buffer holding the sorted keys
+constexpr u32 RS_AND = RS_PAR_END + 1;       // 8 words: AND over all keys
+constexpr u32 RS_OR = RS_AND + 8;            // 8 words: OR over all keys
+constexpr u32 RS_WORDS = RS_OR + 8;
+constexpr u32 RS_GRID = 1024;                // work-groups of the tile loops
+static_assert(RS_WORDS <= 128, "k_rs_init runs 128 threads, one per schedule word");
+
+__host__ __device__ inline u32 slot_pass(u32 slot) {
+  return slot < RPREFIX ? 32 - RPREFIX + slot : slot - RPREFIX;
+}
+
+__global__ void k_rs_init(u32* sched) {
+  const u32 t = threadIdx.x;
+  if (t < RS_WORDS) sched[t] = (t >= RS_AND && t < RS_OR) ? 0xffffffffu : 0u;
+}
+
+// AND and OR of all keys, word by word
+__global__ __launch_bounds__(SB) void k_rs_bits(const u32* keys, u64 n, u32* sched) {
+  __shared__ u32 sa[8], so[8];
+  const u32 t = threadIdx.x;
+  if (t < 8) {
+    sa[t] = 0xffffffffu;
+    so[t] = 0u;
+  }
+  u32 a[8], o[8];
+#pragma unroll
+  for (u32 w = 0; w < 8; w++) {
+    a[w] = 0xffffffffu;
+    o[w] = 0u;
+  }
+  for (u64 i = (u64)blockIdx.x * SB + t; i < n; i += (u64)gridDim.x * SB) {
+    const Fr k = load_fe<Fr>(keys + 8 * i);
+#pragma unroll
+    for (u32 w = 0; w < 8; w++) {
+      a[w] &= k.v[w];
+      o[w] |= k.v[w];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (u32 w = 0; w < 8; w++) {
+    atomicAnd(&sa[w], a[w]);
+    atomicOr(&so[w], o[w]);
+  }
+  __syncthreads();
+  if (t < 8) {
+    atomicAnd(&sched[RS_AND + t], sa[t]);
+    atomicOr(&sched[RS_OR + t], so[t]);
+  }
+}
+
+// the schedule of one phase (0: prefix slots, 1: the remaining slots)
+__global__ void k_rs_plan(u32* sched, u32 phase) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  u32 par = phase ? sched[RS_PAR_PREFIX] : 0u;
+  const bool need = phase == 0 || sched[RS_TIES] != 0;
+  const u32 s0 = phase ? RPREFIX : 0u, s1 = phase ? RSLOTS : RPREFIX;
+  for (u32 s = s0; s < s1; s++) {
+    const u32 pass = slot_pass(s), w = pass >> 2, sh = (pass & 3) * 8;
+    const bool varies = (((sched[RS_AND + w] ^ sched[RS_OR + w]) >> sh) & 255u) != 0;
+    const u32 run = need && varies ? 1u : 0u;
+    sched[2 * s] = run;
+    sched[2 * s + 1] = par;
+    par ^= run;
+  }
+  sched[phase ? RS_PAR_END : RS_PAR_PREFIX] = par;
+}
+
+// RS_TIES when two adjacent prefix-sorted keys agree in their top 8 bytes
+// but not in the rest
+__global__ __launch_bounds__(SB) void k_rs_ties(const u32* a, const u32* b, u64 n, u32* sched) {
+  const u32* k = sched[RS_PAR_PREFIX] ? b : a;
+  for (u64 i = 1 + (u64)blockIdx.x * SB + threadIdx.x; i < n; i += (u64)gridDim.x * SB) {
+    const u32* p = k + 8 * i;
+    const u32* q = p - 8;
+    if (p[7] == q[7] && p[6] == q[6]) {
+      bool diff = false;
+#pragma unroll
+      for (u32 w = 0; w < 6; w++) diff |= p[w] != q[w];
+      if (diff) atomicOr(&sched[RS_TIES], 1u);
+    }
+  }
+}
 
 __device__ __forceinline__ u32 key_digit(const u32* keys, u64 i, u32 pass) {
   return (keys[8 * i + (pass >> 2)] >> ((pass & 3) * 8)) & 255u;
 }
 
-__global__ __launch_bounds__(SB) void k_rs_hist(const u32* keys, u64 n, u32 pass, u32* cnt,
-                                                u32 tiles) {
+__global__ __launch_bounds__(SB) void k_rs_hist(const u32* a, const u32* b, u64 n, u32 slot,
+                                                const u32* sched, u32* cnt, u32 tiles) {
+  if (sched[2 * slot] == 0) return;
+  const u32* keys = sched[2 * slot + 1] ? b : a;
+  const u32 pass = slot_pass(slot);
   __shared__ u32 h[256];
   const u32 t = threadIdx.x;
-  h[t] = 0;
-  __syncthreads();
+  for (u32 tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    h[t] = 0;
+    __syncthreads();
 #pragma unroll
-  for (u32 j = 0; j < SK; j++) {
-    const u64 i = (u64)blockIdx.x * STILE + (u64)j * SB + t;
-    if (i < n) atomicAdd(&h[key_digit(keys, i, pass)], 1u);
+    for (u32 j = 0; j < SK; j++) {
+      const u64 i = (u64)tile * STILE + (u64)j * SB + t;
+      if (i < n) atomicAdd(&h[key_digit(keys, i, pass)], 1u);
+    }
+    __syncthreads();
+    cnt[(u64)t * tiles + tile] = h[t];
+    __syncthreads();
   }
-  __syncthreads();
-  cnt[(u64)t * tiles + blockIdx.x] = h[t];
 }
 
-__global__ __launch_bounds__(SB) void k_rs_scatter(const u32* kin, u32* kout, u64 n, u32 pass,
-                                                   const u32* off, u32 tiles) {
+__global__ __launch_bounds__(SB) void k_rs_scatter(u32* a, u32* b, u64 n, u32 slot,
+                                                   const u32* sched, const u32* off, u32 tiles) {
+  if (sched[2 * slot] == 0) return;
+  const u32* kin = sched[2 * slot + 1] ? b : a;
+  u32* kout = sched[2 * slot + 1] ? a : b;
+  const u32 pass = slot_pass(slot);
   __shared__ u32 base[256];
   __shared__ u32 wcnt[SB / 64][256];
   const u32 t = threadIdx.x, lane = t & 63, w = t >> 6;
-  base[t] = off[(u64)t * tiles + blockIdx.x];
-#pragma unroll
-  for (u32 k = 0; k < SB / 64; k++) wcnt[k][t] = 0;
-  __syncthreads();
   const u64 below = lane ? ((1ull << lane) - 1) : 0ull;
-  for (u32 j = 0; j < SK; j++) {
-    const u64 i = (u64)blockIdx.x * STILE + (u64)j * SB + t;
-    const bool valid = i < n;
-    Fr key = Fr::zero();
-    u32 d = 256;  // lanes past the end form a group of their own
-    if (valid) {
-      key = load_fe<Fr>(kin + 8 * i);
-      d = (key.v[pass >> 2] >> ((pass & 3) * 8)) & 255u;
-    }
-    unsigned long long m = ~0ull;
+  for (u32 tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    base[t] = off[(u64)t * tiles + tile];
 #pragma unroll
-    for (u32 b = 0; b < 9; b++) {
-      const unsigned long long bal = __ballot((d >> b) & 1);
-      m &= ((d >> b) & 1) ? bal : ~bal;
-    }
-    const u32 rank = __popcll(m & below);
-    if (valid && rank == 0) wcnt[w][d] = __popcll(m);
+    for (u32 k = 0; k < SB / 64; k++) wcnt[k][t] = 0;
     __syncthreads();
-    if (valid) {
-      u32 pos = base[d] + rank;
-      for (u32 k = 0; k < w; k++) pos += wcnt[k][d];
-      store_fe<Fr>(kout + 8 * (u64)pos, key);
-    }
-    __syncthreads();
-    u32 add = 0;
+    for (u32 j = 0; j < SK; j++) {
+      const u64 i = (u64)tile * STILE + (u64)j * SB + t;
+      const bool valid = i < n;
+      Fr key = Fr::zero();
+      u32 d = 256;  // lanes past the end form a group of their own
+      if (valid) {
+        key = load_fe<Fr>(kin + 8 * i);
+        d = (key.v[pass >> 2] >> ((pass & 3) * 8)) & 255u;
+      }
+      unsigned long long m = ~0ull;
 #pragma unroll
-    for (u32 k = 0; k < SB / 64; k++) {
-      add += wcnt[k][t];
-      wcnt[k][t] = 0;
+      for (u32 bt = 0; bt < 9; bt++) {
+        const unsigned long long bal = __ballot((d >> bt) & 1);
+        m &= ((d >> bt) & 1) ? bal : ~bal;
+      }
+      const u32 rank = __popcll(m & below);
+      if (valid && rank == 0) wcnt[w][d] = __popcll(m);
+      __syncthreads();
+      if (valid) {
+        u32 pos = base[d] + rank;
+        for (u32 k = 0; k < w; k++) pos += wcnt[k][d];
+        store_fe<Fr>(kout + 8 * (u64)pos, key);
+      }
+      __syncthreads();
+      u32 add = 0;
+#pragma unroll
+      for (u32 k = 0; k < SB / 64; k++) {
+        add += wcnt[k][t];
+        wcnt[k][t] = 0;
+      }
+      base[t] += add;
+      __syncthreads();
     }
-    base[t] += add;
-    __syncthreads();
   }
+}
+
+// the sorted keys into a when the last pass that ran wrote b
+__global__ __launch_bounds__(SB) void k_rs_finish(u32* a, const u32* b, u64 n, const u32* sched) {
+  if (sched[RS_PAR_END] == 0) return;
+  const uint4* src = reinterpret_cast<const uint4*>(b);
+  uint4* dst = reinterpret_cast<uint4*>(a);
+  for (u64 i = (u64)blockIdx.x * SB + threadIdx.x; i < 2 * n; i += (u64)gridDim.x * SB)
+    dst[i] = src[i];
 }
 
 struct SortPlan {
   u32 tiles;
-  size_t cnt, scan, bytes;  // offsets within the sort's own area
+  size_t cnt, sched, scan, bytes;  // offsets within the sort's own area
 };
 
 static void sort_plan(u64 n, SortPlan* p) {
   p->tiles = (u32)((n + STILE - 1) / STILE);
   p->cnt = 0;
-  p->scan = align_up((size_t)p->tiles * 256 * 4, 256);
+  p->sched = align_up((size_t)p->tiles * 256 * 4, 256);
+  p->scan = p->sched + align_up(RS_WORDS * 4, 256);
   p->bytes = p->scan + u32scan_bytes((u64)p->tiles * 256);
 }
 
-// sorts n keys starting in a, ping-ponging with b; an even pass count
-// leaves the result in a
+// sorts n >= 1 keys in a (b: a second buffer of n keys); the result is in a
 static hipError_t sort256(u32* a, u32* b, u64 n, void* area, hipStream_t stream) {
   SortPlan p;
   sort_plan(n, &p);
   char* A = static_cast<char*>(area);
   u32* cnt = reinterpret_cast<u32*>(A + p.cnt);
-  for (u32 pass = 0; pass < RPASSES; pass++) {
-    const u32* src = (pass & 1) ? b : a;
-    u32* dst = (pass & 1) ? a : b;
-    hipLaunchKernelGGL(k_rs_hist, dim3(p.tiles), dim3(SB), 0, stream, src, n, pass, cnt,
+  u32* sched = reinterpret_cast<u32*>(A + p.sched);
+  const u32 grid = std::min(p.tiles, RS_GRID);
+  const u32 egrid = (u32)std::min<u64>((n + SB - 1) / SB, RS_GRID);
+  auto pass = [&](u32 slot) -> hipError_t {
+    hipLaunchKernelGGL(k_rs_hist, dim3(grid), dim3(SB), 0, stream, a, b, n, slot, sched, cnt,
                        p.tiles);
     IMP_CHECK(hipGetLastError());
-    IMP_CHECK(u32scan(cnt, (u64)p.tiles * 256, A + p.scan, stream));
-    hipLaunchKernelGGL(k_rs_scatter, dim3(p.tiles), dim3(SB), 0, stream, src, dst, n, pass, cnt,
+    IMP_CHECK(u32scan(cnt, (u64)p.tiles * 256, A + p.scan, stream, sched + 2 * slot));
+    hipLaunchKernelGGL(k_rs_scatter, dim3(grid), dim3(SB), 0, stream, a, b, n, slot, sched, cnt,
                        p.tiles);
-    IMP_CHECK(hipGetLastError());
-  }
-  return hipSuccess;
+    return hipGetLastError();
+  };
+  hipLaunchKernelGGL(k_rs_init, dim3(1), dim3(128), 0, stream, sched);
+  IMP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_rs_bits, dim3(egrid), dim3(SB), 0, stream, a, n, sched);
+  IMP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_rs_plan, dim3(1), dim3(64), 0, stream, sched, 0u);
+  IMP_CHECK(hipGetLastError());
+  for (u32 slot = 0; slot < RPREFIX; slot++) IMP_CHECK(pass(slot));
+  hipLaunchKernelGGL(k_rs_ties, dim3(egrid), dim3(SB), 0, stream, a, b, n, sched);
+  IMP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_rs_plan, dim3(1), dim3(64), 0, stream, sched, 1u);
+  IMP_CHECK(hipGetLastError());
+  for (u32 slot = RPREFIX; slot < RSLOTS; slot++) IMP_CHECK(pass(slot));
+  hipLaunchKernelGGL(k_rs_finish, dim3(egrid), dim3(SB), 0, stream, a, b, n, sched);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------- permute

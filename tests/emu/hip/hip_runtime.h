@@ -89,6 +89,12 @@ inline void launch(void (*k)(P...), dim3 g, dim3 b, size_t sh, hipStream_t, A&&.
 inline unsigned atomicAdd(unsigned* p, unsigned v) {
   return __atomic_fetch_add(p, v, __ATOMIC_RELAXED);
 }
+inline unsigned atomicAnd(unsigned* p, unsigned v) {
+  return __atomic_fetch_and(p, v, __ATOMIC_RELAXED);
+}
+inline unsigned atomicOr(unsigned* p, unsigned v) {
+  return __atomic_fetch_or(p, v, __ATOMIC_RELAXED);
+}
 inline unsigned long long __ballot(int pred) { return ::emu::wave_ballot(pred); }
 inline unsigned __lane_id() { return ::emu::lane_id(); }
 inline int __shfl(int v, int src, int = 64) {

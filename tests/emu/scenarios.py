@@ -727,13 +727,30 @@ def poly_kate(E):
 def poly_permute(E):
     """permute on lookup-shaped data (core/tests/poly.rs:556-640: table values
     with duplicates, inputs drawn from the usable table rows), a rotated
-    input view, one and two sort tiles, and a missing input value."""
+    input view, one and two sort tiles, and a missing input value.  The
+    sort's schedule takes every branch: small integers with a few spread
+    values (different keys share their top 8 bytes: the full sort runs),
+    spread values only (the top-8-byte order is final), small integers only
+    (the prefix passes see constant bytes and skip), and a table whose keys
+    agree in the top 8 bytes and differ lower down."""
     rng = np.random.default_rng(5)
-    for n, usable, spread in ((40, 37, 30), (3000, 2975, 5000), (2100, 2100, 7)):
-        table_ints = rng.integers(0, spread, n)
-        tab = O.ints_to_array([O.to_mont_int(int(v)) for v in table_ints])
-        # large canonical values too: the top limbs take part in the sort
-        tab[::5] = O.gen_scalars(n, 77)[::5]
+    cases = ((40, 37, 30, 5), (3000, 2975, 5000, 5), (2100, 2100, 7, 5), (3000, 2990, 0, 1),
+             (2500, 2500, 100000, 0), (600, 600, -1, 0))
+    for n, usable, spread, every in cases:
+        if spread > 0:
+            table_ints = rng.integers(0, spread, n)
+            tab = O.ints_to_array([O.to_mont_int(int(v)) for v in table_ints])
+        elif spread == 0:
+            tab = O.gen_scalars(n, 78)
+        else:
+            # one shared top 8 bytes, the low words drawn from a few values
+            hi = (int(rng.integers(1, 1 << 60)) << 192)
+            table_ints = [hi + (int(v) << 64) + int(w) for v, w in
+                          zip(rng.integers(0, 9, n), rng.integers(0, 5, n))]
+            tab = O.ints_to_array([O.to_mont_int(v % O.R_MOD) for v in table_ints])
+        if every:
+            # large canonical values too: the top limbs take part in the sort
+            tab[::every] = O.gen_scalars(n, 77)[::every]
         # inputs drawn from the usable table rows
         inp = tab[rng.integers(0, usable, n)]
         ei, et, missing = O.permute(inp, tab, usable)

@@ -289,6 +289,35 @@ def test_permute(stream, k, blinding):
     assert (rt.logical_numpy() == et).all()
 
 
+@pytest.mark.parametrize("kind", ["spread", "small", "shared_top"])
+def test_permute_sort_schedules(stream, kind):
+    """The radix sort's device-side schedule on each kind of key set: values
+    spread over the field (the top-8-byte order is final, the remaining
+    passes skip), small integers only (constant high bytes skip, the low
+    bytes sort), and keys sharing their top 8 bytes but differing lower
+    down (the full sort runs)."""
+    n = 1 << 16
+    rng = np.random.default_rng(len(kind))
+    if kind == "spread":
+        tab = O.gen_scalars(n, 91)
+    elif kind == "small":
+        tab = O.ints_to_array([O.to_mont_int(int(v)) for v in rng.integers(0, 3 * n, n)])
+    else:
+        hi = int(rng.integers(1, 1 << 60)) << 192
+        tab = O.ints_to_array([O.to_mont_int(hi + (int(v) << 64) + int(w)) for v, w in
+                               zip(rng.integers(0, 300, n), rng.integers(0, 300, n))])
+    inp = tab[rng.integers(0, n, n)]
+    ei, et, missing = O.permute(inp, tab, n)
+    assert missing == 0
+    f = P.PolyPermute()
+    tmp = GpuBuffer(f.get_buffer_size(n))
+    ri, rt = empty(n), empty(n)
+    f.get_fn()([ri, rt, tmp], [dev(inp, 5), dev(tab), stream])
+    stream.sync()
+    assert (ri.logical_numpy() == ei).all()
+    assert (rt.logical_numpy() == et).all()
+
+
 def test_permute_missing_value(stream):
     inp = O.ints_to_array([O.to_mont_int(v) for v in (9, 2, 2, 9, 4, 4)])
     tab = O.ints_to_array([O.to_mont_int(v) for v in (1, 2, 3, 4, 5, 6)])
