@@ -136,6 +136,45 @@ SIGNATURES = {
     "impeller_timing_read": (_INT, [_INT, ctypes.POINTER(ctypes.c_double),
                                     ctypes.POINTER(ctypes.c_ulonglong)]),
     "impeller_version": (ctypes.c_char_p, []),
+    # device-runtime layer (csrc/runtime_api.hip)
+    "impeller_rt_stream_create": (_ERR, [_INT, ctypes.POINTER(ctypes.c_void_p)]),
+    "impeller_rt_stream_destroy": (_ERR, [_P]),
+    "impeller_rt_stream_sync": (_ERR, [_P]),
+    "impeller_rt_stream_alloc": (_ERR, [_P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    "impeller_rt_stream_free": (_ERR, [_P, _P]),
+    "impeller_rt_memcpy_async": (_ERR, [_INT, _P, _P, ctypes.c_size_t, _INT, _P]),
+    "impeller_rt_event_create": (_ERR, [_INT, _INT, ctypes.POINTER(ctypes.c_void_p)]),
+    "impeller_rt_event_record": (_ERR, [_P, _P]),
+    "impeller_rt_event_sync": (_ERR, [_P]),
+    "impeller_rt_stream_wait_event": (_ERR, [_P, _P]),
+    "impeller_rt_event_reset": (_ERR, [_P]),
+    "impeller_rt_event_elapsed": (_ERR, [_P, _P, ctypes.POINTER(ctypes.c_float)]),
+    "impeller_rt_event_raw": (_P, [_P]),
+    "impeller_rt_event_destroy": (_ERR, [_P]),
+    "impeller_rt_alloc_pinned": (_ERR, [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    "impeller_rt_free_pinned": (_ERR, [_P]),
+    "impeller_rt_static_create": (_ERR, [_INT, ctypes.c_size_t, _INT,
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "impeller_rt_static_allocate": (_ERR, [_P, ctypes.c_size_t, ctypes.c_size_t,
+                                           ctypes.POINTER(ctypes.c_void_p)]),
+    "impeller_rt_static_free": (_ERR, [_P, _P]),
+    "impeller_rt_static_base": (_P, [_P]),
+    "impeller_rt_static_destroy": (_ERR, [_P]),
+    "impeller_rt_page_granularity": (_ERR, [_INT, ctypes.POINTER(ctypes.c_size_t)]),
+    "impeller_rt_page_create": (_ERR, [_INT, ctypes.c_size_t, ctypes.c_size_t, _INT,
+                                       ctypes.POINTER(ctypes.c_void_p)]),
+    "impeller_rt_page_allocate": (_ERR, [_P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
+                                         ctypes.c_size_t, ctypes.c_size_t,
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "impeller_rt_page_deallocate": (_ERR, [_P, _P, ctypes.c_size_t]),
+    "impeller_rt_page_size": (ctypes.c_size_t, [_P]),
+    "impeller_rt_page_destroy": (_ERR, [_P]),
+    "impeller_rt_num_cus": (_ERR, [_INT, ctypes.POINTER(ctypes.c_int)]),
+    "impeller_rt_arch_name": (_ERR, [_INT, ctypes.c_char_p, ctypes.c_size_t]),
+    "impeller_rt_file_read_to_device": (_ERR, [_INT, _U64, ctypes.c_size_t, _P, _P,
+                                               ctypes.c_size_t, _P]),
+    "impeller_rt_file_write_from_device": (_ERR, [_INT, _U64, ctypes.c_size_t, _P, _P,
+                                                  ctypes.c_size_t, _P]),
 }
 
 _lib = None

@@ -440,6 +440,86 @@ void impeller_timing_enable(int on);
 void impeller_timing_reset(void);
 int impeller_timing_read(int kernel_id, double *total_ms, unsigned long long *count);
 
+/* ---------------------------------------------------------------------
+ * Device-runtime layer (csrc/runtime_api.hip): what the reference's Rust
+ * runtime takes from its zkpoly_cuda_api crate (cuda_api/src/*.rs), over the
+ * HIP runtime.  A hip_api crate binds these in place of generated CUDA
+ * bindings.  Errors are returned (hipErrorInvalidValue for a broken
+ * contract the reference panics on), never raised.
+ * ------------------------------------------------------------------- */
+typedef struct ImpRtEvent ImpRtEvent;
+typedef struct ImpRtStatic ImpRtStatic;
+typedef struct ImpRtPages ImpRtPages;
+
+/* CudaStream (cuda_api/src/stream.rs:137-257): create, destroy, sync,
+ * stream-ordered alloc / free, async copies (kind 1 h2d, 2 d2h, 3 d2d). */
+hipError_t impeller_rt_stream_create(int device, hipStream_t *out);
+hipError_t impeller_rt_stream_destroy(hipStream_t s);
+hipError_t impeller_rt_stream_sync(hipStream_t s);
+hipError_t impeller_rt_stream_alloc(hipStream_t s, size_t bytes, void **out);
+hipError_t impeller_rt_stream_free(hipStream_t s, void *p);
+hipError_t impeller_rt_memcpy_async(int device, void *dst, const void *src, size_t bytes,
+                                    int kind, hipStream_t s);
+
+/* CudaEvent / CudaEventRaw (stream.rs:1-135).  host_ordered = 1: sync and
+ * stream_wait_event first block until the event has been recorded (by any
+ * host thread); reset forgets the record.  elapsed: ms from a to b. */
+hipError_t impeller_rt_event_create(int device, int host_ordered, ImpRtEvent **out);
+hipError_t impeller_rt_event_record(ImpRtEvent *e, hipStream_t s);
+hipError_t impeller_rt_event_sync(ImpRtEvent *e);
+hipError_t impeller_rt_stream_wait_event(hipStream_t s, ImpRtEvent *e);
+hipError_t impeller_rt_event_reset(ImpRtEvent *e);
+hipError_t impeller_rt_event_elapsed(ImpRtEvent *a, ImpRtEvent *b, float *ms);
+hipEvent_t impeller_rt_event_raw(ImpRtEvent *e);
+hipError_t impeller_rt_event_destroy(ImpRtEvent *e);
+
+/* alloc_pinned / free_pinned (cuda_api/src/mem.rs:18-46). */
+hipError_t impeller_rt_alloc_pinned(size_t bytes, void **out);
+hipError_t impeller_rt_free_pinned(void *p);
+
+/* StaticAllocator (cuda_api/src/mem/static_allocator.rs): one device
+ * allocation of max_size bytes; allocate hands out [offset, offset + bytes)
+ * chosen by the caller, refusing a range that meets a live one when
+ * check_overlap is set. */
+hipError_t impeller_rt_static_create(int device, size_t max_size, int check_overlap,
+                                     ImpRtStatic **out);
+hipError_t impeller_rt_static_allocate(ImpRtStatic *a, size_t offset, size_t bytes, void **out);
+hipError_t impeller_rt_static_free(ImpRtStatic *a, void *p);
+void *impeller_rt_static_base(ImpRtStatic *a);
+hipError_t impeller_rt_static_destroy(ImpRtStatic *a);
+
+/* PageAllocator (cuda_api/src/mem/page_allocator.rs) over HIP virtual
+ * memory: page_num device pages of page_size bytes (a multiple of
+ * impeller_rt_page_granularity); allocate maps page_ids, in order, at the
+ * start of a va_size reservation the first time vaid is seen and returns
+ * the same address for the same vaid afterwards; with memory_check a page
+ * belongs to one live address at a time and destroy refuses while pages are
+ * assigned.  Device pages only (the reference's host-NUMA pages are not
+ * offered). */
+hipError_t impeller_rt_page_granularity(int device, size_t *out);
+hipError_t impeller_rt_page_create(int device, size_t page_size, size_t page_num,
+                                   int memory_check, ImpRtPages **out);
+hipError_t impeller_rt_page_allocate(ImpRtPages *a, size_t va_size, const size_t *page_ids,
+                                     size_t n_pages, size_t vaid, void **out);
+hipError_t impeller_rt_page_deallocate(ImpRtPages *a, void *p, size_t vaid);
+size_t impeller_rt_page_size(ImpRtPages *a);
+hipError_t impeller_rt_page_destroy(ImpRtPages *a);
+
+/* get_num_sms / sm_version (cuda_api/src/device_info.rs): compute units and
+ * the gfx target name ("gfx950"). */
+hipError_t impeller_rt_num_cus(int device, int *out);
+hipError_t impeller_rt_arch_name(int device, char *buf, size_t len);
+
+/* In place of cuFile (cuda_api/src/file.rs): bytes [file_offset,
+ * file_offset + bytes) of a file descriptor to / from device memory through
+ * a caller's pinned staging buffer (the read overlaps file reads with
+ * copies through its two halves); both return with the data in place. */
+hipError_t impeller_rt_file_read_to_device(int fd, uint64_t file_offset, size_t bytes, void *dst,
+                                           void *staging, size_t staging_bytes, hipStream_t s);
+hipError_t impeller_rt_file_write_from_device(int fd, uint64_t file_offset, size_t bytes,
+                                              const void *src, void *staging,
+                                              size_t staging_bytes, hipStream_t s);
+
 /* Library version string. */
 const char *impeller_version(void);
 

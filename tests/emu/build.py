@@ -27,8 +27,14 @@ def lib_path(asan=False):
     return os.path.join(OUT_DIR, "libimpeller_emu_asan.so" if asan else "libimpeller_emu.so")
 
 
+# the runtime layer calls the HIP runtime itself (virtual memory, pinned
+# memory, events): it has no kernels and is not emulated
+NOT_EMULATED = ("runtime_api.hip",)
+
+
 def sources():
-    return sorted(glob.glob(os.path.join(CSRC, "*.hip"))) + [os.path.join(HERE, "emu.cpp")]
+    return sorted(p for p in glob.glob(os.path.join(CSRC, "*.hip"))
+                  if os.path.basename(p) not in NOT_EMULATED) + [os.path.join(HERE, "emu.cpp")]
 
 
 def build(asan=False, verbose=False):

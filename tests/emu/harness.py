@@ -29,6 +29,8 @@ def load(asan=False):
             B.build(asan=asan)
         L = ctypes.CDLL(path)
         for name, (res, args) in _lib.SIGNATURES.items():
+            if name.startswith("impeller_rt_"):
+                continue  # the runtime layer is not emulated (tests/emu/build.py)
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args

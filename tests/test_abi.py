@@ -164,3 +164,22 @@ def test_ntt_workspace_size():
     assert L.impeller_ntt_workspace_size(24, fused, 5) == base + (32 << 24)
     bad = (ctypes.c_uint * 4)(1, 10, 8, 8)  # levels do not add up to 24
     assert L.impeller_ntt_workspace_size(24, bad, 4) == 0
+
+
+def test_runtime_layer_rejects_bad_arguments_without_gpu():
+    """impeller_rt_* check their arguments before any runtime call and
+    return an error where the reference's crate would panic."""
+    L = _lib.lib()
+    out = ctypes.c_void_p()
+    assert L.impeller_rt_static_allocate(None, 0, 16, ctypes.byref(out)) != 0
+    assert L.impeller_rt_memcpy_async(0, None, None, 16, 7, None) != 0   # bad copy kind
+    ids = (ctypes.c_size_t * 1)(0)
+    assert L.impeller_rt_page_allocate(None, 1 << 21, ids, 1, 0, ctypes.byref(out)) != 0
+    assert L.impeller_rt_event_sync(None) != 0
+    assert L.impeller_rt_event_raw(None) is None
+    assert L.impeller_rt_page_size(None) == 0
+    assert L.impeller_rt_file_read_to_device(-1, 0, 16, None, None, 0, None) != 0
+    # the Python mirror of the crate imports without a GPU
+    from impeller_amd import hip_api
+    assert hip_api.H2D == 1 and hip_api.D2H == 2 and hip_api.D2D == 3
+
