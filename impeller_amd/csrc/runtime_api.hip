@@ -16,6 +16,7 @@
 #include <fcntl.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <condition_variable>
 #include <iterator>
 #include <map>
@@ -254,12 +255,13 @@ extern "C" hipError_t impeller_rt_static_destroy(ImpRtStatic* a) {
 // pages there in order; later calls with the same vaid return the same
 // address (the mapping stays until the allocator is destroyed, as in the
 // reference).  With memory_check, a page may belong to one live address at a
-// time, deallocate releases the pages of an address, and destroy refuses to
-// run while pages are still assigned.
+// time, a later allocation of a vaid must name the pages mapped there,
+// deallocate releases the pages of an address, and destroy refuses to run
+// while pages are still assigned.
 struct ImpRtVa {
   char* ptr = nullptr;
   size_t size = 0;
-  size_t pages = 0;
+  std::vector<size_t> mapped;    // the pages mapped there, in order
   std::vector<size_t> assigned;  // pages currently assigned (memory_check)
 };
 
@@ -331,6 +333,11 @@ extern "C" hipError_t impeller_rt_page_allocate(ImpRtPages* a, size_t va_size, c
     if (a->check) {
       RT_REQUIRE(va_size == va.size, "inconsistent va size %zu for vaid %zu (reserved %zu)",
                  va_size, vaid, va.size);
+      // the address keeps the pages of its first allocation: a later one
+      // must name those pages (another list would alias other memory)
+      RT_REQUIRE(n_pages == va.mapped.size() &&
+                     std::equal(page_ids, page_ids + n_pages, va.mapped.begin()),
+                 "vaid %zu is mapped to other pages", vaid);
       for (size_t k = 0; k < n_pages; k++)
         RT_REQUIRE(a->owner[page_ids[k]] == nullptr,
                    "allocating page %zu to virtual address %p while it is still occupied by %p",
@@ -377,7 +384,7 @@ extern "C" hipError_t impeller_rt_page_allocate(ImpRtPages* a, size_t va_size, c
   ImpRtVa va;
   va.ptr = p;
   va.size = va_size;
-  va.pages = n_pages;
+  va.mapped.assign(page_ids, page_ids + n_pages);
   if (a->check)
     for (size_t k = 0; k < n_pages; k++) {
       a->owner[page_ids[k]] = p;
@@ -419,7 +426,7 @@ extern "C" hipError_t impeller_rt_page_destroy(ImpRtPages* a) {
     if (first == hipSuccess) first = r;
   };
   for (auto& kv : a->vas) {
-    for (size_t k = 0; k < kv.second.pages; k++)
+    for (size_t k = 0; k < kv.second.mapped.size(); k++)
       keep(hipMemUnmap(kv.second.ptr + k * a->page_size, a->page_size));
     keep(hipMemAddressFree(kv.second.ptr, kv.second.size));
   }

@@ -493,8 +493,8 @@ hipError_t impeller_rt_static_destroy(ImpRtStatic *a);
  * impeller_rt_page_granularity); allocate maps page_ids, in order, at the
  * start of a va_size reservation the first time vaid is seen and returns
  * the same address for the same vaid afterwards; with memory_check a page
- * belongs to one live address at a time and destroy refuses while pages are
- * assigned.  Device pages only (the reference's host-NUMA pages are not
+ * belongs to one live address at a time, a later call for a vaid must name
+ * the pages mapped there, and destroy refuses while pages are assigned.  Device pages only (the reference's host-NUMA pages are not
  * offered). */
 hipError_t impeller_rt_page_granularity(int device, size_t *out);
 hipError_t impeller_rt_page_create(int device, size_t page_size, size_t page_num,

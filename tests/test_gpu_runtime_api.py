@@ -171,6 +171,9 @@ def test_page_allocator_maps_pages():
         a.allocate(2 * ps, [1, 2], 1)
     a.deallocate(p, 0)
     # page_allocator.rs:463-478: the same vaid gives back the same address
+    # (and must name the pages mapped there)
+    with pytest.raises(_lib.HipError):
+        a.allocate(4 * ps, [1, 0], 0)
     p2 = a.allocate(4 * ps, [0, 1], 0)
     assert p2 == p
     assert (h2d_d2h(s, p2, data) == data).all()
