@@ -935,12 +935,15 @@ __device__ __forceinline__ void final_body(const MsmJob& J, u32* red) {
     __syncthreads();
   }
   if (t != 0) return;
-  G1Xyzz29 acc = load_xyzz29(red + kPt * ((s.n_windows - 1) * nv * G));
+  // Horner over the physical windows on one lane, in the 8 x 32-bit form:
+  // a chain of (n_windows - 1) c dependent doublings, where that form's
+  // shorter product latency counts
+  G1Xyzz acc = xyzz_from_29(load_xyzz29(red + kPt * ((s.n_windows - 1) * nv * G)));
   for (int pw = (int)s.n_windows - 2; pw >= 0; pw--) {
-    for (u32 k = 0; k < s.c; k++) acc = xyzz29_dbl(acc);
-    acc = xyzz29_add(acc, load_xyzz29(red + kPt * ((u32)pw * nv * G)));
+    for (u32 k = 0; k < s.c; k++) acc = xyzz_dbl(acc);
+    acc = xyzz_add(acc, xyzz_from_29(load_xyzz29(red + kPt * ((u32)pw * nv * G))));
   }
-  store_aff(J.result, xyzz_to_aff(xyzz_from_29(acc)));
+  store_aff(J.result, xyzz_to_aff(acc));
 }
 
 // ------------------------------------------------------- batched kernels
