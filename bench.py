@@ -15,7 +15,9 @@ bit.  Then the fused proving chunk at k = 22 under the same three schedules
 Last, the polynomial library (poly_add / sub / mul, scan_mul,
 batched_invert, poly_eval, kate_division, permute) at 2^24 elements, each
 against its HBM and Montgomery-product rooflines, with sampled or
-whole-array parity checks (--no-poly skips it).
+whole-array parity checks, the emitted fused gate against the hand-written
+one, and the device-runtime layer's copy, file and page-mapping rates
+(--no-poly skips these).
 
   python bench.py [--steps K] [--warmup W] [--window C] [--no-cpu] [--no-batch] [--no-chunk]
                   [--no-poly]
@@ -298,6 +300,87 @@ def poly_leg(args, stream, timed, mul_peak, with_cpu):
     del a, b, r, r_u, t_u, src, t_scan, t_inv, t_eval, t_kate, t_perm
     torch.cuda.empty_cache()
     return info
+
+
+def runtime_leg():
+    """The device-runtime layer (impeller_amd.hip_api over impeller_rt_*):
+    pinned host <-> device copy rates on its own stream (256 MiB), the
+    file -> device rate through 8 MiB of pinned staging (a 256 MiB file
+    written just before, so mostly from the page cache), and the cost of
+    mapping virtual-memory pages (allocate of 32 fresh 2 MiB-aligned pages
+    into one address), each checked for content."""
+    import ctypes
+    import os
+    import tempfile
+    import time
+
+    import numpy as np
+    from impeller_amd import hip_api as H
+
+    nbytes = 256 << 20
+    s = H.HipStream(0)
+    a, b = H.HipEvent(0, host_ordered=False), H.HipEvent(0, host_ordered=False)
+    hp = H.alloc_pinned(nbytes)
+    d = s.allocate(nbytes)
+    src = np.frombuffer(os.urandom(1 << 20), np.uint8)
+    host = np.ctypeslib.as_array((ctypes.c_uint8 * nbytes).from_address(hp))
+    host[:] = np.tile(src, nbytes >> 20)
+
+    def rate(kind):
+        best = None
+        for _ in range(3):
+            a.record(s)
+            if kind == "h2d":
+                s.memcpy_h2d(d, hp, nbytes)
+            else:
+                s.memcpy_d2h(hp, d, nbytes)
+            b.record(s)
+            b.sync()
+            ms = a.elapsed(b)
+            best = ms if best is None else min(best, ms)
+        return round(nbytes / best / 1e6, 1)
+
+    h2d = rate("h2d")
+    d2h = rate("d2h")
+    ok = bool((host[: 1 << 20] == src).all())
+    st_bytes = 8 << 20
+    st = H.alloc_pinned(st_bytes)
+    with tempfile.NamedTemporaryFile(dir="/tmp", delete=False) as f:
+        f.write(np.tile(src, nbytes >> 20).tobytes())
+        path = f.name
+    fd = H.open_file(path)
+    t0 = time.perf_counter()
+    H.read_file_to_device(fd, 0, nbytes, d, st, st_bytes, s)
+    file_s = time.perf_counter() - t0
+    os.close(fd)
+    os.unlink(path)
+    back = np.zeros(1 << 20, np.uint8)
+    s.memcpy_d2h(back.ctypes.data, d + nbytes - (1 << 20), 1 << 20)
+    s.sync()
+    ok &= bool((back == src).all())
+    g = H.page_granularity(0)
+    ps = max(g, 2 << 20) // g * g
+    pa = H.PageAllocator(0, ps, 32, True)
+    t0 = time.perf_counter()
+    p = pa.allocate(32 * ps, list(range(32)), 0)
+    map_ms = (time.perf_counter() - t0) * 1e3
+    s.memcpy_h2d(p, hp, min(32 * ps, nbytes))
+    s.memcpy_d2h(back.ctypes.data, p, 1 << 20)
+    s.sync()
+    ok &= bool((back == src).all())
+    pa.deallocate(p, 0)
+    pa.destroy()
+    s.free(d)
+    s.sync()
+    H.free_pinned(st)
+    H.free_pinned(hp)
+    for e in (a, b):
+        e.destroy()
+    s.destroy()
+    return {"pinned_h2d_gb_s": h2d, "pinned_d2h_gb_s": d2h,
+            "file_to_device_gb_s": round(nbytes / file_s / 1e9, 2),
+            "page_map_32_pages_ms": round(map_ms, 3), "page_size": ps, "granularity": g,
+            "contents_equal": ok}
 
 
 def fused_leg(stream, timed, log_n=24):
@@ -735,9 +818,11 @@ def main():
     # ------------------------------------------------- polynomial library
     poly = None
     fused = None
+    rt = None
     if not args.no_poly:
         poly = poly_leg(args, stream, timed, rate_fr32, not args.no_cpu)
         fused = fused_leg(stream, timed)
+        rt = runtime_leg()
 
     # ------------------------------------------------ parity + CPU baseline
     parity = dict(batch_parity)
@@ -840,6 +925,7 @@ def main():
         "proving_chunk": chunk,
         "poly_ops": poly,
         "fused_emitter": fused,
+        "runtime_api": rt,
         "kernels_ms": {name: {"total": round(v[0], 3), "launches": v[1]}
                        for name, v in kt.items() if v[1]},
         "roofline": {
