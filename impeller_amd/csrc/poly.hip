@@ -469,32 +469,34 @@ static hipError_t u32scan(u32* data, u64 n, void* temp, hipStream_t stream,
 // LSD over the 32 bytes of canonical 256-bit numbers (32-byte records),
 // ascending, stable.  Per pass: a digit histogram per tile (digit-major, so
 // one exclusive scan gives every tile's stable output offsets), then a
-// stable scatter ranking each tile's keys round by round: within a wave the
-// keys with the same digit are found by ballots over the digit's bits,
-// across the work-group's waves by per-wave digit counts in LDS.
+// stable scatter ranking each tile's records round by round: within a wave
+// the records with the same digit are found by ballots over the digit's
+// bits, across the work-group's waves by per-wave digit counts in LDS.
 //
-// Fewer passes where the keys allow, decided on the device (no host
-// synchronisation): the keys are first sorted by their top 8 bytes (passes
-// 24..31, slots 0..7).  When no two different keys share those bytes -- the
-// rule for values spread over the field -- that order is already the full
-// order (equal keys keep their input order, as the full sort keeps it), and
-// the 32 remaining slots (passes 0..31 over the prefix-sorted keys) do
-// nothing.  Otherwise they run, an LSD sort of the whole key.  In both phases
-// a byte equal in every key (the AND and the OR of all keys agree there,
-// e.g. the high bytes of small integers) needs no pass.  A slot's kernels
-// read the schedule word (run, source buffer) the plan kernel wrote and
-// return at once for a skipped pass; a last kernel moves the result to the
-// first buffer when an odd number of passes ran.
+// Fewer passes and fewer bytes where the keys allow, decided on the device
+// (no host synchronisation).  Prefix phase (slots 0..7, passes 24..31): the
+// keys' top 8 bytes and their input index, 12-byte records, are sorted, and
+// the keys are then gathered in that order.  When no two different keys
+// share those bytes -- the rule for values spread over the field -- that
+// order is already the full order (equal keys keep their input order, as
+// the full sort keeps it), and the 32 slots of the full phase (passes 0..31
+// over the gathered keys) do nothing.  Otherwise they run, an LSD sort of
+// the whole key.  In both phases a byte equal in every key (the AND and the
+// OR of all keys agree there, e.g. the high bytes of small integers) needs
+// no pass.  A slot's kernels read the schedule word (run, source buffer) the
+// plan kernel wrote and return at once for a skipped pass; a last kernel
+// moves the result to its buffer when an odd number of full passes ran.
 constexpr u32 RPASSES = 32;
 constexpr u32 RPREFIX = 8;                   // slots of the prefix phase (bytes 24..31)
 constexpr u32 RSLOTS = RPREFIX + RPASSES;
-constexpr u32 RS_PAR_PREFIX = 2 * RSLOTS;    // buffer (0: a, 1: b) after the prefix phase
+constexpr u32 RS_PAR_PREFIX = 2 * RSLOTS;    // record buffer (0 or 1) after the prefix phase
 constexpr u32 RS_TIES = RS_PAR_PREFIX + 1;   // two different keys share the top 8 bytes
-constexpr u32 RS_PAR_END = RS_TIES + 1;      // buffer holding the sorted keys
+constexpr u32 RS_PAR_END = RS_TIES + 1;      // key buffer (0: result, 1: scratch) at the end
 constexpr u32 RS_AND = RS_PAR_END + 1;       // 8 words: AND over all keys
 constexpr u32 RS_OR = RS_AND + 8;            // 8 words: OR over all keys
 constexpr u32 RS_WORDS = RS_OR + 8;
 constexpr u32 RS_GRID = 1024;                // work-groups of the tile loops
+constexpr u32 RREC = 3;                      // words per prefix record: key words 6, 7, index
 static_assert(RS_WORDS <= 128, "k_rs_init runs 128 threads, one per schedule word");
 
 __host__ __device__ inline u32 slot_pass(u32 slot) {
@@ -506,8 +508,8 @@ __global__ void k_rs_init(u32* sched) {
   if (t < RS_WORDS) sched[t] = (t >= RS_AND && t < RS_OR) ? 0xffffffffu : 0u;
 }
 
-// AND and OR of all keys, word by word
-__global__ __launch_bounds__(SB) void k_rs_bits(const u32* keys, u64 n, u32* sched) {
+// AND and OR of all keys, word by word, and each key's prefix record
+__global__ __launch_bounds__(SB) void k_rs_bits(const u32* keys, u64 n, u32* sched, u32* rec) {
   __shared__ u32 sa[8], so[8];
   const u32 t = threadIdx.x;
   if (t < 8) {
@@ -527,6 +529,9 @@ __global__ __launch_bounds__(SB) void k_rs_bits(const u32* keys, u64 n, u32* sch
       a[w] &= k.v[w];
       o[w] |= k.v[w];
     }
+    rec[RREC * i] = k.v[6];
+    rec[RREC * i + 1] = k.v[7];
+    rec[RREC * i + 2] = (u32)i;
   }
   __syncthreads();
 #pragma unroll
@@ -541,10 +546,11 @@ __global__ __launch_bounds__(SB) void k_rs_bits(const u32* keys, u64 n, u32* sch
   }
 }
 
-// the schedule of one phase (0: prefix slots, 1: the remaining slots)
+// the schedule of one phase (0: prefix slots, 1: the full slots, which start
+// from the gathered keys in buffer 0)
 __global__ void k_rs_plan(u32* sched, u32 phase) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  u32 par = phase ? sched[RS_PAR_PREFIX] : 0u;
+  u32 par = 0;
   const bool need = phase == 0 || sched[RS_TIES] != 0;
   const u32 s0 = phase ? RPREFIX : 0u, s1 = phase ? RSLOTS : RPREFIX;
   for (u32 s = s0; s < s1; s++) {
@@ -558,10 +564,17 @@ __global__ void k_rs_plan(u32* sched, u32 phase) {
   sched[phase ? RS_PAR_END : RS_PAR_PREFIX] = par;
 }
 
+// out[i] = keys[index of prefix record i]
+__global__ __launch_bounds__(SB) void k_rs_gather(const u32* r0, const u32* r1, const u32* keys,
+                                                  u64 n, const u32* sched, u32* out) {
+  const u32* rec = sched[RS_PAR_PREFIX] ? r1 : r0;
+  for (u64 i = (u64)blockIdx.x * SB + threadIdx.x; i < n; i += (u64)gridDim.x * SB)
+    store_fe<Fr>(out + 8 * i, load_fe<Fr>(keys + 8 * (u64)rec[RREC * i + 2]));
+}
+
 // RS_TIES when two adjacent prefix-sorted keys agree in their top 8 bytes
 // but not in the rest
-__global__ __launch_bounds__(SB) void k_rs_ties(const u32* a, const u32* b, u64 n, u32* sched) {
-  const u32* k = sched[RS_PAR_PREFIX] ? b : a;
+__global__ __launch_bounds__(SB) void k_rs_ties(const u32* k, u64 n, u32* sched) {
   for (u64 i = 1 + (u64)blockIdx.x * SB + threadIdx.x; i < n; i += (u64)gridDim.x * SB) {
     const u32* p = k + 8 * i;
     const u32* q = p - 8;
@@ -574,15 +587,57 @@ __global__ __launch_bounds__(SB) void k_rs_ties(const u32* a, const u32* b, u64 
   }
 }
 
-__device__ __forceinline__ u32 key_digit(const u32* keys, u64 i, u32 pass) {
-  return (keys[8 * i + (pass >> 2)] >> ((pass & 3) * 8)) & 255u;
+// one record of W words: a 32-byte key (W = 8) or a prefix record (W = 3,
+// key words 6 and 7 first)
+template <u32 W>
+struct RsRec {
+  u32 w[W];
+};
+
+template <u32 W>
+__device__ __forceinline__ RsRec<W> rec_load(const u32* base, u64 i) {
+  RsRec<W> r;
+  if constexpr (W == 8) {
+    const Fr k = load_fe<Fr>(base + 8 * i);
+#pragma unroll
+    for (u32 k2 = 0; k2 < 8; k2++) r.w[k2] = k.v[k2];
+  } else {
+#pragma unroll
+    for (u32 k2 = 0; k2 < W; k2++) r.w[k2] = base[W * i + k2];
+  }
+  return r;
 }
 
+template <u32 W>
+__device__ __forceinline__ void rec_store(u32* base, u64 i, const RsRec<W>& r) {
+  if constexpr (W == 8) {
+    Fr k;
+#pragma unroll
+    for (u32 k2 = 0; k2 < 8; k2++) k.v[k2] = r.w[k2];
+    store_fe<Fr>(base + 8 * i, k);
+  } else {
+#pragma unroll
+    for (u32 k2 = 0; k2 < W; k2++) base[W * i + k2] = r.w[k2];
+  }
+}
+
+// the pass's digit: byte (pass & 3) of key word pass >> 2 (prefix records
+// hold key words 6 and 7 as their words 0 and 1)
+template <u32 W>
+__device__ __forceinline__ u32 rec_digit(u32 word, u32 pass) {
+  return (word >> ((pass & 3) * 8)) & 255u;
+}
+template <u32 W>
+__device__ __forceinline__ u32 digit_word(u32 pass) {
+  return W == 8 ? (pass >> 2) : (pass >> 2) - 6;
+}
+
+template <u32 W>
 __global__ __launch_bounds__(SB) void k_rs_hist(const u32* a, const u32* b, u64 n, u32 slot,
                                                 const u32* sched, u32* cnt, u32 tiles) {
   if (sched[2 * slot] == 0) return;
-  const u32* keys = sched[2 * slot + 1] ? b : a;
-  const u32 pass = slot_pass(slot);
+  const u32* recs = sched[2 * slot + 1] ? b : a;
+  const u32 pass = slot_pass(slot), dw = digit_word<W>(pass);
   __shared__ u32 h[256];
   const u32 t = threadIdx.x;
   for (u32 tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
@@ -591,7 +646,7 @@ __global__ __launch_bounds__(SB) void k_rs_hist(const u32* a, const u32* b, u64 
 #pragma unroll
     for (u32 j = 0; j < SK; j++) {
       const u64 i = (u64)tile * STILE + (u64)j * SB + t;
-      if (i < n) atomicAdd(&h[key_digit(keys, i, pass)], 1u);
+      if (i < n) atomicAdd(&h[rec_digit<W>(recs[W * i + dw], pass)], 1u);
     }
     __syncthreads();
     cnt[(u64)t * tiles + tile] = h[t];
@@ -599,12 +654,13 @@ __global__ __launch_bounds__(SB) void k_rs_hist(const u32* a, const u32* b, u64 
   }
 }
 
+template <u32 W>
 __global__ __launch_bounds__(SB) void k_rs_scatter(u32* a, u32* b, u64 n, u32 slot,
                                                    const u32* sched, const u32* off, u32 tiles) {
   if (sched[2 * slot] == 0) return;
-  const u32* kin = sched[2 * slot + 1] ? b : a;
-  u32* kout = sched[2 * slot + 1] ? a : b;
-  const u32 pass = slot_pass(slot);
+  const u32* rin = sched[2 * slot + 1] ? b : a;
+  u32* rout = sched[2 * slot + 1] ? a : b;
+  const u32 pass = slot_pass(slot), dw = digit_word<W>(pass);
   __shared__ u32 base[256];
   __shared__ u32 wcnt[SB / 64][256];
   const u32 t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -617,11 +673,11 @@ __global__ __launch_bounds__(SB) void k_rs_scatter(u32* a, u32* b, u64 n, u32 sl
     for (u32 j = 0; j < SK; j++) {
       const u64 i = (u64)tile * STILE + (u64)j * SB + t;
       const bool valid = i < n;
-      Fr key = Fr::zero();
+      RsRec<W> r = {};
       u32 d = 256;  // lanes past the end form a group of their own
       if (valid) {
-        key = load_fe<Fr>(kin + 8 * i);
-        d = (key.v[pass >> 2] >> ((pass & 3) * 8)) & 255u;
+        r = rec_load<W>(rin, i);
+        d = rec_digit<W>(r.w[dw], pass);
       }
       unsigned long long m = ~0ull;
 #pragma unroll
@@ -635,7 +691,7 @@ __global__ __launch_bounds__(SB) void k_rs_scatter(u32* a, u32* b, u64 n, u32 sl
       if (valid) {
         u32 pos = base[d] + rank;
         for (u32 k = 0; k < w; k++) pos += wcnt[k][d];
-        store_fe<Fr>(kout + 8 * (u64)pos, key);
+        rec_store<W>(rout, pos, r);
       }
       __syncthreads();
       u32 add = 0;
@@ -650,7 +706,7 @@ __global__ __launch_bounds__(SB) void k_rs_scatter(u32* a, u32* b, u64 n, u32 sl
   }
 }
 
-// the sorted keys into a when the last pass that ran wrote b
+// the sorted keys into a when the last full pass that ran wrote b
 __global__ __launch_bounds__(SB) void k_rs_finish(u32* a, const u32* b, u64 n, const u32* sched) {
   if (sched[RS_PAR_END] == 0) return;
   const uint4* src = reinterpret_cast<const uint4*>(b);
@@ -661,7 +717,7 @@ __global__ __launch_bounds__(SB) void k_rs_finish(u32* a, const u32* b, u64 n, c
 
 struct SortPlan {
   u32 tiles;
-  size_t cnt, sched, scan, bytes;  // offsets within the sort's own area
+  size_t cnt, sched, scan, rec0, rec1, bytes;  // offsets within the sort's own area
 };
 
 static void sort_plan(u64 n, SortPlan* p) {
@@ -669,39 +725,46 @@ static void sort_plan(u64 n, SortPlan* p) {
   p->cnt = 0;
   p->sched = align_up((size_t)p->tiles * 256 * 4, 256);
   p->scan = p->sched + align_up(RS_WORDS * 4, 256);
-  p->bytes = p->scan + u32scan_bytes((u64)p->tiles * 256);
+  p->rec0 = p->scan + align_up(u32scan_bytes((u64)p->tiles * 256), 256);
+  p->rec1 = p->rec0 + align_up((size_t)n * RREC * 4, 256);
+  p->bytes = p->rec1 + align_up((size_t)n * RREC * 4, 256);
 }
 
-// sorts n >= 1 keys in a (b: a second buffer of n keys); the result is in a
+// sorts the n >= 1 keys in b into a (b is clobbered)
 static hipError_t sort256(u32* a, u32* b, u64 n, void* area, hipStream_t stream) {
   SortPlan p;
   sort_plan(n, &p);
   char* A = static_cast<char*>(area);
   u32* cnt = reinterpret_cast<u32*>(A + p.cnt);
   u32* sched = reinterpret_cast<u32*>(A + p.sched);
+  u32* r0 = reinterpret_cast<u32*>(A + p.rec0);
+  u32* r1 = reinterpret_cast<u32*>(A + p.rec1);
   const u32 grid = std::min(p.tiles, RS_GRID);
   const u32 egrid = (u32)std::min<u64>((n + SB - 1) / SB, RS_GRID);
-  auto pass = [&](u32 slot) -> hipError_t {
-    hipLaunchKernelGGL(k_rs_hist, dim3(grid), dim3(SB), 0, stream, a, b, n, slot, sched, cnt,
-                       p.tiles);
+  auto pass = [&](u32 slot, auto hist, auto scatter, u32* x, u32* y) -> hipError_t {
+    hipLaunchKernelGGL(hist, dim3(grid), dim3(SB), 0, stream, x, y, n, slot, sched, cnt, p.tiles);
     IMP_CHECK(hipGetLastError());
     IMP_CHECK(u32scan(cnt, (u64)p.tiles * 256, A + p.scan, stream, sched + 2 * slot));
-    hipLaunchKernelGGL(k_rs_scatter, dim3(grid), dim3(SB), 0, stream, a, b, n, slot, sched, cnt,
+    hipLaunchKernelGGL(scatter, dim3(grid), dim3(SB), 0, stream, x, y, n, slot, sched, cnt,
                        p.tiles);
     return hipGetLastError();
   };
   hipLaunchKernelGGL(k_rs_init, dim3(1), dim3(128), 0, stream, sched);
   IMP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_rs_bits, dim3(egrid), dim3(SB), 0, stream, a, n, sched);
+  hipLaunchKernelGGL(k_rs_bits, dim3(egrid), dim3(SB), 0, stream, b, n, sched, r0);
   IMP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_rs_plan, dim3(1), dim3(64), 0, stream, sched, 0u);
   IMP_CHECK(hipGetLastError());
-  for (u32 slot = 0; slot < RPREFIX; slot++) IMP_CHECK(pass(slot));
-  hipLaunchKernelGGL(k_rs_ties, dim3(egrid), dim3(SB), 0, stream, a, b, n, sched);
+  for (u32 slot = 0; slot < RPREFIX; slot++)
+    IMP_CHECK(pass(slot, k_rs_hist<RREC>, k_rs_scatter<RREC>, r0, r1));
+  hipLaunchKernelGGL(k_rs_gather, dim3(egrid), dim3(SB), 0, stream, r0, r1, b, n, sched, a);
+  IMP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_rs_ties, dim3(egrid), dim3(SB), 0, stream, a, n, sched);
   IMP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_rs_plan, dim3(1), dim3(64), 0, stream, sched, 1u);
   IMP_CHECK(hipGetLastError());
-  for (u32 slot = RPREFIX; slot < RSLOTS; slot++) IMP_CHECK(pass(slot));
+  for (u32 slot = RPREFIX; slot < RSLOTS; slot++)
+    IMP_CHECK(pass(slot, k_rs_hist<8>, k_rs_scatter<8>, a, b));
   hipLaunchKernelGGL(k_rs_finish, dim3(egrid), dim3(SB), 0, stream, a, b, n, sched);
   return hipGetLastError();
 }
@@ -1072,12 +1135,13 @@ extern "C" hipError_t permute(void* temp_buffer, unsigned long* buffer_size, uns
   auto at = [&](size_t off) { return reinterpret_cast<u32*>(T + off); };
   const u64 n = usable;
   const u32 g = (u32)((n + 255) / 256);
-  hipLaunchKernelGGL(k_to_number, dim3(g), dim3(256), 0, stream, vi, n, at(pp.k0));
+  // each column as numbers into k2, sorted from there into k0 (inputs) and
+  // k1 (table)
+  hipLaunchKernelGGL(k_to_number, dim3(g), dim3(256), 0, stream, vi, n, at(pp.k2));
   IMP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_to_number, dim3(g), dim3(256), 0, stream, vt, n, at(pp.k1));
-  IMP_CHECK(hipGetLastError());
-  // sorted inputs end in k0 (ping-pong with k2), sorted table in k1 (with k2)
   IMP_CHECK(sort256(at(pp.k0), at(pp.k2), n, T + pp.sort, stream));
+  hipLaunchKernelGGL(k_to_number, dim3(g), dim3(256), 0, stream, vt, n, at(pp.k2));
+  IMP_CHECK(hipGetLastError());
   IMP_CHECK(sort256(at(pp.k1), at(pp.k2), n, T + pp.sort, stream));
   hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(256), 0, stream, at(pp.avail), n, 1u);
   IMP_CHECK(hipGetLastError());
