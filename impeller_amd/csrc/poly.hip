@@ -706,6 +706,130 @@ __global__ __launch_bounds__(SB) void k_rs_scatter(u32* a, u32* b, u64 n, u32 sl
   }
 }
 
+// Prefix-record passes: each thread takes the run of SK = 8 consecutive
+// records [8t, 8t + 8) of the tile; two stable 4-bit splits in LDS (counts
+// per (field value, thread), one block scan, ranks) order the tile by the
+// pass's digit, input order kept within a digit; then consecutive threads
+// write consecutive positions of each digit's run, record i of digit d to
+// gbase[d] + (i - dbase[d]) (gbase: the digit's scanned global start for
+// this tile, dbase: its first staged position).
+struct RsSplitLds {
+  union {
+    u32 cnt[16 * SB];  // split counts, then running ranks (swizzled)
+    struct {
+      u32 gbase[256];
+      u32 dbase[256];
+    } run;
+  };
+  u32 part[SB / 64];
+  u32 st[RREC][STILE];  // the staged tile, one plane per record word (swizzled)
+};
+
+// LDS swizzles: a thread's run of 16 counters / 8 staged words lands in
+// distinct banks across a wave, and 64 consecutive words stay distinct
+__device__ __forceinline__ u32 cswz(u32 i) { return i ^ ((i >> 6) & 15u); }
+__device__ __forceinline__ u32 sswz(u32 i) { return i ^ ((i >> 6) & 7u); }
+
+__device__ __forceinline__ void rs_split(RsSplitLds& L, const u32 (&r)[RREC][SK], u32 n, u32 dw,
+                                         u32 sh) {
+  const u32 t = threadIdx.x;
+#pragma unroll
+  for (u32 d = 0; d < 16; d++) L.cnt[cswz(d * SB + t)] = 0;
+#pragma unroll
+  for (u32 j = 0; j < SK; j++)
+    if (j < n) L.cnt[cswz(((r[dw][j] >> sh) & 15u) * SB + t)] += 1;
+  __syncthreads();
+  u32 loc[16], sum = 0;
+#pragma unroll
+  for (u32 k = 0; k < 16; k++) {
+    loc[k] = L.cnt[cswz(16 * t + k)];
+    sum += loc[k];
+  }
+  u32 total;
+  u32 run = block_excl_u32(sum, L.part, &total);
+#pragma unroll
+  for (u32 k = 0; k < 16; k++) {
+    L.cnt[cswz(16 * t + k)] = run;
+    run += loc[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (u32 j = 0; j < SK; j++) {
+    if (j < n) {
+      const u32 c = cswz(((r[dw][j] >> sh) & 15u) * SB + t);
+      const u32 rk = L.cnt[c];
+      L.cnt[c] = rk + 1;
+#pragma unroll
+      for (u32 w = 0; w < RREC; w++) L.st[w][sswz(rk)] = r[w][j];
+    }
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(SB) void k_rs_scatter_rec(u32* a, u32* b, u64 n, u32 slot,
+                                                       const u32* sched, const u32* off,
+                                                       u32 tiles) {
+  if (sched[2 * slot] == 0) return;
+  const u32* rin = sched[2 * slot + 1] ? b : a;
+  u32* rout = sched[2 * slot + 1] ? a : b;
+  const u32 pass = slot_pass(slot), dw = digit_word<RREC>(pass), sh = (pass & 3) * 8;
+  __shared__ RsSplitLds L;
+  const u32 t = threadIdx.x;
+  for (u32 tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const u64 tile0 = (u64)tile * STILE;
+    const u32 n_tile = (u32)(n - tile0 < STILE ? n - tile0 : STILE);
+    const u32 nr = n_tile > SK * t ? (n_tile - SK * t < SK ? n_tile - SK * t : SK) : 0;
+    u32 r[RREC][SK];
+    const u32* src = rin + RREC * (tile0 + (u64)SK * t);
+    if (nr == SK) {
+      const uint4* q = reinterpret_cast<const uint4*>(src);
+      u32 v[RREC * SK];
+#pragma unroll
+      for (u32 k = 0; k < RREC * SK / 4; k++) {
+        const uint4 x = q[k];
+        v[4 * k] = x.x;
+        v[4 * k + 1] = x.y;
+        v[4 * k + 2] = x.z;
+        v[4 * k + 3] = x.w;
+      }
+#pragma unroll
+      for (u32 j = 0; j < SK; j++)
+#pragma unroll
+        for (u32 w = 0; w < RREC; w++) r[w][j] = v[RREC * j + w];
+    } else {
+#pragma unroll
+      for (u32 j = 0; j < SK; j++)
+#pragma unroll
+        for (u32 w = 0; w < RREC; w++) r[w][j] = j < nr ? src[RREC * j + w] : 0u;
+    }
+    __syncthreads();  // the previous tile's writes out of LDS are done
+    rs_split(L, r, nr, dw, sh);
+#pragma unroll
+    for (u32 j = 0; j < SK; j++)
+#pragma unroll
+      for (u32 w = 0; w < RREC; w++) r[w][j] = L.st[w][sswz(SK * t + j)];
+    __syncthreads();
+    rs_split(L, r, nr, dw, sh + 4);
+    // digit run starts (the split counters are dead: gbase / dbase reuse them)
+    L.run.gbase[t] = off[(u64)t * tiles + tile];
+#pragma unroll
+    for (u32 j = 0; j < SK; j++) {
+      const u32 i = SK * t + j;
+      if (i < n_tile) {
+        const u32 d = (L.st[dw][sswz(i)] >> sh) & 255u;
+        if (i == 0 || ((L.st[dw][sswz(i - 1)] >> sh) & 255u) != d) L.run.dbase[d] = i;
+      }
+    }
+    __syncthreads();
+    for (u32 i = t; i < n_tile; i += SB) {
+      const u32 d = (L.st[dw][sswz(i)] >> sh) & 255u;
+      const u64 pos = (u64)L.run.gbase[d] + i - L.run.dbase[d];
+#pragma unroll
+      for (u32 w = 0; w < RREC; w++) rout[RREC * pos + w] = L.st[w][sswz(i)];
+    }
+  }
+}
+
 // the sorted keys into a when the last full pass that ran wrote b
 __global__ __launch_bounds__(SB) void k_rs_finish(u32* a, const u32* b, u64 n, const u32* sched) {
   if (sched[RS_PAR_END] == 0) return;
@@ -756,7 +880,7 @@ static hipError_t sort256(u32* a, u32* b, u64 n, void* area, hipStream_t stream)
   hipLaunchKernelGGL(k_rs_plan, dim3(1), dim3(64), 0, stream, sched, 0u);
   IMP_CHECK(hipGetLastError());
   for (u32 slot = 0; slot < RPREFIX; slot++)
-    IMP_CHECK(pass(slot, k_rs_hist<RREC>, k_rs_scatter<RREC>, r0, r1));
+    IMP_CHECK(pass(slot, k_rs_hist<RREC>, k_rs_scatter_rec, r0, r1));
   hipLaunchKernelGGL(k_rs_gather, dim3(egrid), dim3(SB), 0, stream, r0, r1, b, n, sched, a);
   IMP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_rs_ties, dim3(egrid), dim3(SB), 0, stream, a, n, sched);
