@@ -573,18 +573,21 @@ __global__ __launch_bounds__(SB) void k_rs_gather(const u32* r0, const u32* r1, 
 }
 
 // RS_TIES when two adjacent prefix-sorted keys agree in their top 8 bytes
-// but not in the rest
+// but not in the rest (one atomic per wave that found such a pair: on a
+// table of small integers every pair does)
 __global__ __launch_bounds__(SB) void k_rs_ties(const u32* k, u64 n, u32* sched) {
+  bool tie = false;
   for (u64 i = 1 + (u64)blockIdx.x * SB + threadIdx.x; i < n; i += (u64)gridDim.x * SB) {
     const u32* p = k + 8 * i;
     const u32* q = p - 8;
     if (p[7] == q[7] && p[6] == q[6]) {
-      bool diff = false;
 #pragma unroll
-      for (u32 w = 0; w < 6; w++) diff |= p[w] != q[w];
-      if (diff) atomicOr(&sched[RS_TIES], 1u);
+      for (u32 w = 0; w < 6; w++) tie |= p[w] != q[w];
     }
   }
+  const unsigned long long any = __ballot(tie);
+  if (any != 0 && (threadIdx.x & 63) == (u32)(__ffsll((long long)any) - 1))
+    atomicOr(&sched[RS_TIES], 1u);
 }
 
 // one record of W words: a 32-byte key (W = 8) or a prefix record (W = 3,
@@ -713,6 +716,7 @@ __global__ __launch_bounds__(SB) void k_rs_scatter(u32* a, u32* b, u64 n, u32 sl
 // write consecutive positions of each digit's run, record i of digit d to
 // gbase[d] + (i - dbase[d]) (gbase: the digit's scanned global start for
 // this tile, dbase: its first staged position).
+template <u32 W>
 struct RsSplitLds {
   union {
     u32 cnt[16 * SB];  // split counts, then running ranks (swizzled)
@@ -721,23 +725,35 @@ struct RsSplitLds {
       u32 dbase[256];
     } run;
   };
-  u32 part[SB / 64];
-  u32 st[RREC][STILE];  // the staged tile, one plane per record word (swizzled)
-};
+  u32 st[W][STILE];  // the staged tile, one plane per record word (swizzled)
+};  // 80 KiB for 32-byte keys: two work-groups share a CU's 160 KiB
 
 // LDS swizzles: a thread's run of 16 counters / 8 staged words lands in
 // distinct banks across a wave, and 64 consecutive words stay distinct
 __device__ __forceinline__ u32 cswz(u32 i) { return i ^ ((i >> 6) & 15u); }
 __device__ __forceinline__ u32 sswz(u32 i) { return i ^ ((i >> 6) & 7u); }
 
-__device__ __forceinline__ void rs_split(RsSplitLds& L, const u32 (&r)[RREC][SK], u32 n, u32 dw,
+template <u32 W>
+__device__ __forceinline__ void rs_split(RsSplitLds<W>& L, const u32 (&r)[W][SK], u32 n, u32 dw,
                                          u32 sh) {
   const u32 t = threadIdx.x;
+  // the digit words, picked with constant indices so r stays in registers
+  u32 dg[SK];
+  u32 sel[W];
+#pragma unroll
+  for (u32 w = 0; w < W; w++) sel[w] = 0u - (u32)(dw == w);
+#pragma unroll
+  for (u32 j = 0; j < SK; j++) {
+    u32 x = 0;
+#pragma unroll
+    for (u32 w = 0; w < W; w++) x |= r[w][j] & sel[w];
+    dg[j] = x;
+  }
 #pragma unroll
   for (u32 d = 0; d < 16; d++) L.cnt[cswz(d * SB + t)] = 0;
 #pragma unroll
   for (u32 j = 0; j < SK; j++)
-    if (j < n) L.cnt[cswz(((r[dw][j] >> sh) & 15u) * SB + t)] += 1;
+    if (j < n) L.cnt[cswz(((dg[j] >> sh) & 15u) * SB + t)] += 1;
   __syncthreads();
   u32 loc[16], sum = 0;
 #pragma unroll
@@ -746,7 +762,10 @@ __device__ __forceinline__ void rs_split(RsSplitLds& L, const u32 (&r)[RREC][SK]
     sum += loc[k];
   }
   u32 total;
-  u32 run = block_excl_u32(sum, L.part, &total);
+  // the scan's four wave sums go to the first words of the staging planes:
+  // no thread reads or writes those planes until the scan's last barrier
+  // (each run was taken into registers before the split began)
+  u32 run = block_excl_u32(sum, &L.st[0][0], &total);
 #pragma unroll
   for (u32 k = 0; k < 16; k++) {
     L.cnt[cswz(16 * t + k)] = run;
@@ -756,60 +775,56 @@ __device__ __forceinline__ void rs_split(RsSplitLds& L, const u32 (&r)[RREC][SK]
 #pragma unroll
   for (u32 j = 0; j < SK; j++) {
     if (j < n) {
-      const u32 c = cswz(((r[dw][j] >> sh) & 15u) * SB + t);
+      const u32 c = cswz(((dg[j] >> sh) & 15u) * SB + t);
       const u32 rk = L.cnt[c];
       L.cnt[c] = rk + 1;
 #pragma unroll
-      for (u32 w = 0; w < RREC; w++) L.st[w][sswz(rk)] = r[w][j];
+      for (u32 w = 0; w < W; w++) L.st[w][sswz(rk)] = r[w][j];
     }
   }
   __syncthreads();
 }
 
-__global__ __launch_bounds__(SB) void k_rs_scatter_rec(u32* a, u32* b, u64 n, u32 slot,
-                                                       const u32* sched, const u32* off,
-                                                       u32 tiles) {
+template <u32 W>
+__global__ __launch_bounds__(SB) void k_rs_scatter_split(u32* a, u32* b, u64 n, u32 slot,
+                                                         const u32* sched, const u32* off,
+                                                         u32 tiles) {
   if (sched[2 * slot] == 0) return;
   const u32* rin = sched[2 * slot + 1] ? b : a;
   u32* rout = sched[2 * slot + 1] ? a : b;
-  const u32 pass = slot_pass(slot), dw = digit_word<RREC>(pass), sh = (pass & 3) * 8;
-  __shared__ RsSplitLds L;
+  const u32 pass = slot_pass(slot), dw = digit_word<W>(pass), sh = (pass & 3) * 8;
+  __shared__ RsSplitLds<W> L;
   const u32 t = threadIdx.x;
   for (u32 tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
     const u64 tile0 = (u64)tile * STILE;
     const u32 n_tile = (u32)(n - tile0 < STILE ? n - tile0 : STILE);
     const u32 nr = n_tile > SK * t ? (n_tile - SK * t < SK ? n_tile - SK * t : SK) : 0;
-    u32 r[RREC][SK];
-    const u32* src = rin + RREC * (tile0 + (u64)SK * t);
+    u32 r[W][SK];
+    const u32* src = rin + W * (tile0 + (u64)SK * t);
     if (nr == SK) {
+      // the run's W * SK words are 16-byte aligned (W * SK * 4 is a multiple of 16)
       const uint4* q = reinterpret_cast<const uint4*>(src);
-      u32 v[RREC * SK];
 #pragma unroll
-      for (u32 k = 0; k < RREC * SK / 4; k++) {
+      for (u32 k = 0; k < W * SK / 4; k++) {
         const uint4 x = q[k];
-        v[4 * k] = x.x;
-        v[4 * k + 1] = x.y;
-        v[4 * k + 2] = x.z;
-        v[4 * k + 3] = x.w;
+        const u32 e[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (u32 m = 0; m < 4; m++) r[(4 * k + m) % W][(4 * k + m) / W] = e[m];
       }
-#pragma unroll
-      for (u32 j = 0; j < SK; j++)
-#pragma unroll
-        for (u32 w = 0; w < RREC; w++) r[w][j] = v[RREC * j + w];
     } else {
 #pragma unroll
       for (u32 j = 0; j < SK; j++)
 #pragma unroll
-        for (u32 w = 0; w < RREC; w++) r[w][j] = j < nr ? src[RREC * j + w] : 0u;
+        for (u32 w = 0; w < W; w++) r[w][j] = j < nr ? src[W * j + w] : 0u;
     }
     __syncthreads();  // the previous tile's writes out of LDS are done
-    rs_split(L, r, nr, dw, sh);
+    rs_split<W>(L, r, nr, dw, sh);
 #pragma unroll
     for (u32 j = 0; j < SK; j++)
 #pragma unroll
-      for (u32 w = 0; w < RREC; w++) r[w][j] = L.st[w][sswz(SK * t + j)];
+      for (u32 w = 0; w < W; w++) r[w][j] = L.st[w][sswz(SK * t + j)];
     __syncthreads();
-    rs_split(L, r, nr, dw, sh + 4);
+    rs_split<W>(L, r, nr, dw, sh + 4);
     // digit run starts (the split counters are dead: gbase / dbase reuse them)
     L.run.gbase[t] = off[(u64)t * tiles + tile];
 #pragma unroll
@@ -824,8 +839,15 @@ __global__ __launch_bounds__(SB) void k_rs_scatter_rec(u32* a, u32* b, u64 n, u3
     for (u32 i = t; i < n_tile; i += SB) {
       const u32 d = (L.st[dw][sswz(i)] >> sh) & 255u;
       const u64 pos = (u64)L.run.gbase[d] + i - L.run.dbase[d];
+      if constexpr (W == 8) {
+        Fr v;
 #pragma unroll
-      for (u32 w = 0; w < RREC; w++) rout[RREC * pos + w] = L.st[w][sswz(i)];
+        for (u32 w = 0; w < 8; w++) v.v[w] = L.st[w][sswz(i)];
+        store_fe<Fr>(rout + 8 * pos, v);
+      } else {
+#pragma unroll
+        for (u32 w = 0; w < W; w++) rout[W * pos + w] = L.st[w][sswz(i)];
+      }
     }
   }
 }
@@ -880,7 +902,7 @@ static hipError_t sort256(u32* a, u32* b, u64 n, void* area, hipStream_t stream)
   hipLaunchKernelGGL(k_rs_plan, dim3(1), dim3(64), 0, stream, sched, 0u);
   IMP_CHECK(hipGetLastError());
   for (u32 slot = 0; slot < RPREFIX; slot++)
-    IMP_CHECK(pass(slot, k_rs_hist<RREC>, k_rs_scatter_rec, r0, r1));
+    IMP_CHECK(pass(slot, k_rs_hist<RREC>, k_rs_scatter_split<RREC>, r0, r1));
   hipLaunchKernelGGL(k_rs_gather, dim3(egrid), dim3(SB), 0, stream, r0, r1, b, n, sched, a);
   IMP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_rs_ties, dim3(egrid), dim3(SB), 0, stream, a, n, sched);
@@ -888,7 +910,7 @@ static hipError_t sort256(u32* a, u32* b, u64 n, void* area, hipStream_t stream)
   hipLaunchKernelGGL(k_rs_plan, dim3(1), dim3(64), 0, stream, sched, 1u);
   IMP_CHECK(hipGetLastError());
   for (u32 slot = RPREFIX; slot < RSLOTS; slot++)
-    IMP_CHECK(pass(slot, k_rs_hist<8>, k_rs_scatter<8>, a, b));
+    IMP_CHECK(pass(slot, k_rs_hist<8>, k_rs_scatter_split<8>, a, b));
   hipLaunchKernelGGL(k_rs_finish, dim3(egrid), dim3(SB), 0, stream, a, b, n, sched);
   return hipGetLastError();
 }
