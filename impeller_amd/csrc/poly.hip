@@ -469,9 +469,8 @@ static hipError_t u32scan(u32* data, u64 n, void* temp, hipStream_t stream,
 // LSD over the 32 bytes of canonical 256-bit numbers (32-byte records),
 // ascending, stable.  Per pass: a digit histogram per tile (digit-major, so
 // one exclusive scan gives every tile's stable output offsets), then a
-// stable scatter ranking each tile's records round by round: within a wave
-// the records with the same digit are found by ballots over the digit's
-// bits, across the work-group's waves by per-wave digit counts in LDS.
+// stable scatter (k_rs_scatter_split: the tile ordered by the digit in LDS,
+// written out by digit runs).
 //
 // Fewer passes and fewer bytes where the keys allow, decided on the device
 // (no host synchronisation).  Prefix phase (slots 0..7, passes 24..31): the
@@ -590,40 +589,6 @@ __global__ __launch_bounds__(SB) void k_rs_ties(const u32* k, u64 n, u32* sched)
     atomicOr(&sched[RS_TIES], 1u);
 }
 
-// one record of W words: a 32-byte key (W = 8) or a prefix record (W = 3,
-// key words 6 and 7 first)
-template <u32 W>
-struct RsRec {
-  u32 w[W];
-};
-
-template <u32 W>
-__device__ __forceinline__ RsRec<W> rec_load(const u32* base, u64 i) {
-  RsRec<W> r;
-  if constexpr (W == 8) {
-    const Fr k = load_fe<Fr>(base + 8 * i);
-#pragma unroll
-    for (u32 k2 = 0; k2 < 8; k2++) r.w[k2] = k.v[k2];
-  } else {
-#pragma unroll
-    for (u32 k2 = 0; k2 < W; k2++) r.w[k2] = base[W * i + k2];
-  }
-  return r;
-}
-
-template <u32 W>
-__device__ __forceinline__ void rec_store(u32* base, u64 i, const RsRec<W>& r) {
-  if constexpr (W == 8) {
-    Fr k;
-#pragma unroll
-    for (u32 k2 = 0; k2 < 8; k2++) k.v[k2] = r.w[k2];
-    store_fe<Fr>(base + 8 * i, k);
-  } else {
-#pragma unroll
-    for (u32 k2 = 0; k2 < W; k2++) base[W * i + k2] = r.w[k2];
-  }
-}
-
 // the pass's digit: byte (pass & 3) of key word pass >> 2 (prefix records
 // hold key words 6 and 7 as their words 0 and 1)
 template <u32 W>
@@ -654,58 +619,6 @@ __global__ __launch_bounds__(SB) void k_rs_hist(const u32* a, const u32* b, u64 
     __syncthreads();
     cnt[(u64)t * tiles + tile] = h[t];
     __syncthreads();
-  }
-}
-
-template <u32 W>
-__global__ __launch_bounds__(SB) void k_rs_scatter(u32* a, u32* b, u64 n, u32 slot,
-                                                   const u32* sched, const u32* off, u32 tiles) {
-  if (sched[2 * slot] == 0) return;
-  const u32* rin = sched[2 * slot + 1] ? b : a;
-  u32* rout = sched[2 * slot + 1] ? a : b;
-  const u32 pass = slot_pass(slot), dw = digit_word<W>(pass);
-  __shared__ u32 base[256];
-  __shared__ u32 wcnt[SB / 64][256];
-  const u32 t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const u64 below = lane ? ((1ull << lane) - 1) : 0ull;
-  for (u32 tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-    base[t] = off[(u64)t * tiles + tile];
-#pragma unroll
-    for (u32 k = 0; k < SB / 64; k++) wcnt[k][t] = 0;
-    __syncthreads();
-    for (u32 j = 0; j < SK; j++) {
-      const u64 i = (u64)tile * STILE + (u64)j * SB + t;
-      const bool valid = i < n;
-      RsRec<W> r = {};
-      u32 d = 256;  // lanes past the end form a group of their own
-      if (valid) {
-        r = rec_load<W>(rin, i);
-        d = rec_digit<W>(r.w[dw], pass);
-      }
-      unsigned long long m = ~0ull;
-#pragma unroll
-      for (u32 bt = 0; bt < 9; bt++) {
-        const unsigned long long bal = __ballot((d >> bt) & 1);
-        m &= ((d >> bt) & 1) ? bal : ~bal;
-      }
-      const u32 rank = __popcll(m & below);
-      if (valid && rank == 0) wcnt[w][d] = __popcll(m);
-      __syncthreads();
-      if (valid) {
-        u32 pos = base[d] + rank;
-        for (u32 k = 0; k < w; k++) pos += wcnt[k][d];
-        rec_store<W>(rout, pos, r);
-      }
-      __syncthreads();
-      u32 add = 0;
-#pragma unroll
-      for (u32 k = 0; k < SB / 64; k++) {
-        add += wcnt[k][t];
-        wcnt[k][t] = 0;
-      }
-      base[t] += add;
-      __syncthreads();
-    }
   }
 }
 
