@@ -612,6 +612,97 @@ __device__ __forceinline__ G1Xyzz xyzz_add(const G1Xyzz& p, const G1Xyzz& q) {
   return r;
 }
 
+// ------------------------------------------- group-of-four (quad) forms
+// For chains of dependent point operations on few lanes (the window sums'
+// doubling chains and Horner), where one wave's latency sets the time: the
+// four lanes of a group hold the same point(s) and split each formula's
+// products into rounds of at most four independent ones, one per lane,
+// every lane then receiving all of a round's results (quad_bcast).  The
+// formulas and their operands are those of xyzz_dbl / xyzz_add, so the
+// coordinates are identical.  q = lane % 4; every lane of the group calls
+// with the same points.
+// v as held by lane 4 * (lane / 4) + K, for every lane of each group of four
+// consecutive lanes: a DPP quad permutation (one VALU move, no LDS).  Every
+// lane of a group that uses it must be active.
+template <int K>
+__device__ __forceinline__ u32 quad_bcast(u32 v) {
+  static_assert(K >= 0 && K < 4, "lane in the group");
+#if defined(IMP_EMU)
+  return (u32)::emu::quad_shfl(v, K);
+#elif defined(__HIP_DEVICE_COMPILE__)
+  return (u32)__builtin_amdgcn_mov_dpp((int)v, K * 0x55, 0xF, 0xF, false);
+#else
+  return v;
+#endif
+}
+
+template <int K>
+__device__ __forceinline__ Fq quad_fe(const Fq& a) {
+  Fq r;
+#pragma unroll
+  for (int i = 0; i < 8; i++) r.v[i] = quad_bcast<K>(a.v[i]);
+  return r;
+}
+// a_q, by masks: a chain of selects on q is turned by the compiler into an
+// index into a private array (scratch memory)
+__device__ __forceinline__ Fq pick4(u32 q, const Fq& a0, const Fq& a1, const Fq& a2, const Fq& a3) {
+  const u32 m0 = 0u - (u32)(q == 0), m1 = 0u - (u32)(q == 1), m2 = 0u - (u32)(q == 2),
+            m3 = 0u - (u32)(q == 3);
+  Fq r;
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+    r.v[i] = (a0.v[i] & m0) | (a1.v[i] & m1) | (a2.v[i] & m2) | (a3.v[i] & m3);
+  return r;
+}
+
+// dbl-2008-s-1 in three rounds: {u^2, x^2}, {u v, x v, v zz, m^2},
+// {m (s - x3), w y, w zzz}
+__device__ __forceinline__ G1Xyzz xyzz_dbl_quad(const G1Xyzz& p, u32 q) {
+  typedef FqParams Q;
+  if (p.is_identity()) return p;
+  const Fq u = dbl<Q>(p.y);
+  const Fq a1 = pick4(q, u, p.x, u, p.x);
+  const Fq r1 = mul<Q>(a1, a1);
+  const Fq v = quad_fe<0>(r1), x2 = quad_fe<1>(r1);
+  const Fq m = add<Q>(dbl<Q>(x2), x2);
+  const Fq r2 = mul<Q>(pick4(q, u, p.x, p.zz, m), pick4(q, v, v, v, m));
+  const Fq w = quad_fe<0>(r2), s = quad_fe<1>(r2), zz3 = quad_fe<2>(r2), m2 = quad_fe<3>(r2);
+  const Fq x3 = sub<Q>(sub<Q>(m2, s), s);
+  const Fq r3 = mul<Q>(pick4(q, m, w, w, w), pick4(q, sub<Q>(s, x3), p.y, p.zzz, p.zzz));
+  G1Xyzz r;
+  r.x = x3;
+  r.y = sub<Q>(quad_fe<0>(r3), quad_fe<1>(r3));
+  r.zz = zz3;
+  r.zzz = quad_fe<2>(r3);
+  return r;
+}
+
+// add-2008-s in four rounds: {u1, u2, s1, s2}, {P^2, R^2, zz1 zz2,
+// zzz1 zzz2}, {P pp, u1 pp, zz12 pp}, {R (qq - x3), s1 ppp, zzz12 ppp}
+__device__ __forceinline__ G1Xyzz xyzz_add_quad(const G1Xyzz& p, const G1Xyzz& o, u32 q) {
+  typedef FqParams Q;
+  if (p.is_identity()) return o;
+  if (o.is_identity()) return p;
+  const Fq r1 = mul<Q>(pick4(q, p.x, o.x, p.y, o.y), pick4(q, o.zz, p.zz, o.zzz, p.zzz));
+  const Fq u1 = quad_fe<0>(r1), u2 = quad_fe<1>(r1), s1 = quad_fe<2>(r1), s2 = quad_fe<3>(r1);
+  const Fq P_ = sub<Q>(u2, u1);
+  const Fq R_ = sub<Q>(s2, s1);
+  if (P_.is_zero() && R_.is_zero()) return xyzz_dbl_quad(p, q);
+  const Fq r2 = mul<Q>(pick4(q, P_, R_, p.zz, p.zzz), pick4(q, P_, R_, o.zz, o.zzz));
+  const Fq pp = quad_fe<0>(r2), rr = quad_fe<1>(r2), zz12 = quad_fe<2>(r2),
+           zzz12 = quad_fe<3>(r2);
+  const Fq r3 = mul<Q>(pick4(q, P_, u1, zz12, zz12), pp);
+  const Fq ppp = quad_fe<0>(r3), qq = quad_fe<1>(r3), zz3 = quad_fe<2>(r3);
+  const Fq x3 = sub<Q>(sub<Q>(sub<Q>(rr, ppp), qq), qq);
+  const Fq r4 = mul<Q>(pick4(q, R_, s1, zzz12, zzz12), pick4(q, sub<Q>(qq, x3), ppp, ppp, ppp));
+  G1Xyzz r;
+  r.x = x3;
+  r.y = sub<Q>(quad_fe<0>(r4), quad_fe<1>(r4));
+  r.zz = zz3;
+  r.zzz = quad_fe<2>(r4);
+  return r;
+}
+
 __device__ __forceinline__ G1Aff aff_neg(const G1Aff& a) {
   G1Aff r;
   r.x = a.x;

@@ -884,7 +884,19 @@ __device__ __forceinline__ void sum_body(const MsmJob& J, u32 level, u64 lb, u32
 }
 
 // one work-group per job (blockDim.x == 256); the shape guarantees
-// n_windows * (lbits + sbits) * fin_g <= 256
+// n_windows * (lbits + sbits) * fin_g <= 256.  Everything here is a chain of
+// dependent EC operations on a few lanes, where one wave's latency counts,
+// not the chip's product rate: the values are converted to the 8 x 32-bit
+// form as they are loaded, and each value is worked on by a group of four
+// lanes (xyzz_dbl_quad / xyzz_add_quad: three or four rounds of products
+// instead of nine or fourteen in a row) wherever the work-group has four
+// lanes per value; otherwise by one lane.  Horner over the physical windows
+// always runs on lanes 0..3.  An 8 x 32-bit XYZZ takes 32 of a red slot's
+// kPt words; the group's lane 0 writes the slot, after a barrier (every lane
+// has read its operands by then).
+__device__ __forceinline__ G1Xyzz final_add(const G1Xyzz& a, const G1Xyzz& b, bool quad, u32 q) {
+  return quad ? xyzz_add_quad(a, b, q) : xyzz_add(a, b);
+}
 __device__ __forceinline__ void final_body(const MsmJob& J, u32* red) {
   const MsmShape& s = J.s;
   const u32 t = threadIdx.x;
@@ -896,54 +908,64 @@ __device__ __forceinline__ void final_body(const MsmJob& J, u32* red) {
   const u32 nv = s.lbits + s.sbits, G = s.fin_g, nval = s.n_windows * nv;
   const u32* parts = s.sum_levels == 2 ? J.sparts1 : J.sparts;
   // value (pw, i) at red[(pw * nv + i) * G], its sum partials in the next G - 1 slots
-  if (t < nval * G) {
-    const u32 gi = t % G, i = (t / G) % nv, pw = t / (G * nv);
-    G1Xyzz29 v = G1Xyzz29::identity();
-    if (i == 0 || i >= s.sbits) {
-      const u32 q = i == 0 ? 0 : i - s.sbits + 1;
-      v = load_xyzz29_g(parts + kPt * (((u64)pw * (s.lbits + 1) + q) * G + gi));
-    }
-    store_xyzz29(red + kPt * t, v);
-  }
-  __syncthreads();
-  for (u32 d = 1; d < G; d <<= 1) {
-    if (t < nval * G && (t % (2 * d)) == 0) {
-      G1Xyzz29 a = load_xyzz29(red + kPt * t), b = load_xyzz29(red + kPt * (t + d));
-      store_xyzz29(red + kPt * t, xyzz29_add(a, b));
+  {
+    const bool quad = nval * G * 4 <= 256;
+    const u32 q = quad ? t & 3 : 0, vt = quad ? t >> 2 : t;
+    if (vt < nval * G && q == 0) {
+      const u32 gi = vt % G, i = (vt / G) % nv, pw = vt / (G * nv);
+      G1Xyzz v = G1Xyzz::identity();
+      if (i == 0 || i >= s.sbits) {
+        const u32 qs = i == 0 ? 0 : i - s.sbits + 1;
+        v = xyzz_from_29(load_xyzz29_g(parts + kPt * (((u64)pw * (s.lbits + 1) + qs) * G + gi)));
+      }
+      store_xyzz(red + kPt * vt, v);
     }
     __syncthreads();
+    for (u32 d = 1; d < G; d <<= 1) {
+      const bool live = vt < nval * G && (vt % (2 * d)) == 0;
+      G1Xyzz r;
+      if (live) r = final_add(load_xyzz(red + kPt * vt), load_xyzz(red + kPt * (vt + d)), quad, q);
+      __syncthreads();  // every lane of a group has read its slots
+      if (live && q == 0) store_xyzz(red + kPt * vt, r);
+      __syncthreads();
+    }
   }
-  // S = sum_i 2^i a_i per window: lane (pw, i) doubles a_i i times (the
-  // lanes run side by side, so the chain is nv - 1 doublings deep), then a
+  // S = sum_i 2^i a_i per window: value (pw, i) is doubled i times (the
+  // values side by side, so the chain is nv - 1 doublings deep), then a
   // tree adds the nv terms
-  if (t < nval) {
-    const u32 i = t % nv;
-    if (i >= s.sbits) {  // a_1 .. a_(sbits-1) are the identity
-      G1Xyzz29 b = load_xyzz29(red + kPt * (t * G));
-      for (u32 k = 0; k < i; k++) b = xyzz29_dbl(b);
-      store_xyzz29(red + kPt * (t * G), b);
-    }
-  }
-  __syncthreads();
-  for (u32 h = 1; h < nv; h <<= 1) {
-    if (t < nval) {
-      const u32 i = t % nv;
-      if ((i % (2 * h)) == 0 && i + h < nv)
-        store_xyzz29(red + kPt * (t * G), xyzz29_add(load_xyzz29(red + kPt * (t * G)),
-                                                     load_xyzz29(red + kPt * ((t + h) * G))));
+  {
+    const bool quad = nval * 4 <= 256;
+    const u32 q = quad ? t & 3 : 0, vt = quad ? t >> 2 : t;
+    // a_1 .. a_(sbits-1) are the identity
+    const bool dlive = vt < nval && vt % nv >= s.sbits;
+    G1Xyzz b;
+    if (dlive) {
+      b = load_xyzz(red + kPt * (vt * G));
+      for (u32 k = 0, i = vt % nv; k < i; k++) b = quad ? xyzz_dbl_quad(b, q) : xyzz_dbl(b);
     }
     __syncthreads();
+    if (dlive && q == 0) store_xyzz(red + kPt * (vt * G), b);
+    __syncthreads();
+    for (u32 h = 1; h < nv; h <<= 1) {
+      const bool live = vt < nval && ((vt % nv) % (2 * h)) == 0 && vt % nv + h < nv;
+      G1Xyzz r;
+      if (live)
+        r = final_add(load_xyzz(red + kPt * (vt * G)), load_xyzz(red + kPt * ((vt + h) * G)),
+                      quad, q);
+      __syncthreads();
+      if (live && q == 0) store_xyzz(red + kPt * (vt * G), r);
+      __syncthreads();
+    }
   }
-  if (t != 0) return;
-  // Horner over the physical windows on one lane, in the 8 x 32-bit form:
-  // a chain of (n_windows - 1) c dependent doublings, where that form's
-  // shorter product latency counts
-  G1Xyzz acc = xyzz_from_29(load_xyzz29(red + kPt * ((s.n_windows - 1) * nv * G)));
+  if (t >= 4) return;
+  // Horner over the physical windows on lanes 0..3: (n_windows - 1) c
+  // dependent doublings
+  G1Xyzz acc = load_xyzz(red + kPt * ((s.n_windows - 1) * nv * G));
   for (int pw = (int)s.n_windows - 2; pw >= 0; pw--) {
-    for (u32 k = 0; k < s.c; k++) acc = xyzz_dbl(acc);
-    acc = xyzz_add(acc, xyzz_from_29(load_xyzz29(red + kPt * ((u32)pw * nv * G))));
+    for (u32 k = 0; k < s.c; k++) acc = xyzz_dbl_quad(acc, t);
+    acc = xyzz_add_quad(acc, load_xyzz(red + kPt * ((u32)pw * nv * G)), t);
   }
-  store_aff(J.result, xyzz_to_aff(acc));
+  if (t == 0) store_aff(J.result, xyzz_to_aff(acc));
 }
 
 // ------------------------------------------------------- batched kernels
@@ -1053,7 +1075,7 @@ __global__ __launch_bounds__(256) void kb_sum(const MsmJob* jobs, u32 njobs, con
   IMP_BATCH_LOOP(sum_body(J, level, lb, red))
 }
 __global__ __launch_bounds__(256) void kb_final(const MsmJob* jobs, u32 njobs, const u32* prefix) {
-  __shared__ u32 red[256 * kPt];
+  __shared__ __align__(16) u32 red[256 * kPt];
   IMP_BATCH_LOOP(final_body(J, red))
 }
 
@@ -1123,16 +1145,19 @@ static u32 stage_blocks(const MsmShape& s, u32 kind, u32 arg) {
 }
 
 // part: 0 a whole MSM; 1 a part that only accumulates its bucket totals;
-// 2 the last part: accumulates, then reduces the kept sums to the result
+// 2 the last part: accumulates, then reduces the kept sums to the result.
+// The stages before *points_at read only the scalars (digits, the sort,
+// the bucket bounds and task offsets); the base conversion and everything
+// after it read the points, so a caller streaming the points in can launch
+// the first stages while the points are still being copied.
 static void plan_stages(const std::vector<MsmShape>& shapes, std::vector<StageLaunch>& out,
-                        std::vector<u32>& prefix, u32 part = 0) {
+                        std::vector<u32>& prefix, u32 part = 0, u32* points_at = nullptr) {
   u32 maxp = 0, maxl = 0;
   for (auto& s : shapes) {
     if (s.passes > maxp) maxp = s.passes;
     if (s.levels > maxl) maxl = s.levels;
   }
   std::vector<std::pair<u32, u32>> kinds;
-  kinds.push_back({MS_CONVERT, 0});
   kinds.push_back({MS_DIGITS, 0});
   auto scan3 = [&](u32 sk, u32 x) {
     kinds.push_back({MS_SCAN_A, scan_arg(sk, x)});
@@ -1147,6 +1172,8 @@ static void plan_stages(const std::vector<MsmShape>& shapes, std::vector<StageLa
   kinds.push_back({MS_ZERO, 0});
   kinds.push_back({MS_BOUNDS, 0});
   scan3(SC_TASKS, 0);
+  if (points_at) *points_at = (u32)kinds.size();
+  kinds.push_back({MS_CONVERT, 0});
   kinds.push_back({MS_TASKS, 0});
   for (u32 l = 1; l <= maxl; l++) {
     scan3(SC_LEVEL, l);
@@ -1480,13 +1507,14 @@ static hipError_t msm_host(u32 window, u32 target, void* const* buffers,
   // once, msm_impl.cuh:373-580)
   std::vector<StageLaunch> stl[2][3];
   std::vector<u32> pfl[2][3];
+  u32 pat[2][3];  // first stage that reads the points
   {
     std::vector<MsmShape> one(1, h.s);
     const u64 last_len = h.len - (u64)(h.parts - 1) * h.part_len;
     for (u32 sh = 0; sh < 2; sh++) {
       one[0] = sh ? msm_shape(last_len, window, target) : h.s;
       for (u32 m = 0; m < 3; m++) {
-        plan_stages(one, stl[sh][m], pfl[sh][m], m);
+        plan_stages(one, stl[sh][m], pfl[sh][m], m, &pat[sh][m]);
         IMP_REQUIRE(pfl[sh][m].size() <= kMaxStages * 2, "stage plan");
       }
     }
@@ -1553,34 +1581,18 @@ static hipError_t msm_host(u32 window, u32 target, void* const* buffers,
                                hipMemcpyHostToDevice, g.st));
     }
     IMP_CHECK(hipStreamSynchronize(g.st));
-    // issue copies and MSMs in operation order
+    // issue copies and MSMs in operation order.  On the copy stream an
+    // operation's scalars go first, then (for the first operation of a part
+    // not yet resident) the part's points; the compute stream runs the
+    // stages that read only the scalars (digits, sort, bucket bounds) once
+    // the scalars are in, and waits for the points before the rest, so the
+    // sort overlaps the larger point copy.
     u32 prev_part = ~0u;
     for (size_t o = 0; o < ops.size(); o++) {
       const Op& op = ops[o];
       const u32 k = op.pstage;
       const u64 off = (u64)op.part * h.part_len;
       const u64 cur = op.part + 1 == h.parts ? h.len - off : h.part_len;
-      if (op.part != prev_part) {
-        if (resident[k] != (long long)op.part) {
-          // the stage's previous part is done once its last MSM has run
-          IMP_CHECK(hipStreamWaitEvent(g.cp, pt_free[k], 0));
-          u32* dst = (u32*)(buf + h.off_pts + k * h.pts_stage);
-          for (u32 t = 0; t < h.s.n_pre; t++)
-            IMP_CHECK(hipMemcpyAsync(dst + (size_t)16 * t * h.part_len, h_points[t] + 16 * off,
-                                     (size_t)cur * 64, hipMemcpyHostToDevice, g.cp));
-          if (h.s.n_pre > 1) {
-            // table format, in place; rows past a short last part unused
-            for (u32 t = 0; t < h.s.n_pre; t++)
-              IMP_CHECK(to_table(dst + (size_t)16 * t * h.part_len,
-                                 dst + (size_t)16 * t * h.part_len, cur, g.cp));
-          }
-          IMP_CHECK(hipEventRecord(pt_ready[k], g.cp));
-          resident[k] = op.part;
-          next_pt = (k + 1) % h.sp;
-        }
-        IMP_CHECK(hipStreamWaitEvent(g.st, pt_ready[k], 0));
-        prev_part = op.part;
-      }
       const u32 sk = next_sc;
       next_sc = (next_sc + 1) % h.ss;
       IMP_CHECK(hipStreamWaitEvent(g.cp, sc_free[sk], 0));
@@ -1588,11 +1600,39 @@ static hipError_t msm_host(u32 window, u32 target, void* const* buffers,
                                h_scaler_batch[b0 + op.batch] + 8 * off, (size_t)cur * 32,
                                hipMemcpyHostToDevice, g.cp));
       IMP_CHECK(hipEventRecord(sc_ready[sk], g.cp));
+      const bool new_part = op.part != prev_part;
+      if (new_part && resident[k] != (long long)op.part) {
+        // the stage's previous part is done once its last MSM has run
+        IMP_CHECK(hipStreamWaitEvent(g.cp, pt_free[k], 0));
+        u32* dst = (u32*)(buf + h.off_pts + k * h.pts_stage);
+        for (u32 t = 0; t < h.s.n_pre; t++)
+          IMP_CHECK(hipMemcpyAsync(dst + (size_t)16 * t * h.part_len, h_points[t] + 16 * off,
+                                   (size_t)cur * 64, hipMemcpyHostToDevice, g.cp));
+        if (h.s.n_pre > 1) {
+          // table format, in place; rows past a short last part unused
+          for (u32 t = 0; t < h.s.n_pre; t++)
+            IMP_CHECK(to_table(dst + (size_t)16 * t * h.part_len,
+                               dst + (size_t)16 * t * h.part_len, cur, g.cp));
+        }
+        IMP_CHECK(hipEventRecord(pt_ready[k], g.cp));
+        resident[k] = op.part;
+        next_pt = (k + 1) % h.sp;
+      }
       IMP_CHECK(hipStreamWaitEvent(g.st, sc_ready[sk], 0));
       char* d = desc + o * h.op_desc;
-      const std::vector<StageLaunch>& sl = stl[op.part + 1 == h.parts ? 1 : 0][mode[o]];
-      IMP_CHECK(launch_stages((const MsmJob*)d, 1, (const u32*)(d + align256(sizeof(MsmJob))),
-                              sl.data(), (u32)sl.size(), 0, g.st));
+      const u32 sh = op.part + 1 == h.parts ? 1 : 0;
+      const std::vector<StageLaunch>& sl = stl[sh][mode[o]];
+      const u32* pf = (const u32*)(d + align256(sizeof(MsmJob)));
+      // the part's points are in place for every later operation on it (the
+      // compute stream waited for them at the part's first operation)
+      const u32 split = new_part ? pat[sh][mode[o]] : 0;
+      IMP_CHECK(launch_stages((const MsmJob*)d, 1, pf, sl.data(), split, 0, g.st));
+      if (new_part) {
+        IMP_CHECK(hipStreamWaitEvent(g.st, pt_ready[k], 0));
+        prev_part = op.part;
+      }
+      IMP_CHECK(launch_stages((const MsmJob*)d, 1, pf + (size_t)split * 2, sl.data() + split,
+                              (u32)sl.size() - split, 0, g.st));
       IMP_CHECK(hipEventRecord(sc_free[sk], g.st));
       const bool last_of_part = o + 1 == ops.size() || ops[o + 1].part != op.part;
       if (last_of_part) IMP_CHECK(hipEventRecord(pt_free[k], g.st));

@@ -30,7 +30,7 @@ struct Fiber {
   ucontext_t ctx;
   char* stack = nullptr;
   int state = 0;  // 0 runnable, 1 waiting at the barrier, 2 returned, 3 in a wave collective
-  int wkind = 0;  // collective: 1 ballot, 2 shuffle
+  int wkind = 0;  // collective: 1 ballot, 2 shuffle, 3 broadcast inside a group of four lanes
   unsigned long long wval = 0, wres = 0;
   int warg = 0;
 #ifdef EMU_ASAN
@@ -103,9 +103,43 @@ void run_group(unsigned nthreads) {
       __sanitizer_finish_switch_fiber(g_sched_fake, nullptr, nullptr);
 #endif
     }
+    // group-of-four broadcasts: complete once every lane of the group that
+    // has not returned is in one (the other lanes of the wave may be anywhere)
+    bool released = false;
+    for (unsigned q0 = 0; q0 < nthreads; q0 += 4) {
+      unsigned q1 = q0 + 4 < nthreads ? q0 + 4 : nthreads;
+      unsigned in_q = 0, live = 0;
+      for (unsigned t = q0; t < q1; t++) {
+        in_q += g_fibers[t].state == 3 && g_fibers[t].wkind == 3;
+        live += g_fibers[t].state != 2;
+      }
+      if (in_q == 0) continue;
+      if (in_q != live) {
+        fprintf(stderr, "[emu] divergent group-of-four broadcast in block %u, lanes %u..%u:",
+                blockIdx_.x, q0, q1 - 1);
+        for (unsigned t = q0; t < q1; t++)
+          fprintf(stderr, " %u:%d/%d", t, g_fibers[t].state, g_fibers[t].wkind);
+        fprintf(stderr, "\n");
+        abort();
+      }
+      for (unsigned t = q0; t < q1; t++) {
+        Fiber& f = g_fibers[t];
+        if (f.state != 3) continue;
+        const Fiber& src = g_fibers[q0 + ((unsigned)f.warg & 3)];
+        if (src.state != 3 || src.wkind != 3) {
+          fprintf(stderr, "[emu] broadcast from a lane that has returned (block %u, lane %u)\n",
+                  blockIdx_.x, q0 + ((unsigned)f.warg & 3));
+          abort();
+        }
+        f.wres = src.wval;
+      }
+      for (unsigned t = q0; t < q1; t++)
+        if (g_fibers[t].state == 3) g_fibers[t].state = 0;
+      released = true;
+    }
+    if (released) continue;
     // wave collectives: complete once every lane of the wave that has not
     // returned is in one
-    bool released = false;
     for (unsigned w0 = 0; w0 < nthreads; w0 += 64) {
       unsigned w1 = w0 + 64 < nthreads ? w0 + 64 : nthreads;
       unsigned in_op = 0, done = 0, at_bar = 0;
@@ -189,6 +223,7 @@ static unsigned long long wave_op(int kind, unsigned long long v, int arg) {
 
 unsigned long long wave_ballot(int pred) { return wave_op(1, pred ? 1 : 0, 0); }
 unsigned long long wave_shfl(unsigned long long v, int src) { return wave_op(2, v, src); }
+unsigned long long quad_shfl(unsigned long long v, int k) { return wave_op(3, v, k); }
 unsigned lane_id() { return (unsigned)g_cur & 63u; }
 
 void launch_impl(dim3 grid, dim3 block, size_t shmem, std::function<void()> body) {

@@ -64,6 +64,8 @@ void syncthreads();
 // wave-wide collectives: every lane of the wave that has not returned takes part
 unsigned long long wave_ballot(int pred);
 unsigned long long wave_shfl(unsigned long long v, int src);
+// v of lane 4 * (lane / 4) + k, every lane of that group of four taking part
+unsigned long long quad_shfl(unsigned long long v, int k);
 unsigned lane_id();
 void* dyn_lds();
 // runs body for every work-item of the grid; while a stream capture is open
