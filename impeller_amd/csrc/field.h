@@ -376,19 +376,6 @@ __device__ __forceinline__ Fp<P> inv_fermat(const Fp<P>& a) {
   return pow_big<P>(a, e);
 }
 
-// x / 2 mod m for x < m (m odd): x even -> x >> 1, else (x + m) >> 1 (< 2^255)
-template <class P>
-__device__ __forceinline__ void half_mod(u32 x[8]) {
-  const u32 odd = x[0] & 1u;
-  u32 c = 0;
-  u32 s[8];
-#pragma unroll
-  for (int i = 0; i < 8; i++) s[i] = adc(x[i], odd ? P::m[i] : 0u, c, &c);
-#pragma unroll
-  for (int i = 0; i < 7; i++) x[i] = (s[i] >> 1) | (s[i + 1] << 31);
-  x[7] = s[7] >> 1;
-}
-
 // x = x - y mod m for x, y < m
 template <class P>
 __device__ __forceinline__ void sub_mod_limbs(u32 x[8], const u32 y[8]) {
@@ -401,13 +388,45 @@ __device__ __forceinline__ void sub_mod_limbs(u32 x[8], const u32 y[8]) {
   for (int i = 0; i < 8; i++) x[i] = adc(x[i], P::m[i] & mask, c, &c);
 }
 
+// x = x / 2^k mod m for x < m, 1 <= k <= 31: adding c m with
+// c = x (-m^-1) mod 2^k makes the sum divisible by 2^k, and
+// (x + c m) / 2^k < (m + (2^k - 1) m) / 2^k = m.
+template <class P>
+__device__ __forceinline__ void div2k_mod(u32 x[8], u32 k) {
+  const u32 c = (x[0] * P::np) & ((1u << k) - 1u);
+  u32 s[9];
+  u64 carry = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const u64 t = (u64)c * P::m[i] + x[i] + carry;
+    s[i] = (u32)t;
+    carry = t >> 32;
+  }
+  s[8] = (u32)carry;
+#pragma unroll
+  for (int i = 0; i < 8; i++) x[i] = (s[i] >> k) | (s[i + 1] << (32 - k));
+}
+
+// u = u / 2^k (plain integer), 1 <= k <= 31
+__device__ __forceinline__ void shr_limbs(u32 u[8], const u32 d[8], u32 k) {
+#pragma unroll
+  for (int i = 0; i < 7; i++) u[i] = (d[i] >> k) | (d[i + 1] << (32 - k));
+  u[7] = d[7] >> k;
+}
+
+// trailing zero bits of an even value's low word, at most 31
+__device__ __forceinline__ u32 even_shift(u32 w0) { return (u32)__builtin_ctz(w0 | 0x80000000u); }
+
 // Inversion by the binary extended Euclidean algorithm (Hankerson, Menezes
 // and Vanstone, Guide to Elliptic Curve Cryptography, Alg. 2.22, with the
-// two odd/odd cases folded into one step).  Invariants: x1 * A = u and
-// x2 * A = v (mod m) for the input representative A = a R; every step halves
-// u or v, so the loop ends within 2 * 254 steps.  The result A^-1 = a^-1 R^-1
-// is brought to Montgomery form a^-1 R by two products with R^2.  Same value
-// as inv_fermat (the inverse is unique); inv(0) = 0.
+// two odd/odd cases folded into one step, and every even value divided by
+// all the factors of two of its low word at once, its coefficient by the
+// same 2^k through one multiple of m).  Invariants: x1 * A = u and
+// x2 * A = v (mod m) for the input representative A = a R; every step
+// halves u or v at least once, so the loop ends within 2 * 256 steps.  The
+// result A^-1 = a^-1 R^-1 is brought to Montgomery form a^-1 R by two
+// products with R^2.  Same value as inv_fermat (the inverse is unique);
+// inv(0) = 0.
 template <class P>
 __device__ __forceinline__ Fp<P> inv(const Fp<P>& a_in) {
   const Fp<P> a = reduce_once<P>(a_in);  // inputs below 2m
@@ -430,35 +449,31 @@ __device__ __forceinline__ Fp<P> inv(const Fp<P>& a_in) {
     }
     if ((u_hi == 0 && u[0] == 1) || (v_hi == 0 && v[0] == 1)) break;
     if ((u[0] & 1u) == 0) {
-#pragma unroll
-      for (int i = 0; i < 7; i++) u[i] = (u[i] >> 1) | (u[i + 1] << 31);
-      u[7] >>= 1;
-      half_mod<P>(x1);
+      const u32 k = even_shift(u[0]);
+      shr_limbs(u, u, k);
+      div2k_mod<P>(x1, k);
     } else if ((v[0] & 1u) == 0) {
-#pragma unroll
-      for (int i = 0; i < 7; i++) v[i] = (v[i] >> 1) | (v[i + 1] << 31);
-      v[7] >>= 1;
-      half_mod<P>(x2);
+      const u32 k = even_shift(v[0]);
+      shr_limbs(v, v, k);
+      div2k_mod<P>(x2, k);
     } else {
       // both odd: subtract the smaller from the larger (the difference is even)
       u32 d[8], borrow = 0;
 #pragma unroll
       for (int i = 0; i < 8; i++) d[i] = sbb(u[i], v[i], borrow, &borrow);
       if (!borrow) {
-#pragma unroll
-        for (int i = 0; i < 7; i++) u[i] = (d[i] >> 1) | (d[i + 1] << 31);
-        u[7] = d[7] >> 1;
+        const u32 k = even_shift(d[0]);
+        shr_limbs(u, d, k);
         sub_mod_limbs<P>(x1, x2);
-        half_mod<P>(x1);
+        div2k_mod<P>(x1, k);
       } else {
         borrow = 0;
 #pragma unroll
         for (int i = 0; i < 8; i++) d[i] = sbb(v[i], u[i], borrow, &borrow);
-#pragma unroll
-        for (int i = 0; i < 7; i++) v[i] = (d[i] >> 1) | (d[i + 1] << 31);
-        v[7] = d[7] >> 1;
+        const u32 k = even_shift(d[0]);
+        shr_limbs(v, d, k);
         sub_mod_limbs<P>(x2, x1);
-        half_mod<P>(x2);
+        div2k_mod<P>(x2, k);
       }
     }
   }
@@ -612,15 +627,7 @@ __device__ __forceinline__ G1Xyzz xyzz_add(const G1Xyzz& p, const G1Xyzz& q) {
   return r;
 }
 
-// ------------------------------------------- group-of-four (quad) forms
-// For chains of dependent point operations on few lanes (the window sums'
-// doubling chains and Horner), where one wave's latency sets the time: the
-// four lanes of a group hold the same point(s) and split each formula's
-// products into rounds of at most four independent ones, one per lane,
-// every lane then receiving all of a round's results (quad_bcast).  The
-// formulas and their operands are those of xyzz_dbl / xyzz_add, so the
-// coordinates are identical.  q = lane % 4; every lane of the group calls
-// with the same points.
+// ---------------------------------------------- group-of-four broadcast
 // v as held by lane 4 * (lane / 4) + K, for every lane of each group of four
 // consecutive lanes: a DPP quad permutation (one VALU move, no LDS).  Every
 // lane of a group that uses it must be active.
@@ -634,73 +641,6 @@ __device__ __forceinline__ u32 quad_bcast(u32 v) {
 #else
   return v;
 #endif
-}
-
-template <int K>
-__device__ __forceinline__ Fq quad_fe(const Fq& a) {
-  Fq r;
-#pragma unroll
-  for (int i = 0; i < 8; i++) r.v[i] = quad_bcast<K>(a.v[i]);
-  return r;
-}
-// a_q, by masks: a chain of selects on q is turned by the compiler into an
-// index into a private array (scratch memory)
-__device__ __forceinline__ Fq pick4(u32 q, const Fq& a0, const Fq& a1, const Fq& a2, const Fq& a3) {
-  const u32 m0 = 0u - (u32)(q == 0), m1 = 0u - (u32)(q == 1), m2 = 0u - (u32)(q == 2),
-            m3 = 0u - (u32)(q == 3);
-  Fq r;
-#pragma unroll
-  for (int i = 0; i < 8; i++)
-    r.v[i] = (a0.v[i] & m0) | (a1.v[i] & m1) | (a2.v[i] & m2) | (a3.v[i] & m3);
-  return r;
-}
-
-// dbl-2008-s-1 in three rounds: {u^2, x^2}, {u v, x v, v zz, m^2},
-// {m (s - x3), w y, w zzz}
-__device__ __forceinline__ G1Xyzz xyzz_dbl_quad(const G1Xyzz& p, u32 q) {
-  typedef FqParams Q;
-  if (p.is_identity()) return p;
-  const Fq u = dbl<Q>(p.y);
-  const Fq a1 = pick4(q, u, p.x, u, p.x);
-  const Fq r1 = mul<Q>(a1, a1);
-  const Fq v = quad_fe<0>(r1), x2 = quad_fe<1>(r1);
-  const Fq m = add<Q>(dbl<Q>(x2), x2);
-  const Fq r2 = mul<Q>(pick4(q, u, p.x, p.zz, m), pick4(q, v, v, v, m));
-  const Fq w = quad_fe<0>(r2), s = quad_fe<1>(r2), zz3 = quad_fe<2>(r2), m2 = quad_fe<3>(r2);
-  const Fq x3 = sub<Q>(sub<Q>(m2, s), s);
-  const Fq r3 = mul<Q>(pick4(q, m, w, w, w), pick4(q, sub<Q>(s, x3), p.y, p.zzz, p.zzz));
-  G1Xyzz r;
-  r.x = x3;
-  r.y = sub<Q>(quad_fe<0>(r3), quad_fe<1>(r3));
-  r.zz = zz3;
-  r.zzz = quad_fe<2>(r3);
-  return r;
-}
-
-// add-2008-s in four rounds: {u1, u2, s1, s2}, {P^2, R^2, zz1 zz2,
-// zzz1 zzz2}, {P pp, u1 pp, zz12 pp}, {R (qq - x3), s1 ppp, zzz12 ppp}
-__device__ __forceinline__ G1Xyzz xyzz_add_quad(const G1Xyzz& p, const G1Xyzz& o, u32 q) {
-  typedef FqParams Q;
-  if (p.is_identity()) return o;
-  if (o.is_identity()) return p;
-  const Fq r1 = mul<Q>(pick4(q, p.x, o.x, p.y, o.y), pick4(q, o.zz, p.zz, o.zzz, p.zzz));
-  const Fq u1 = quad_fe<0>(r1), u2 = quad_fe<1>(r1), s1 = quad_fe<2>(r1), s2 = quad_fe<3>(r1);
-  const Fq P_ = sub<Q>(u2, u1);
-  const Fq R_ = sub<Q>(s2, s1);
-  if (P_.is_zero() && R_.is_zero()) return xyzz_dbl_quad(p, q);
-  const Fq r2 = mul<Q>(pick4(q, P_, R_, p.zz, p.zzz), pick4(q, P_, R_, o.zz, o.zzz));
-  const Fq pp = quad_fe<0>(r2), rr = quad_fe<1>(r2), zz12 = quad_fe<2>(r2),
-           zzz12 = quad_fe<3>(r2);
-  const Fq r3 = mul<Q>(pick4(q, P_, u1, zz12, zz12), pp);
-  const Fq ppp = quad_fe<0>(r3), qq = quad_fe<1>(r3), zz3 = quad_fe<2>(r3);
-  const Fq x3 = sub<Q>(sub<Q>(sub<Q>(rr, ppp), qq), qq);
-  const Fq r4 = mul<Q>(pick4(q, R_, s1, zzz12, zzz12), pick4(q, sub<Q>(qq, x3), ppp, ppp, ppp));
-  G1Xyzz r;
-  r.x = x3;
-  r.y = sub<Q>(quad_fe<0>(r4), quad_fe<1>(r4));
-  r.zz = zz3;
-  r.zzz = quad_fe<2>(r4);
-  return r;
 }
 
 __device__ __forceinline__ G1Aff aff_neg(const G1Aff& a) {

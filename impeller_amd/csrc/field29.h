@@ -445,6 +445,92 @@ __device__ __forceinline__ G1Xyzz29 xyzz29_add(const G1Xyzz29& p, const G1Xyzz29
   return r;
 }
 
+// ------------------------------------------- group-of-four (quad) forms
+// xyzz29_dbl / xyzz29_add with their products split into rounds of at most
+// four independent ones, one per lane of a group of four lanes that all hold
+// the same points (q = lane % 4), the round's results broadcast to the whole
+// group (quad_bcast, field.h).  For chains of dependent operations on few
+// lanes, where one wave's latency sets the time.  Every product has the
+// operands of the corresponding product in xyzz29_dbl / xyzz29_add (a
+// square is taken as the product of equal operands), so the bounds noted
+// there hold; the values are the same field elements.
+template <int K>
+__device__ __forceinline__ Fq29 quad_fe29(const Fq29& a) {
+  Fq29 r;
+#pragma unroll
+  for (int i = 0; i < 9; i++) r.v[i] = quad_bcast<K>(a.v[i]);
+  return r;
+}
+// a_q for q = lane % 4, as two levels of selects on the bits of q (one or
+// two v_cndmask per limb); the form sel(q, a0, a1, a2, a3) written as a
+// chain of comparisons with q is turned by the compiler into an index into
+// a private array (scratch memory)
+__device__ __forceinline__ Fq29 pick4_29(u32 q, const Fq29& a0, const Fq29& a1, const Fq29& a2,
+                                         const Fq29& a3) {
+  const bool b0 = (q & 1) != 0, b1 = (q & 2) != 0;
+  Fq29 r;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const u32 lo = b0 ? a1.v[i] : a0.v[i], hi = b0 ? a3.v[i] : a2.v[i];
+    r.v[i] = b1 ? hi : lo;
+  }
+  return r;
+}
+
+// dbl-2008-s-1: {u u, x x}, {u v, x v, v zz, m3 m3}, {m3 t, w y, w zzz}
+__device__ __forceinline__ G1Xyzz29 xyzz29_dbl_quad(const G1Xyzz29& p, u32 q) {
+  if (p.is_identity()) return p;
+  const Fq29 zero = Fq29::zero();
+  const Fq29 u = add29<Q29>(p.y, p.y);
+  const Fq29 a1 = pick4_29(q, u, p.x, u, p.x);
+  const Fq29 r1 = mul29<Q29>(a1, a1);
+  const Fq29 v = quad_fe29<0>(r1), x2 = quad_fe29<1>(r1);
+  Fq29 m3 = add29<Q29>(add29<Q29>(x2, x2), x2);
+  norm29<Q29>(m3);
+  const Fq29 r2 = mul29<Q29>(pick4_29(q, u, p.x, v, m3), pick4_29(q, v, v, p.zz, m3));
+  const Fq29 w = quad_fe29<0>(r2), s = quad_fe29<1>(r2), zz3 = quad_fe29<2>(r2),
+             msq = quad_fe29<3>(r2);
+  const Fq29 x3 = sub3_29<Q29, 4>(msq, s, s, zero);
+  const Fq29 t = sub29<Q29, 8>(s, x3);
+  const Fq29 r3 = mul29<Q29>(pick4_29(q, m3, w, w, w), pick4_29(q, t, p.y, p.zzz, p.zzz));
+  G1Xyzz29 r;
+  r.x = x3;
+  r.y = sub29<Q29, 2>(quad_fe29<0>(r3), quad_fe29<1>(r3));
+  r.zz = zz3;
+  r.zzz = quad_fe29<2>(r3);
+  return r;
+}
+
+// add-2008-s: {u1, u2, s1, s2}, {P P, R R, zz1 zz2, zzz1 zzz2},
+// {P pp, u1 pp, zz12 pp}, {R t, s1 ppp, zzz12 ppp}
+__device__ __forceinline__ G1Xyzz29 xyzz29_add_quad(const G1Xyzz29& p, const G1Xyzz29& o, u32 q) {
+  if (p.is_identity()) return o;
+  if (o.is_identity()) return p;
+  const Fq29 r1 = mul29<Q29>(pick4_29(q, p.x, o.x, p.y, o.y), pick4_29(q, o.zz, p.zz, o.zzz, p.zzz));
+  const Fq29 u1 = quad_fe29<0>(r1), u2 = quad_fe29<1>(r1), s1 = quad_fe29<2>(r1),
+             s2 = quad_fe29<3>(r1);
+  const Fq29 P_ = sub29<Q29, 2>(u2, u1);
+  const Fq29 R_ = sub29<Q29, 2>(s2, s1);
+  const Fq29 r2 = mul29<Q29>(pick4_29(q, P_, R_, p.zz, p.zzz), pick4_29(q, P_, R_, o.zz, o.zzz));
+  const Fq29 pp = quad_fe29<0>(r2), rr = quad_fe29<1>(r2), zz12 = quad_fe29<2>(r2),
+             zzz12 = quad_fe29<3>(r2);
+  if (is_zero29<Q29>(pp)) {
+    if (is_zero29<Q29>(rr)) return xyzz29_dbl_quad(p, q);
+    return G1Xyzz29::identity();
+  }
+  const Fq29 r3 = mul29<Q29>(pick4_29(q, P_, u1, zz12, zz12), pp);
+  const Fq29 ppp = quad_fe29<0>(r3), qq = quad_fe29<1>(r3), zz3 = quad_fe29<2>(r3);
+  const Fq29 x3 = sub3_29<Q29, 6>(rr, ppp, qq, qq);
+  const Fq29 t = sub29<Q29, 8>(qq, x3);
+  const Fq29 r4 = mul29<Q29>(pick4_29(q, R_, s1, zzz12, zzz12), pick4_29(q, t, ppp, ppp, ppp));
+  G1Xyzz29 r;
+  r.x = x3;
+  r.y = sub29<Q29, 2>(quad_fe29<0>(r4), quad_fe29<1>(r4));
+  r.zz = zz3;
+  r.zzz = quad_fe29<2>(r4);
+  return r;
+}
+
 // ---------------------------------------------------- conversions / memory
 // XYZZ29 in memory: 4 x 9 u32 (144 bytes), limbs as held
 __device__ __forceinline__ void store_xyzz29(u32* p, const G1Xyzz29& a) {

@@ -124,7 +124,19 @@ __global__ void k_field_op(int op, u32* r, const u32* a, const u32* b, u64 n) {
 }
 
 __global__ void k_g1_op(int op, u32* r, const u32* a, const u32* b, u64 n) {
-  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  const u64 g = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (op >= 9) {
+    // the group-of-four forms: lanes 4i .. 4i + 3 work on item i together
+    const u64 i = g >> 2;
+    const u32 q = (u32)(g & 3);
+    if (i >= n) return;
+    const G1Xyzz29 x = xyzz_to_29(load_xyzz(a + 32 * i));
+    const G1Xyzz29 res = op == 9 ? xyzz29_add_quad(x, xyzz_to_29(load_xyzz(b + 32 * i)), q)
+                                 : xyzz29_dbl_quad(x, q);
+    if (q == 0) store_xyzz(r + 32 * i, xyzz_from_29(res));
+    return;
+  }
+  const u64 i = g;
   if (i >= n) return;
   switch (op) {
     case 0: store_xyzz(r + 32 * i, xyzz_madd(load_xyzz(a + 32 * i), load_aff(b + 16 * i))); break;
@@ -282,10 +294,12 @@ extern "C" hipError_t impeller_g1_op(int op, uint32_t* r_d, const uint32_t* a_d,
                                      const uint32_t* b_d, unsigned long long n,
                                      hipStream_t stream) {
   IMP_REQUIRE(r_d && a_d, "null pointer");
-  IMP_REQUIRE(op >= 0 && op <= 8, "op");
-  IMP_REQUIRE(b_d || op == 2 || op == 3 || op == 7 || op == 8, "b operand");
+  IMP_REQUIRE(op >= 0 && op <= 10, "op");
+  IMP_REQUIRE(b_d || op == 2 || op == 3 || op == 7 || op == 8 || op == 10, "b operand");
+  IMP_REQUIRE(n < (1ull << 36), "n");
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_g1_op, dim3((u32)((n + 63) / 64)), dim3(64), 0, stream, op, r_d, a_d,
+  const u64 lanes = op >= 9 ? 4 * n : n;
+  hipLaunchKernelGGL(k_g1_op, dim3((u32)((lanes + 63) / 64)), dim3(64), 0, stream, op, r_d, a_d,
                      b_d, (u64)n);
   return hipGetLastError();
 }

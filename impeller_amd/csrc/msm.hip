@@ -61,7 +61,7 @@ struct MsmShape {
   u32 nseg;       // reduce segments per physical window (nb / seg)
   u32 lbits;      // log2(nseg)
   u32 sbits;      // log2(seg)
-  u32 g0;         // level-0 work-groups per window sum (power of two, 512 inputs each)
+  u32 g0;         // level-0 work-groups of the sum T_s (power of two, 1024 inputs each)
   u32 sum_levels; // 1: the final reduces the g0 partials; 2: a second level does
   u32 fin_g;      // partials per window sum that the final reads (g0 or 1)
 };
@@ -97,12 +97,13 @@ static MsmShape msm_shape(u64 N, u32 c, u32 target_windows, u32 task = 64) {
   s.seg = 1u << s.sbits;
   s.nseg = s.nb / s.seg;  // nb >= 8 for c >= 4
   s.lbits = c - 1 - s.sbits;
-  // window sums: lbits + 1 of them (the T_s and one per segment-index bit);
-  // level 0 gives each work-group 512 inputs (two per thread, then a tree),
-  // and when the final block cannot hold all g0 partials of every sum a
-  // second level reduces them to one
+  // window sums: lbits + 1 of them (the T_s over all nseg segments, and one
+  // per segment-index bit over the nseg / 2 segments with that bit set);
+  // level 0 gives each work-group 1024 inputs (four per thread, then a
+  // tree), and when the final block cannot hold all g0 partial slots of
+  // every sum a second level reduces them to one
   const u32 nv = s.lbits + s.sbits;  // = c - 1
-  s.g0 = s.nseg > 512 ? s.nseg / 512 : 1;
+  s.g0 = s.nseg > 1024 ? s.nseg / 1024 : 1;
   s.sum_levels = (u64)s.n_windows * nv * s.g0 <= 256 ? 1 : 2;
   s.fin_g = s.sum_levels == 1 ? s.g0 : 1;
   return s;
@@ -827,54 +828,77 @@ __device__ __forceinline__ void reduce_body(const MsmJob& J, u64 t) {
   store_xyzz29_g(J.rparts + kPt * ((u64)(2 * pw + 1) * s.nseg + sg), run);
 }
 
-// pairwise tree over the 256 XYZZ values red[0..255] with the given stride
-// pattern: after it red[0] is their sum (blockDim.x == 256)
+// pairwise tree over the 256 XYZZ values red[0..255]: after it red[0] is
+// their sum (blockDim.x == 256).  The first level adds on one lane per pair;
+// from 64 pairs down each addition runs on a group of four lanes
+// (xyzz29_add_quad), whose lane 0 writes the slot after a barrier.
 __device__ __forceinline__ void block_tree_sum(u32* red) {
   const u32 t = threadIdx.x;
-  for (u32 d = 128; d > 0; d >>= 1) {
-    if (t < d) {
-      G1Xyzz29 a = load_xyzz29(red + kPt * t), b = load_xyzz29(red + kPt * (t + d));
-      store_xyzz29(red + kPt * t, xyzz29_add(a, b));
-    }
+  if (t < 128)
+    store_xyzz29(red + kPt * t,
+                 xyzz29_add(load_xyzz29(red + kPt * t), load_xyzz29(red + kPt * (t + 128))));
+  __syncthreads();
+  const u32 q = t & 3, qt = t >> 2;
+  for (u32 d = 64; d > 0; d >>= 1) {
+    const bool live = qt < d;
+    G1Xyzz29 r;
+    if (live) r = xyzz29_add_quad(load_xyzz29(red + kPt * qt), load_xyzz29(red + kPt * (qt + d)), q);
+    __syncthreads();
+    if (live && q == 0) store_xyzz29(red + kPt * qt, r);
     __syncthreads();
   }
 }
 
+// Level-0 work-groups (and live partial slots) of window sum q: g0 for the
+// T_s (q = 0), half as many for a V_l, which has half the inputs.
+__host__ __device__ __forceinline__ u32 sum_groups(const MsmShape& s, u32 q) {
+  return q == 0 || s.g0 == 1 ? s.g0 : s.g0 / 2;
+}
+
 // Level 0, work-group lb: window pw, sum q (0: the T_s; q = l + 1: V_l),
-// inputs [512 g, 512 g + 512) of that sum, two per thread, then a tree.
-// Level 1, work-group lb: window pw, sum q, its g0 level-0 partials.
+// inputs [1024 g, 1024 g + 1024) of that sum, four per thread (two pairs,
+// then their sum), then a tree; the work-groups of a window are the g0 of
+// the T_s, then sum_groups(s, 1) for each V_l.  Level 1, work-group lb:
+// window pw, sum q, its level-0 partials.
 __device__ __forceinline__ void sum_body(const MsmJob& J, u32 level, u64 lb, u32* red) {
   const MsmShape& s = J.s;
   const u32 t = threadIdx.x, nq = s.lbits + 1;
   G1Xyzz29 acc = G1Xyzz29::identity();
   __syncthreads();
   if (level == 0) {
-    const u32 g = (u32)(lb % s.g0);
-    const u32 q = (u32)((lb / s.g0) % nq);
-    const u32 pw = (u32)(lb / ((u64)s.g0 * nq));
+    const u32 gh = sum_groups(s, 1), per_win = s.g0 + s.lbits * gh;
+    const u32 pw = (u32)(lb / per_win), r = (u32)(lb % per_win);
+    const u32 q = r < s.g0 ? 0 : 1 + (r - s.g0) / gh;
+    const u32 g = r < s.g0 ? r : (r - s.g0) % gh;
     const u32 count = q == 0 ? s.nseg : s.nseg / 2;
     const u32* src = J.rparts + kPt * ((u64)(2 * pw + (q ? 1 : 0)) * s.nseg);
     const u32 l = q ? q - 1 : 0;
-    // both inputs loaded before the addition
-    G1Xyzz29 in[2];
+    // both inputs of a pair loaded before its addition
+    G1Xyzz29 pair[2];
 #pragma unroll
-    for (u32 h = 0; h < 2; h++) {
-      const u32 i = g * 512 + h * 256 + t;
-      in[h] = G1Xyzz29::identity();
-      if (i < count) {
-        // q > 0: the i-th segment index with bit l set
-        const u32 sg = q == 0 ? i : (((i >> l) << (l + 1)) | (1u << l) | (i & ((1u << l) - 1)));
-        in[h] = load_xyzz29_g(src + kPt * (u64)sg);
+    for (u32 hp = 0; hp < 2; hp++) {
+      G1Xyzz29 in[2];
+#pragma unroll
+      for (u32 h = 0; h < 2; h++) {
+        const u32 i = g * 1024 + (2 * hp + h) * 256 + t;
+        in[h] = G1Xyzz29::identity();
+        if (i < count) {
+          // q > 0: the i-th segment index with bit l set
+          const u32 sg = q == 0 ? i : (((i >> l) << (l + 1)) | (1u << l) | (i & ((1u << l) - 1)));
+          in[h] = load_xyzz29_g(src + kPt * (u64)sg);
+        }
       }
+      pair[hp] = xyzz29_add(in[0], in[1]);
     }
-    acc = xyzz29_add(in[0], in[1]);
+    acc = xyzz29_add(pair[0], pair[1]);
     store_xyzz29(red + kPt * t, acc);
     __syncthreads();
     block_tree_sum(red);
     if (t < kPt) J.sparts[kPt * (((u64)pw * nq + q) * s.g0 + g) + t] = red[t];
   } else {
     const u64 v = lb;  // pw * nq + q
-    for (u32 i = t; i < s.g0; i += 256)
+    const u32 ng = sum_groups(s, (u32)(v % nq));
+    for (u32 i = t; i < ng; i += 256)
       acc = xyzz29_add(acc, load_xyzz29_g(J.sparts + kPt * (v * s.g0 + i)));
     store_xyzz29(red + kPt * t, acc);
     __syncthreads();
@@ -886,16 +910,15 @@ __device__ __forceinline__ void sum_body(const MsmJob& J, u32 level, u64 lb, u32
 // one work-group per job (blockDim.x == 256); the shape guarantees
 // n_windows * (lbits + sbits) * fin_g <= 256.  Everything here is a chain of
 // dependent EC operations on a few lanes, where one wave's latency counts,
-// not the chip's product rate: the values are converted to the 8 x 32-bit
-// form as they are loaded, and each value is worked on by a group of four
-// lanes (xyzz_dbl_quad / xyzz_add_quad: three or four rounds of products
-// instead of nine or fourteen in a row) wherever the work-group has four
-// lanes per value; otherwise by one lane.  Horner over the physical windows
-// always runs on lanes 0..3.  An 8 x 32-bit XYZZ takes 32 of a red slot's
-// kPt words; the group's lane 0 writes the slot, after a barrier (every lane
-// has read its operands by then).
-__device__ __forceinline__ G1Xyzz final_add(const G1Xyzz& a, const G1Xyzz& b, bool quad, u32 q) {
-  return quad ? xyzz_add_quad(a, b, q) : xyzz_add(a, b);
+// not the chip's product rate: each value is worked on by a group of four
+// lanes (xyzz29_dbl_quad / xyzz29_add_quad: three or four rounds of
+// products instead of nine or fourteen in a row) wherever the work-group
+// has four lanes per value, otherwise by one lane; Horner over the physical
+// windows always runs on lanes 0..3.  The group's lane 0 writes a value's
+// slot, after a barrier (every lane has read its operands by then).
+__device__ __forceinline__ G1Xyzz29 final_add(const G1Xyzz29& a, const G1Xyzz29& b, bool quad,
+                                              u32 q) {
+  return quad ? xyzz29_add_quad(a, b, q) : xyzz29_add(a, b);
 }
 __device__ __forceinline__ void final_body(const MsmJob& J, u32* red) {
   const MsmShape& s = J.s;
@@ -913,20 +936,21 @@ __device__ __forceinline__ void final_body(const MsmJob& J, u32* red) {
     const u32 q = quad ? t & 3 : 0, vt = quad ? t >> 2 : t;
     if (vt < nval * G && q == 0) {
       const u32 gi = vt % G, i = (vt / G) % nv, pw = vt / (G * nv);
-      G1Xyzz v = G1Xyzz::identity();
-      if (i == 0 || i >= s.sbits) {
-        const u32 qs = i == 0 ? 0 : i - s.sbits + 1;
-        v = xyzz_from_29(load_xyzz29_g(parts + kPt * (((u64)pw * (s.lbits + 1) + qs) * G + gi)));
-      }
-      store_xyzz(red + kPt * vt, v);
+      G1Xyzz29 v = G1Xyzz29::identity();
+      const u32 qs = i == 0 ? 0 : i - s.sbits + 1;
+      // with one sum level, slots past the sum's live partials are unused
+      if ((i == 0 || i >= s.sbits) && (G == 1 || gi < sum_groups(s, qs)))
+        v = load_xyzz29_g(parts + kPt * (((u64)pw * (s.lbits + 1) + qs) * G + gi));
+      store_xyzz29(red + kPt * vt, v);
     }
     __syncthreads();
     for (u32 d = 1; d < G; d <<= 1) {
       const bool live = vt < nval * G && (vt % (2 * d)) == 0;
-      G1Xyzz r;
-      if (live) r = final_add(load_xyzz(red + kPt * vt), load_xyzz(red + kPt * (vt + d)), quad, q);
-      __syncthreads();  // every lane of a group has read its slots
-      if (live && q == 0) store_xyzz(red + kPt * vt, r);
+      G1Xyzz29 r;
+      if (live)
+        r = final_add(load_xyzz29(red + kPt * vt), load_xyzz29(red + kPt * (vt + d)), quad, q);
+      __syncthreads();
+      if (live && q == 0) store_xyzz29(red + kPt * vt, r);
       __syncthreads();
     }
   }
@@ -938,34 +962,34 @@ __device__ __forceinline__ void final_body(const MsmJob& J, u32* red) {
     const u32 q = quad ? t & 3 : 0, vt = quad ? t >> 2 : t;
     // a_1 .. a_(sbits-1) are the identity
     const bool dlive = vt < nval && vt % nv >= s.sbits;
-    G1Xyzz b;
+    G1Xyzz29 b;
     if (dlive) {
-      b = load_xyzz(red + kPt * (vt * G));
-      for (u32 k = 0, i = vt % nv; k < i; k++) b = quad ? xyzz_dbl_quad(b, q) : xyzz_dbl(b);
+      b = load_xyzz29(red + kPt * (vt * G));
+      for (u32 k = 0, i = vt % nv; k < i; k++) b = quad ? xyzz29_dbl_quad(b, q) : xyzz29_dbl(b);
     }
     __syncthreads();
-    if (dlive && q == 0) store_xyzz(red + kPt * (vt * G), b);
+    if (dlive && q == 0) store_xyzz29(red + kPt * (vt * G), b);
     __syncthreads();
     for (u32 h = 1; h < nv; h <<= 1) {
       const bool live = vt < nval && ((vt % nv) % (2 * h)) == 0 && vt % nv + h < nv;
-      G1Xyzz r;
+      G1Xyzz29 r;
       if (live)
-        r = final_add(load_xyzz(red + kPt * (vt * G)), load_xyzz(red + kPt * ((vt + h) * G)),
+        r = final_add(load_xyzz29(red + kPt * (vt * G)), load_xyzz29(red + kPt * ((vt + h) * G)),
                       quad, q);
       __syncthreads();
-      if (live && q == 0) store_xyzz(red + kPt * (vt * G), r);
+      if (live && q == 0) store_xyzz29(red + kPt * (vt * G), r);
       __syncthreads();
     }
   }
   if (t >= 4) return;
   // Horner over the physical windows on lanes 0..3: (n_windows - 1) c
   // dependent doublings
-  G1Xyzz acc = load_xyzz(red + kPt * ((s.n_windows - 1) * nv * G));
+  G1Xyzz29 acc = load_xyzz29(red + kPt * ((s.n_windows - 1) * nv * G));
   for (int pw = (int)s.n_windows - 2; pw >= 0; pw--) {
-    for (u32 k = 0; k < s.c; k++) acc = xyzz_dbl_quad(acc, t);
-    acc = xyzz_add_quad(acc, load_xyzz(red + kPt * ((u32)pw * nv * G)), t);
+    for (u32 k = 0; k < s.c; k++) acc = xyzz29_dbl_quad(acc, t);
+    acc = xyzz29_add_quad(acc, load_xyzz29(red + kPt * ((u32)pw * nv * G)), t);
   }
-  if (t == 0) store_aff(J.result, xyzz_to_aff(acc));
+  if (t == 0) store_aff(J.result, xyzz_to_aff(xyzz_from_29(acc)));
 }
 
 // ------------------------------------------------------- batched kernels
@@ -1138,7 +1162,7 @@ static u32 stage_blocks(const MsmShape& s, u32 kind, u32 arg) {
     case MS_REDUCE: return (u32)cdiv((u64)s.n_windows * s.nseg, 256);
     case MS_ACCUM: return (u32)cdiv((u64)s.n_windows * s.nb, 256);
     case MS_SUM:
-      if (arg == 0) return s.n_windows * (s.lbits + 1) * s.g0;
+      if (arg == 0) return s.n_windows * (s.g0 + s.lbits * sum_groups(s, 1));
       return s.sum_levels == 2 ? s.n_windows * (s.lbits + 1) : 0u;
     default: return 1;
   }

@@ -399,9 +399,9 @@ hipError_t impeller_gen_bases(uint32_t *out_d, unsigned long long n, unsigned lo
 hipError_t impeller_field_op(int field, int op, uint32_t *r_d, const uint32_t *a_d,
                              const uint32_t *b_d, unsigned long long n, hipStream_t stream);
 /* G1 helpers: op 0 madd(xyzz, aff), 1 add(xyzz, xyzz), 2 dbl(xyzz),
- * 3 to_affine(xyzz) -> aff, 4 scalar_mul(aff, integer) -> aff, and the
- * 29-bit forms 5 madd, 6 add, 7 dbl, 8 -aff through the table format;
- * n items. */
+ * 3 to_affine(xyzz) -> aff, 4 scalar_mul(aff, integer) -> aff, the
+ * 29-bit forms 5 madd, 6 add, 7 dbl, 8 -aff through the table format, and
+ * the 29-bit group-of-four-lanes forms 9 add, 10 dbl; n items. */
 hipError_t impeller_g1_op(int op, uint32_t *r_d, const uint32_t *a_d, const uint32_t *b_d,
                           unsigned long long n, hipStream_t stream);
 /* Montgomery-multiply throughput probe (no reference analogue): blocks x 256
@@ -442,7 +442,8 @@ int impeller_timing_read(int kernel_id, double *total_ms, unsigned long long *co
 
 /* ---------------------------------------------------------------------
  * Device-runtime layer (csrc/runtime_api.hip): what the reference's Rust
- * runtime takes from its zkpoly_cuda_api crate (cuda_api/src/*.rs), over the
+ * runtime takes from its zkpoly_cuda_api crate (the .rs files of
+ * cuda_api/src), over the
  * HIP runtime.  A hip_api crate binds these in place of generated CUDA
  * bindings.  Errors are returned (hipErrorInvalidValue for a broken
  * contract the reference panics on), never raised.

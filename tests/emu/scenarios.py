@@ -450,6 +450,9 @@ def arith29(E):
     y[21][8:16] = O.neg(x[21][8:16], O.FQ)                        # P + (-P)
     _same_xyzz(_g1(E, 6, x, y, 32, n), _g1(E, 1, x, y, 32, n))
     _same_xyzz(_g1(E, 7, x, None, 32, n), _g1(E, 2, x, None, 32, n))
+    # the group-of-four forms: the same coordinates as the one-lane forms
+    _same_xyzz(_g1(E, 9, x, y, 32, n), _g1(E, 1, x, y, 32, n))
+    _same_xyzz(_g1(E, 10, x, None, 32, n), _g1(E, 2, x, None, 32, n))
     neg = bases[:n].copy()
     neg[:, 8:] = np.stack([O.neg(bases[i][8:], O.FQ) for i in range(n)])
     neg[0] = 0

@@ -100,6 +100,36 @@ def test_curve_ops_vs_oracle(stream):
     assert (got == np.stack([O.g1_mul(bases[i], ks[i]) for i in range(8)])).all()
 
 
+def test_quad_point_ops_vs_oracle(stream):
+    """The group-of-four-lanes 29-bit addition and doubling (the MSM's final
+    window sums run on them) against the oracle's XYZZ formulas, including
+    the identity, doubling and P + (-P) branches of the addition."""
+    n = 96
+    bases = O.gen_bases(n + 1, 91)
+    x = np.stack([O.xyzz_op("orc_affine_to_xyzz", bases[i]) for i in range(n)])
+    x = np.stack([O.xyzz_op("orc_xyzz_dbl", x[i]) for i in range(n)])
+    x[5] = 0
+    x[5][24:] = O.one(O.FQ)                       # identity
+    y = np.roll(x, 1, axis=0).copy()
+    y[20] = x[20]                                 # doubling branch
+    y[21] = x[21]
+    y[21][8:16] = O.neg(x[21][8:16], O.FQ)        # P + (-P)
+    got = g1_op(stream, 9, x, y, 32, n)
+    exp = np.stack([O.xyzz_op("orc_xyzz_add", x[i], y[i]) for i in range(n)])
+    for i in range(n):
+        if not exp[i][16:24].any():               # identity: only zz == 0 carries meaning
+            assert not got[i][16:24].any(), i
+        else:
+            assert (got[i] == exp[i]).all(), i
+    got = g1_op(stream, 10, x, None, 32, n)
+    exp = np.stack([O.xyzz_op("orc_xyzz_dbl", x[i]) for i in range(n)])
+    for i in range(n):
+        if not exp[i][16:24].any():
+            assert not got[i][16:24].any(), i
+        else:
+            assert (got[i] == exp[i]).all(), i
+
+
 def test_generators_match_oracle(stream):
     n = 4096
     for kind in (0, 1, 2):
