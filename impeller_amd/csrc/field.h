@@ -376,114 +376,216 @@ __device__ __forceinline__ Fp<P> inv_fermat(const Fp<P>& a) {
   return pow_big<P>(a, e);
 }
 
-// x = x - y mod m for x, y < m
-template <class P>
-__device__ __forceinline__ void sub_mod_limbs(u32 x[8], const u32 y[8]) {
-  u32 borrow = 0;
-#pragma unroll
-  for (int i = 0; i < 8; i++) x[i] = sbb(x[i], y[i], borrow, &borrow);
-  const u32 mask = 0u - borrow;
-  u32 c = 0;
-#pragma unroll
-  for (int i = 0; i < 8; i++) x[i] = adc(x[i], P::m[i] & mask, c, &c);
-}
+// ---------------------------------------------------------------- inversion
+// Optimized binary GCD (T. Pornin, "Optimized Binary GCD for Modular
+// Inversion", IACR ePrint 2020/972, Algorithm 2, k = 32).  The classic
+// binary GCD (a <- y, b <- m, u <- 1, v <- 0; halve a when even, else
+// order a >= b and set a <- (a - b) / 2, u and v following so that
+// a = u y and b = v y mod m) is run 31 steps at a time on 64-bit
+// approximations of a and b (their low 31 bits and their top 33 bits at
+// the length of the larger one); the steps are recorded as a matrix
+// (f0 g0; f1 g1) with entries in [-2^31, 2^31], then applied to the full
+// values, a <- (a f0 + b g0) / 2^31 and b <- (a f1 + b g1) / 2^31 (exact;
+// a negative result is negated with its row), and to u, v with the
+// division by 2^31 done modulo m.  ceil((2 * 254 - 1) / 31) = 17 rounds
+// bring a to 0 and b to gcd(y, m) = 1, and then v = 1 / y mod m.
+namespace inv_detail {
+// m < 2^254 for both BN254 fields
+constexpr u32 kRounds = (2 * 254 - 1 + 30) / 31;
 
-// x = x / 2^k mod m for x < m, 1 <= k <= 31: adding c m with
-// c = x (-m^-1) mod 2^k makes the sum divisible by 2^k, and
-// (x + c m) / 2^k < (m + (2^k - 1) m) / 2^k = m.
-template <class P>
-__device__ __forceinline__ void div2k_mod(u32 x[8], u32 k) {
-  const u32 c = (x[0] * P::np) & ((1u << k) - 1u);
-  u32 s[9];
-  u64 carry = 0;
+// x f + y g for x, y < 2^254 and |f|, |g| <= 2^31, as a 9-limb two's
+// complement integer (|result| < 2^286)
+__device__ __forceinline__ void lin_comb(const u32 x[8], const u32 y[8], long long f, long long g,
+                                         u32 r[9]) {
+  const bool nf = f < 0, ng = g < 0;
+  const u64 mf = (u64)(nf ? -f : f), mg = (u64)(ng ? -g : g);
+  u32 p[9], q[9];
+  u64 cp = 0, cq = 0;
 #pragma unroll
   for (int i = 0; i < 8; i++) {
-    const u64 t = (u64)c * P::m[i] + x[i] + carry;
+    const u64 tp = (u64)x[i] * mf + cp;
+    const u64 tq = (u64)y[i] * mg + cq;
+    p[i] = (u32)tp;
+    q[i] = (u32)tq;
+    cp = tp >> 32;
+    cq = tq >> 32;
+  }
+  p[8] = (u32)cp;
+  q[8] = (u32)cq;
+  // negate where the factor is negative (two's complement over 9 limbs)
+  const u32 sp = 0u - (u32)nf, sq = 0u - (u32)ng;
+  u32 bp = (u32)nf, bq = (u32)ng;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const u64 tp = (u64)(p[i] ^ sp) + bp;
+    const u64 tq = (u64)(q[i] ^ sq) + bq;
+    p[i] = (u32)tp;
+    q[i] = (u32)tq;
+    bp = (u32)(tp >> 32);
+    bq = (u32)(tq >> 32);
+  }
+  u32 c = 0;
+#pragma unroll
+  for (int i = 0; i < 9; i++) r[i] = adc(p[i], q[i], c, &c);
+}
+
+// x = |r| >> 31 for a 9-limb two's complement r divisible by 2^31; returns
+// whether r was negative
+__device__ __forceinline__ bool abs_shr31(const u32 r_in[9], u32 x[8]) {
+  const bool neg = (r_in[8] >> 31) != 0;
+  const u32 s = 0u - (u32)neg;
+  u32 r[9], b = (u32)neg;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const u64 t = (u64)(r_in[i] ^ s) + b;
+    r[i] = (u32)t;
+    b = (u32)(t >> 32);
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) x[i] = (r[i] >> 31) | (r[i + 1] << 1);
+  return neg;
+}
+
+// bits [p, p + 64) of x (p <= 223), the limbs picked by masks (an index
+// into x would put it in scratch memory)
+__device__ __forceinline__ u64 window64(const u32 x[8], u32 p) {
+  const u32 w = p >> 5, sh = p & 31;
+  u32 x0 = 0, x1 = 0, x2 = 0;
+#pragma unroll
+  for (u32 i = 0; i < 8; i++) {
+    x0 |= x[i] & (0u - (u32)(i == w));
+    x1 |= x[i] & (0u - (u32)(i == w + 1));
+    x2 |= x[i] & (0u - (u32)(i == w + 2));
+  }
+  const u64 lo = ((u64)x1 << 32) | x0;
+  return sh ? (lo >> sh) | ((u64)x2 << (64 - sh)) : lo;
+}
+
+// bit length of the larger of x and y
+__device__ __forceinline__ u32 bit_len2(const u32 x[8], const u32 y[8]) {
+  u32 len = 0;
+#pragma unroll
+  for (u32 i = 0; i < 8; i++) {
+    const u32 o = x[i] | y[i];
+    if (o) len = 32 * i + 32 - (u32)__builtin_clz(o);
+  }
+  return len;
+}
+
+// (u f + v g) / 2^31 mod m for u, v < m and |f|, |g| <= 2^31: the sum w
+// has |w| < m 2^32; m 2^32 is added when it is negative, then
+// c = w (-m^-1) mod 2^31 makes w + c m divisible by 2^31, and the quotient
+// (< 3m) is brought below m by two conditional subtractions.
+template <class P>
+__device__ __forceinline__ void mod_update(const u32 u[8], const u32 v[8], long long f, long long g,
+                                           u32 out[8]) {
+  u32 w[9];
+  lin_comb(u, v, f, g, w);
+  const u32 neg = 0u - (w[8] >> 31);
+  u32 c = 0;
+#pragma unroll
+  for (int i = 1; i < 9; i++) w[i] = adc(w[i], P::m[i - 1] & neg, c, &c);
+  const u32 k = (w[0] * P::np) & 0x7FFFFFFFu;
+  u32 s[10];
+  u64 carry = 0;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const u64 t = (u64)k * (i < 8 ? P::m[i] : 0u) + w[i] + carry;
     s[i] = (u32)t;
     carry = t >> 32;
   }
-  s[8] = (u32)carry;
+  s[9] = (u32)carry;
+  u32 t[9];
 #pragma unroll
-  for (int i = 0; i < 8; i++) x[i] = (s[i] >> k) | (s[i + 1] << (32 - k));
-}
-
-// u = u / 2^k (plain integer), 1 <= k <= 31
-__device__ __forceinline__ void shr_limbs(u32 u[8], const u32 d[8], u32 k) {
+  for (int i = 0; i < 9; i++) t[i] = (s[i] >> 31) | (s[i + 1] << 1);
 #pragma unroll
-  for (int i = 0; i < 7; i++) u[i] = (d[i] >> k) | (d[i + 1] << (32 - k));
-  u[7] = d[7] >> k;
+  for (int rep = 0; rep < 2; rep++) {
+    u32 d[9], borrow = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) d[i] = sbb(t[i], i < 8 ? P::m[i] : 0u, borrow, &borrow);
+    const u32 keep = 0u - borrow;  // t < m: keep t
+#pragma unroll
+    for (int i = 0; i < 9; i++) t[i] = (t[i] & keep) | (d[i] & ~keep);
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) out[i] = t[i];
 }
+}  // namespace inv_detail
 
-// trailing zero bits of an even value's low word, at most 31
-__device__ __forceinline__ u32 even_shift(u32 w0) { return (u32)__builtin_ctz(w0 | 0x80000000u); }
-
-// Inversion by the binary extended Euclidean algorithm (Hankerson, Menezes
-// and Vanstone, Guide to Elliptic Curve Cryptography, Alg. 2.22, with the
-// two odd/odd cases folded into one step, and every even value divided by
-// all the factors of two of its low word at once, its coefficient by the
-// same 2^k through one multiple of m).  Invariants: x1 * A = u and
-// x2 * A = v (mod m) for the input representative A = a R; every step
-// halves u or v at least once, so the loop ends within 2 * 256 steps.  The
-// result A^-1 = a^-1 R^-1 is brought to Montgomery form a^-1 R by two
-// products with R^2.  Same value as inv_fermat (the inverse is unique);
-// inv(0) = 0.
+// 1 / a in Montgomery form: with A = a R the input representative, the GCD
+// gives A^-1 = a^-1 R^-1, brought to a^-1 R by two products with R^2.  Same
+// value as inv_fermat (the inverse is unique); inv(0) = 0.
 template <class P>
 __device__ __forceinline__ Fp<P> inv(const Fp<P>& a_in) {
-  const Fp<P> a = reduce_once<P>(a_in);  // inputs below 2m
-  if (a.is_zero()) return a;
-  u32 u[8], v[8], x1[8], x2[8];
+  using namespace inv_detail;
+  static_assert((P::m[7] >> 30) == 0 && (P::m[7] >> 29) != 0, "a 254-bit modulus");
+  const Fp<P> y = reduce_once<P>(a_in);  // inputs below 2m
+  if (y.is_zero()) return y;
+  u32 a[8], b[8], u[8], v[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) {
-    u[i] = a.v[i];
-    v[i] = P::m[i];
-    x1[i] = i == 0 ? 1u : 0u;
-    x2[i] = 0;
+    a[i] = y.v[i];
+    b[i] = P::m[i];
+    u[i] = i == 0 ? 1u : 0u;
+    v[i] = 0;
   }
-  // at most 2 * 256 steps for u, v < 2^256; the bound only guards the loop
-  for (int step = 0; step < 1024; step++) {
-    u32 u_hi = 0, v_hi = 0;
-#pragma unroll
-    for (int i = 1; i < 8; i++) {
-      u_hi |= u[i];
-      v_hi |= v[i];
-    }
-    if ((u_hi == 0 && u[0] == 1) || (v_hi == 0 && v[0] == 1)) break;
-    if ((u[0] & 1u) == 0) {
-      const u32 k = even_shift(u[0]);
-      shr_limbs(u, u, k);
-      div2k_mod<P>(x1, k);
-    } else if ((v[0] & 1u) == 0) {
-      const u32 k = even_shift(v[0]);
-      shr_limbs(v, v, k);
-      div2k_mod<P>(x2, k);
-    } else {
-      // both odd: subtract the smaller from the larger (the difference is even)
-      u32 d[8], borrow = 0;
-#pragma unroll
-      for (int i = 0; i < 8; i++) d[i] = sbb(u[i], v[i], borrow, &borrow);
-      if (!borrow) {
-        const u32 k = even_shift(d[0]);
-        shr_limbs(u, d, k);
-        sub_mod_limbs<P>(x1, x2);
-        div2k_mod<P>(x1, k);
-      } else {
-        borrow = 0;
-#pragma unroll
-        for (int i = 0; i < 8; i++) d[i] = sbb(v[i], u[i], borrow, &borrow);
-        const u32 k = even_shift(d[0]);
-        shr_limbs(v, d, k);
-        sub_mod_limbs<P>(x2, x1);
-        div2k_mod<P>(x2, k);
+  for (u32 round = 0; round < kRounds; round++) {
+    u32 n = bit_len2(a, b);
+    n = n < 64 ? 64 : n;
+    u64 A = (u64)(a[0] & 0x7FFFFFFFu) | (window64(a, n - 33) << 31);
+    u64 B = (u64)(b[0] & 0x7FFFFFFFu) | (window64(b, n - 33) << 31);
+    // the matrix entries in 32-bit two's complement: |f| + |g| <= 2^31 in
+    // each row, so the one value that does not fit, +2^31, comes with a 0
+    // beside it; read back as -2^31 it negates its row's result, which the
+    // sign correction below undoes (a zero result stays zero)
+    u32 f0 = 1, g0 = 0, f1 = 0, g1 = 1;
+#pragma unroll 1
+    for (int j = 0; j < 31; j++) {
+      if (A & 1) {
+        if (A < B) {
+          const u64 tA = A;
+          A = B;
+          B = tA;
+          const u32 tf = f0, tg = g0;
+          f0 = f1;
+          g0 = g1;
+          f1 = tf;
+          g1 = tg;
+        }
+        A -= B;
+        f0 -= f1;
+        g0 -= g1;
       }
+      A >>= 1;
+      f1 <<= 1;
+      g1 <<= 1;
+    }
+    long long F0 = (int)f0, G0 = (int)g0, F1 = (int)f1, G1 = (int)g1;
+    u32 ra[9], rb[9], na[8], nb[8];
+    lin_comb(a, b, F0, G0, ra);
+    lin_comb(a, b, F1, G1, rb);
+    if (abs_shr31(ra, na)) {
+      F0 = -F0;
+      G0 = -G0;
+    }
+    if (abs_shr31(rb, nb)) {
+      F1 = -F1;
+      G1 = -G1;
+    }
+    u32 nu[8], nv[8];
+    mod_update<P>(u, v, F0, G0, nu);
+    mod_update<P>(u, v, F1, G1, nv);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      a[i] = na[i];
+      b[i] = nb[i];
+      u[i] = nu[i];
+      v[i] = nv[i];
     }
   }
-  u32 u_hi = 0;
-#pragma unroll
-  for (int i = 1; i < 8; i++) u_hi |= u[i];
-  const bool use1 = u_hi == 0 && u[0] == 1;
   Fp<P> r;
 #pragma unroll
-  for (int i = 0; i < 8; i++) r.v[i] = use1 ? x1[i] : x2[i];
+  for (int i = 0; i < 8; i++) r.v[i] = v[i];
   return mul<P>(mul<P>(r, Fp<P>::r2()), Fp<P>::r2());
 }
 
