@@ -343,9 +343,11 @@ def msm_graph_replay(E):
 
 
 def single_msm(E):
+    # (16384, 8, 0): 32 windows of 4 sort tiles each
     for n, c, target, task, kind in ((1, 8, 0, 64, 0), (7, 13, 0, 64, 0), (1000, 10, 1, 16, 1),
                                      (4096, 16, 0, 64, 2), (4096, 12, 2, 32, 0),
-                                     (1024, 17, 1, 32, 0), (512, 20, 1, 32, 2)):
+                                     (1024, 17, 1, 32, 0), (512, 20, 1, 32, 2),
+                                     (16384, 8, 0, 64, 0)):
         bases = u32(n * 16)
         E.call("impeller_gen_bases", ptr(bases), n, 77, 0, None)
         nt = int(E.call("impeller_msm_tables", c, target))
