@@ -393,6 +393,11 @@ def field_inverse(E):
         vals = [0, 1, 2, m - 1, m - 2, (m + 1) // 2, 1 << 200, (1 << 253) % m]
         rng = random.Random(5 + f)
         vals += [rng.randrange(m) for _ in range(56)]
+        # every bit length, long runs of trailing zeros, values next to m
+        vals += [1 << k for k in range(254)] + [(1 << k) - 1 for k in range(1, 254)]
+        vals += [m - (1 << k) for k in range(0, 253, 5)]
+        vals += [rng.randrange(1, 1 << 40) << rng.randrange(0, 200) for _ in range(300)]
+        vals += [rng.randrange(1 << rng.randrange(1, 255)) % m for _ in range(300)]
         a = O.ints_to_array([v % m for v in vals])
         out = u32(a.size)
         E.call("impeller_field_op", f, 7, ptr(out), ptr(flat(a)), None, len(vals), None)

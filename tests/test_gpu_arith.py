@@ -49,6 +49,30 @@ def test_field_ops_vs_oracle(stream, f):
     assert (field_op(stream, f, 7, a[:k]) == np.stack([O.inv(a[i], f) for i in range(k)])).all()
 
 
+@pytest.mark.parametrize("f", [O.FR, O.FQ])
+def test_inverse_many_values(stream, f):
+    """The single-lane inversion (optimized binary GCD, 17 rounds of 31
+    steps on 64-bit approximations) on 12 000 inputs: random values, values
+    of every bit length, powers of two and their neighbours, values next to
+    m and values with long runs of trailing zero bits.  The input A is the
+    Montgomery representative of a (A = a R); the output is a^-1 R, i.e.
+    A^-1 R^2 mod m (and 0 for 0)."""
+    m = O.modulus(f)
+    R = 1 << 256
+    rng = random.Random(31 + f)
+    vals = [0, 1, 2, 3, m - 1, m - 2, (m + 1) // 2, (1 << 64) - 1, 1 << 64, 1 << 63, (1 << 253)]
+    vals += [1 << k for k in range(254)] + [(1 << k) - 1 for k in range(1, 254)]
+    vals += [m - (1 << k) for k in range(0, 253, 3)]
+    vals += [rng.randrange(1, 1 << 40) << rng.randrange(0, 200) for _ in range(1000)]
+    vals += [rng.randrange(1 << rng.randrange(1, 255)) for _ in range(3000)]
+    vals += [rng.randrange(m) for _ in range(12000 - len(vals))]
+    vals = [v % m for v in vals]
+    got = field_op(stream, f, 7, O.ints_to_array(vals))
+    for i, v in enumerate(vals):
+        exp = 0 if v == 0 else pow(v, -1, m) * R * R % m
+        assert O.limbs_to_int(got[i]) == exp, (f, hex(v))
+
+
 def test_field_kat_on_device(stream):
     kat = json.load(open(os.path.join(GOLD, "field_kat.json")))["instances"]
     for inst in ("instance1", "instance2"):
