@@ -484,8 +484,8 @@ __device__ __forceinline__ void zero_body(const MsmJob& J, u64 i) {
 
 // bucket run bounds in the sorted keys (bucket 0, the zero digits, first:
 // its end is their count): work-group lb covers 4096 consecutive keys of one
-// physical window; lanes read consecutive keys (and their neighbours, which
-// the cache already holds), so every load instruction is one coalesced run
+// physical window, thread t the run of 16 keys [16 t, 16 t + 16) of it, read
+// with four 16-byte loads plus the keys just before and after the run
 static const u32 kBoundsSpan = 4096;
 
 __host__ __device__ inline u64 bounds_blocks_per_window(const MsmShape& s) {
@@ -496,20 +496,29 @@ __device__ __forceinline__ void bounds_body(const MsmJob& J, u64 lb) {
   const MsmShape& s = J.s;
   const u64 bpw = bounds_blocks_per_window(s);
   const u32 pw = (u32)(lb / bpw);
-  const u64 i0 = (lb - (u64)pw * bpw) * kBoundsSpan;
+  const u64 i0 = (lb - (u64)pw * bpw) * kBoundsSpan + 16 * threadIdx.x;
   if (pw >= s.n_windows || i0 >= s.seg_len) return;
   const u32* kk = sorted_keys(J) + (u64)pw * s.seg_len;
   u32* bs = J.bstart + (u64)pw * (s.nb + 1);
   u32* be = J.bend + (u64)pw * (s.nb + 1);
-#pragma unroll 4
-  for (u32 j = 0; j < kBoundsSpan / 256; j++) {
-    const u64 i = i0 + (u64)j * 256 + threadIdx.x;
-    if (i >= s.seg_len) break;
-    const u32 cur = gld(kk + i);
-    const u32 before = i ? gld(kk + i - 1) : 0xffffffffu;
-    const u32 after = i + 1 < s.seg_len ? gld(kk + i + 1) : 0xffffffffu;
-    if (cur != before) gst(bs + cur, (u32)i);
-    if (cur != after) gst(be + cur, (u32)(i + 1));
+  const u32 n = s.seg_len - i0 < 16 ? (u32)(s.seg_len - i0) : 16u;
+  u32 k[16];
+  if (n == 16 && aligned16(kk)) {
+    gld16(kk + i0, k);
+  } else {
+#pragma unroll
+    for (u32 j = 0; j < 16; j++) k[j] = j < n ? gld(kk + i0 + j) : 0u;
+  }
+  const u32 before = i0 ? gld(kk + i0 - 1) : 0xffffffffu;
+  const u32 after = i0 + n < s.seg_len ? gld(kk + i0 + n) : 0xffffffffu;
+#pragma unroll
+  for (u32 j = 0; j < 16; j++) {
+    if (j < n) {
+      const u32 prev = j ? k[j - 1] : before;
+      const u32 next = j + 1 < n ? k[j + 1] : after;
+      if (k[j] != prev) gst(bs + k[j], (u32)(i0 + j));
+      if (k[j] != next) gst(be + k[j], (u32)(i0 + j + 1));
+    }
   }
 }
 
