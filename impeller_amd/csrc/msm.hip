@@ -281,30 +281,36 @@ __device__ __forceinline__ void gld16(const u32* p, u32 v[16]) {
 // LSD radix sort, 8-bit digits, stable per pass.  Tile histograms go to
 // hist[(pw * 256 + digit) * tiles + tile]; their exclusive scan in that order
 // is each (digit, tile) run's global start.
+// Digit counts of one tile: thread t counts its run of 16 keys [16 t,
+// 16 t + 16), read with four 16-byte loads, into its wave's own 256 counters
+// (atomics of different waves never meet), and the four waves' counters are
+// summed per digit at the end.  cnt holds 4 x 256 words.
 __device__ __forceinline__ void hist_body(const MsmJob& J, u32 pass, u64 lb, u32* cnt) {
   const MsmShape& s = J.s;
   const u32 tile = (u32)(lb % s.tiles), pw = (u32)(lb / s.tiles), t = threadIdx.x;
   const u32 shift = kRadixBits * pass;
   const u32* keys = (pass & 1) ? J.kB : J.kA;
   __syncthreads();
-  for (u32 b = t; b < kRadix; b += 256) cnt[b] = 0;
+  for (u32 b = t; b < 4 * kRadix; b += 256) cnt[b] = 0;
   __syncthreads();
   const u32* kk = keys + (u64)pw * s.seg_len;
-  const u64 tile0 = (u64)tile * kSortTile;
-  // all loads first (one wait), then the LDS counts
-  u32 k[kSortTile / 256];
+  const u64 i0 = (u64)tile * kSortTile + 16 * t;
+  const u32 n = i0 < s.seg_len ? (s.seg_len - i0 < 16 ? (u32)(s.seg_len - i0) : 16u) : 0u;
+  u32 k[16];
+  if (n == 16 && aligned16(kk)) {
+    gld16(kk + i0, k);
+  } else {
 #pragma unroll
-  for (u32 j = 0; j < kSortTile / 256; j++) {
-    const u64 idx = tile0 + (u64)j * 256 + t;
-    k[j] = idx < s.seg_len ? gld(kk + idx) : 0u;
+    for (u32 j = 0; j < 16; j++) k[j] = j < n ? gld(kk + i0 + j) : 0u;
   }
+  u32* mine = cnt + (t >> 6) * kRadix;
 #pragma unroll
-  for (u32 j = 0; j < kSortTile / 256; j++) {
-    const u64 idx = tile0 + (u64)j * 256 + t;
-    if (idx < s.seg_len) atomicAdd(&cnt[(k[j] >> shift) & (kRadix - 1)], 1u);
-  }
+  for (u32 j = 0; j < 16; j++)
+    if (j < n) atomicAdd(&mine[(k[j] >> shift) & (kRadix - 1)], 1u);
   __syncthreads();
-  for (u32 b = t; b < kRadix; b += 256) gst(J.hist + ((u64)pw * kRadix + b) * s.tiles + tile, cnt[b]);
+  for (u32 b = t; b < kRadix; b += 256)
+    gst(J.hist + ((u64)pw * kRadix + b) * s.tiles + tile,
+        cnt[b] + cnt[kRadix + b] + cnt[2 * kRadix + b] + cnt[3 * kRadix + b]);
 }
 
 __device__ __forceinline__ u32 block_excl_scan(u32 v, u32* part, u32* total);
@@ -1060,7 +1066,7 @@ __global__ __launch_bounds__(256) void kb_digits(const MsmJob* jobs, u32 njobs, 
 }
 __global__ __launch_bounds__(256) void kb_hist(const MsmJob* jobs, u32 njobs, const u32* prefix,
                                                u32 pass) {
-  __shared__ u32 cnt[kRadix];
+  __shared__ u32 cnt[4 * kRadix];
   IMP_BATCH_LOOP_XCD(hist_body(J, pass, lb, cnt))
 }
 __global__ __launch_bounds__(256) void kb_scan_a(const MsmJob* jobs, u32 njobs,
