@@ -111,7 +111,7 @@ static MsmShape msm_shape(u64 N, u32 c, u32 target_windows, u32 task = 64) {
 
 struct MsmLayout {
   size_t keysA, valsA, keysB, valsB, hist, bstart, bend, csum, toff, loff, partA, partB, rparts,
-      sparts, sparts1, conv, total;
+      sparts, sparts1, maxnt, conv, total;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -135,6 +135,7 @@ static MsmLayout msm_layout(const MsmShape& s) {
   l.rparts = o; o = align256(o + (size_t)s.n_windows * 2 * s.nseg * kPt * 4);
   l.sparts = o; o = align256(o + (size_t)s.n_windows * (s.lbits + 1) * s.g0 * kPt * 4);
   l.sparts1 = o; o = align256(o + (size_t)s.n_windows * (s.lbits + 1) * kPt * 4);
+  l.maxnt = o; o = align256(o + (size_t)s.n_windows * 4);
   // one table (the plain bases, affine R-form) is converted per call
   l.conv = o; o = align256(o + (s.n_pre == 1 ? (size_t)s.N * 64 : 0));
   l.total = o;
@@ -153,6 +154,10 @@ struct MsmJob {
   u32* result;
   u32 *kA, *vA, *kB, *vB, *hist, *bstart, *bend, *csum, *toff, *loff, *partA, *partB, *rparts,
       *sparts, *sparts1;
+  // the most tasks any bucket of a window has (set by the task-offset scan):
+  // combine level x has work only where some bucket has more than
+  // 2^(g (x - 1)) tasks, and its scans and combine return at once otherwise
+  u32* maxnt;
   // bucket sums kept across the parts of one MSM (n_windows x nb XYZZ, or
   // null): the accumulate stage stores (bmode 1, the first part) or adds
   // (bmode 2) this part's bucket totals, and the reduce stage of the last
@@ -190,6 +195,7 @@ static MsmJob make_msm_job(const MsmShape& s, u64 tstride, const u32* bases, con
   J.rparts = (u32*)(ws + l.rparts);
   J.sparts = (u32*)(ws + l.sparts);
   J.sparts1 = (u32*)(ws + l.sparts1);
+  J.maxnt = (u32*)(ws + l.maxnt);
   J.bacc = nullptr;
   J.bmode = 0;
   return J;
@@ -486,6 +492,7 @@ __device__ __forceinline__ void zero_body(const MsmJob& J, u64 i) {
   u64 nbw = (u64)J.s.n_windows * (J.s.nb + 1);
   if (i < nbw) J.bstart[i] = 0;
   else if (i < 2 * nbw) J.bend[i - nbw] = 0;
+  else if (i < 2 * nbw + J.s.n_windows) J.maxnt[i - 2 * nbw] = 0;
 }
 
 // bucket run bounds in the sorted keys (bucket 0, the zero digits, first:
@@ -554,6 +561,12 @@ __device__ __forceinline__ u32 level_groups(u32 nt, u32 x, u32 g) {
   return nprev > 1 ? (u32)(((u64)nt + (1ull << sh) - 1) >> sh) : 0u;
 }
 
+// combine level x (>= 1) of window pw has no bucket to work on: no bucket
+// has more than 2^(g (x - 1)) tasks (level_groups is then 0 everywhere)
+__device__ __forceinline__ bool level_idle(const MsmJob& J, u32 x, u32 pw) {
+  return gld(J.maxnt + pw) <= (1u << (J.s.g * (x - 1)));
+}
+
 __device__ __forceinline__ u32 scan_in(const MsmJob& J, u32 kind, u32 x, u32 pw, u64 i) {
   const MsmShape& s = J.s;
   if (kind == SC_HIST) return J.hist[(u64)pw * kRadix * s.tiles + i];
@@ -604,19 +617,33 @@ __device__ __forceinline__ u32* scan_out(const MsmJob& J, u32 kind, u32 pw) {
 __device__ __forceinline__ void scan_a_body(const MsmJob& J, u32 kind, u32 x, u64 lb, u32* part) {
   const u32 nch = scan_chunks(J.s, kind);
   const u32 pw = (u32)(lb / nch), ch = (u32)(lb % nch);
+  if (kind == SC_LEVEL && level_idle(J, x, pw)) return;  // uniform over the work-group
   const u64 len = scan_len(J.s, kind);
   const u64 base = (u64)ch * kScanChunk + (u64)threadIdx.x * 16;
   u32 v[16];
   scan_vals16(J, kind, x, pw, base, len, v);
-  u32 sum = 0;
+  u32 sum = 0, mx = 0;
 #pragma unroll
-  for (u32 j = 0; j < 16; j++) sum += v[j];
+  for (u32 j = 0; j < 16; j++) {
+    sum += v[j];
+    mx = v[j] > mx ? v[j] : mx;
+  }
+  if (kind == SC_TASKS) {
+    // the window's largest task count: a wave maximum, one atomic per wave
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+      const u32 o = (u32)__shfl_xor((int)mx, d);
+      mx = o > mx ? o : mx;
+    }
+    if ((threadIdx.x & 63) == 0 && mx > 1) atomicMax(J.maxnt + pw, mx);
+  }
   u32 total;
   block_excl_scan(sum, part, &total);
   if (threadIdx.x == 0) J.csum[(u64)pw * (J.s.chunks + 1) + ch] = total;
 }
 
-__device__ __forceinline__ void scan_b_body(const MsmJob& J, u32 kind, u64 pw, u32* part) {
+__device__ __forceinline__ void scan_b_body(const MsmJob& J, u32 kind, u32 x, u64 pw, u32* part) {
+  if (kind == SC_LEVEL && level_idle(J, x, (u32)pw)) return;
   const u32 nch = scan_chunks(J.s, kind);
   u32* cs = J.csum + pw * (J.s.chunks + 1);
   scan_body(cs, cs, nch, true, part);
@@ -627,6 +654,7 @@ __device__ __forceinline__ void scan_b_body(const MsmJob& J, u32 kind, u64 pw, u
 __device__ __forceinline__ void scan_c_body(const MsmJob& J, u32 kind, u32 x, u64 lb, u32* part) {
   const u32 nch = scan_chunks(J.s, kind);
   const u32 pw = (u32)(lb / nch), ch = (u32)(lb % nch);
+  if (kind == SC_LEVEL && level_idle(J, x, pw)) return;
   const u64 len = scan_len(J.s, kind);
   const u64 base = (u64)ch * kScanChunk + (u64)threadIdx.x * 16;
   u32 v[16];
@@ -761,7 +789,7 @@ __device__ __forceinline__ void combine_body(const MsmJob& J, u32 level, u64 gi)
   if (level > s.levels) return;
   const u64 cap = combine_cap(s, level);
   const u32 pw = (u32)(gi / cap);
-  if (pw >= s.n_windows) return;
+  if (pw >= s.n_windows || level_idle(J, level, pw)) return;
   const u32 o = (u32)(gi - (u64)pw * cap);
   const u32* lo = J.loff + (u64)pw * (s.nb + 2);
   if (o >= lo[s.nb + 1]) return;
@@ -1077,7 +1105,7 @@ __global__ __launch_bounds__(256) void kb_scan_a(const MsmJob* jobs, u32 njobs,
 __global__ __launch_bounds__(256) void kb_scan_b(const MsmJob* jobs, u32 njobs,
                                                  const u32* prefix, u32 arg) {
   __shared__ u32 part[256];
-  IMP_BATCH_LOOP(scan_b_body(J, arg & 255, lb, part))
+  IMP_BATCH_LOOP(scan_b_body(J, arg & 255, arg >> 8, lb, part))
 }
 __global__ __launch_bounds__(256) void kb_scan_c(const MsmJob* jobs, u32 njobs,
                                                  const u32* prefix, u32 arg) {
@@ -1169,7 +1197,7 @@ static u32 stage_blocks(const MsmShape& s, u32 kind, u32 arg) {
       if (!live) return 0;
       return kind == MS_SCAN_B ? s.n_windows : s.n_windows * scan_chunks(s, sk);
     }
-    case MS_ZERO: return (u32)cdiv(2ull * s.n_windows * (s.nb + 1), 256);
+    case MS_ZERO: return (u32)cdiv(2ull * s.n_windows * (s.nb + 1) + s.n_windows, 256);
     case MS_BOUNDS: return (u32)(s.n_windows * bounds_blocks_per_window(s));
     case MS_TASKS: return (u32)cdiv((u64)s.n_windows * chunks_per_window(s), 256);
     case MS_COMBINE:

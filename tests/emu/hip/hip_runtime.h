@@ -97,6 +97,12 @@ inline unsigned atomicAnd(unsigned* p, unsigned v) {
 inline unsigned atomicOr(unsigned* p, unsigned v) {
   return __atomic_fetch_or(p, v, __ATOMIC_RELAXED);
 }
+inline unsigned atomicMax(unsigned* p, unsigned v) {
+  unsigned cur = __atomic_load_n(p, __ATOMIC_RELAXED);
+  while (cur < v && !__atomic_compare_exchange_n(p, &cur, v, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+  }
+  return cur;
+}
 inline unsigned long long __ballot(int pred) { return ::emu::wave_ballot(pred); }
 inline unsigned __lane_id() { return ::emu::lane_id(); }
 inline int __shfl(int v, int src, int = 64) {
@@ -104,6 +110,9 @@ inline int __shfl(int v, int src, int = 64) {
 }
 inline unsigned __shfl(unsigned v, int src, int = 64) {
   return (unsigned)::emu::wave_shfl(v, src);
+}
+inline int __shfl_xor(int v, int mask, int = 64) {
+  return (int)(unsigned)::emu::wave_shfl((unsigned)v, (int)(::emu::lane_id() ^ (unsigned)mask));
 }
 inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
 inline unsigned __popcll(unsigned long long x) { return (unsigned)__builtin_popcountll(x); }
