@@ -682,8 +682,13 @@ __device__ __forceinline__ void scan_c_body(const MsmJob& J, u32 kind, u32 x, u6
   }
 }
 
-// largest b in [1, nb] with toff[b] <= t
+// largest b in [1, nb] with toff[b] <= t (toff nondecreasing over [1, nb],
+// toff[nb + 1] its total).  First guess b = t + 1, exact when every bucket
+// up to there has one group (the common case at combine level 1, where a
+// bucket's few tasks form one group): two loads instead of a binary search
+// of log2(nb) dependent loads.
 __device__ __forceinline__ u32 find_bucket(const u32* toff, u32 nb, u32 t) {
+  if (t + 1 <= nb && toff[t + 1] <= t && (t + 1 == nb || toff[t + 2] > t)) return t + 1;
   u32 lo = 1, hi = nb;
   while (lo < hi) {
     u32 mid = (lo + hi + 1) >> 1;
