@@ -232,33 +232,63 @@ __device__ __forceinline__ u32 bucket_levels(u32 nt, u32 g) {
 // positive, so the top window never carries out).  Logical window w goes to
 // physical window w % n_windows and table w / n_windows (msm_impl.cuh:56-67).
 // key = bucket |digit|, val = point index (table * tstride + i) | sign << 31.
-__device__ __forceinline__ void digits_body(const MsmJob& J, u64 i) {
+__device__ __forceinline__ bool aligned16(const u32* p) { return ((uintptr_t)p & 15) == 0; }
+
+// Signed c-bit digits of the scalars of points i0 .. i0 + 3 (thread-wise):
+// each scalar (as an integer) is shifted right by c after every window, so
+// window w's bits are always the low bits of limb 0 (no indexing into the
+// limbs by a run-time position); each slot's four keys and four values go
+// out as one 16-byte store each when aligned.
+__device__ __forceinline__ void digits_body(const MsmJob& J, u64 i0) {
   const MsmShape& s = J.s;
-  if (i >= s.N) return;
-  Fr v = to_number<FrParams>(load_fe<Fr>(J.scalars + 8 * i));
+  if (i0 >= s.N) return;
+  const u32 np = s.N - i0 < 4 ? (u32)(s.N - i0) : 4u;
+  u32 v[4][8];
+#pragma unroll
+  for (u32 p = 0; p < 4; p++) {
+    const Fr x = p < np ? to_number<FrParams>(load_fe<Fr>(J.scalars + 8 * (i0 + p))) : Fr::zero();
+#pragma unroll
+    for (u32 l = 0; l < 8; l++) v[p][l] = x.v[l];
+  }
   const u32 c = s.c, half = 1u << (c - 1), mask = (1u << c) - 1;
-  u32 carry = 0;
+  u32 carry[4] = {0, 0, 0, 0};
   const u32 slots = s.n_windows * s.n_pre;
   for (u32 w = 0; w < slots; w++) {
-    u32 pw = w % s.n_windows, grp = w / s.n_windows;
-    u64 idx = (u64)pw * s.seg_len + (u64)grp * s.N + i;
-    u32 key = 0, sign = 0;
-    if (w < s.W) {
-      u32 bit = w * c, limb = bit >> 5, off = bit & 31;
-      u64 word = v.v[limb];
-      if (limb + 1 < 8) word |= (u64)v.v[limb + 1] << 32;
-      u32 d = (u32)((word >> off) & mask) + carry;
-      if (d > half) {
-        key = (1u << c) - d;
-        sign = 0x80000000u;
-        carry = 1;
-      } else {
-        key = d;
-        carry = 0;
+    const u32 pw = w % s.n_windows, grp = w / s.n_windows;
+    const u64 idx = (u64)pw * s.seg_len + (u64)grp * s.N + i0;
+    u32 key[4], val[4];
+#pragma unroll
+    for (u32 p = 0; p < 4; p++) {
+      u32 k = 0, sign = 0;
+      if (w < s.W) {
+        const u32 d = (v[p][0] & mask) + carry[p];
+        if (d > half) {
+          k = (1u << c) - d;
+          sign = 0x80000000u;
+          carry[p] = 1;
+        } else {
+          k = d;
+          carry[p] = 0;
+        }
+#pragma unroll
+        for (u32 l = 0; l < 7; l++) v[p][l] = (v[p][l] >> c) | (v[p][l + 1] << (32 - c));
+        v[p][7] >>= c;
+      }
+      key[p] = k;
+      val[p] = (u32)((u64)grp * J.tstride + i0 + p) | sign;
+    }
+    if (np == 4 && aligned16(J.kA + idx) && aligned16(J.vA + idx)) {
+      gst(reinterpret_cast<uint4*>(J.kA + idx), make_uint4(key[0], key[1], key[2], key[3]));
+      gst(reinterpret_cast<uint4*>(J.vA + idx), make_uint4(val[0], val[1], val[2], val[3]));
+    } else {
+#pragma unroll
+      for (u32 p = 0; p < 4; p++) {
+        if (p < np) {
+          gst(J.kA + idx + p, key[p]);
+          gst(J.vA + idx + p, val[p]);
+        }
       }
     }
-    gst(J.kA + idx, key);
-    gst(J.vA + idx, (u32)((u64)grp * J.tstride + i) | sign);
   }
 }
 
@@ -271,7 +301,6 @@ __device__ __forceinline__ void convert_body(const MsmJob& J, u64 i) {
 // 16 consecutive words from p (16-byte aligned) in four 16-byte loads: a
 // thread's run of 16 keys or counts, read with a quarter of the instructions
 // that word loads would take (lanes 64 bytes apart).
-__device__ __forceinline__ bool aligned16(const u32* p) { return ((uintptr_t)p & 15) == 0; }
 __device__ __forceinline__ void gld16(const u32* p, u32 v[16]) {
   const uint4* q = reinterpret_cast<const uint4*>(p);
 #pragma unroll
@@ -1095,7 +1124,7 @@ __global__ __launch_bounds__(256) void kb_convert(const MsmJob* jobs, u32 njobs,
   IMP_BATCH_LOOP(convert_body(J, lb * blockDim.x + threadIdx.x))
 }
 __global__ __launch_bounds__(256) void kb_digits(const MsmJob* jobs, u32 njobs, const u32* prefix) {
-  IMP_BATCH_LOOP(digits_body(J, lb * blockDim.x + threadIdx.x))
+  IMP_BATCH_LOOP(digits_body(J, 4 * (lb * blockDim.x + threadIdx.x)))
 }
 __global__ __launch_bounds__(256) void kb_hist(const MsmJob* jobs, u32 njobs, const u32* prefix,
                                                u32 pass) {
@@ -1191,7 +1220,7 @@ static u32 stage_blocks(const MsmShape& s, u32 kind, u32 arg) {
   if (s.N == 0) return kind == MS_FINAL ? 1u : 0u;
   switch (kind) {
     case MS_CONVERT: return s.n_pre == 1 ? (u32)cdiv(s.N, 256) : 0u;
-    case MS_DIGITS: return (u32)cdiv(s.N, 256);
+    case MS_DIGITS: return (u32)cdiv(s.N, 1024);  // four points per thread
     case MS_HIST:
     case MS_SCATTER: return arg < s.passes ? s.tiles * s.n_windows : 0;
     case MS_SCAN_A:
