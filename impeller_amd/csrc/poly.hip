@@ -798,13 +798,21 @@ static hipError_t sort256(u32* a, u32* b, u64 n, void* area, hipStream_t stream)
   u32* sched = reinterpret_cast<u32*>(A + p.sched);
   u32* r0 = reinterpret_cast<u32*>(A + p.rec0);
   u32* r1 = reinterpret_cast<u32*>(A + p.rec1);
+  // The prefix passes (which run for all but constant bytes) launch one
+  // work-group per tile, up to 8 x RS_GRID, so that many tiles' loads are in
+  // flight at once (record histogram 79 -> 58 us, scatter 132 -> 123 us per
+  // pass at 2^24); the full passes, which mostly return at once, stride
+  // over the tiles with RS_GRID work-groups, so a skipped pass costs two
+  // small launches.
   const u32 grid = std::min(p.tiles, RS_GRID);
+  const u32 pgrid = std::min(p.tiles, 8 * RS_GRID);
   const u32 egrid = (u32)std::min<u64>((n + SB - 1) / SB, RS_GRID);
   auto pass = [&](u32 slot, auto hist, auto scatter, u32* x, u32* y) -> hipError_t {
-    hipLaunchKernelGGL(hist, dim3(grid), dim3(SB), 0, stream, x, y, n, slot, sched, cnt, p.tiles);
+    const u32 g = slot < RPREFIX ? pgrid : grid;
+    hipLaunchKernelGGL(hist, dim3(g), dim3(SB), 0, stream, x, y, n, slot, sched, cnt, p.tiles);
     IMP_CHECK(hipGetLastError());
     IMP_CHECK(u32scan(cnt, (u64)p.tiles * 256, A + p.scan, stream, sched + 2 * slot));
-    hipLaunchKernelGGL(scatter, dim3(grid), dim3(SB), 0, stream, x, y, n, slot, sched, cnt,
+    hipLaunchKernelGGL(scatter, dim3(g), dim3(SB), 0, stream, x, y, n, slot, sched, cnt,
                        p.tiles);
     return hipGetLastError();
   };
