@@ -861,27 +861,43 @@ __device__ __forceinline__ int cmp_num(const Fr& a, const Fr& b) {
 }
 
 // run starts of the sorted inputs take their value and mark one table copy used
-__global__ void k_perm_lookup(const u32* sa, const u32* st, u64 n, u32* start, u32* avail,
-                              View res_table) {
+// first position of the sorted table st whose value is not below key
+// (upper: above key) within [lo, hi)
+__device__ __forceinline__ u64 table_bound(const u32* st, u64 lo, u64 hi, const Fr& key, bool upper) {
+  while (lo < hi) {
+    const u64 mid = lo + (hi - lo) / 2;
+    const int c = cmp_num(load_fe<Fr>(st + 8 * mid), key);
+    if (c < 0 || (upper && c == 0)) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// The table window of each block of 256 sorted inputs: win[2b] = first
+// table position not below input 256 b, win[2b + 1] = first position above
+// the block's last input (two threads per block, one search each).  Every
+// input of the block finds its value's first table copy inside the window.
+__global__ void k_perm_window(const u32* sa, const u32* st, u64 n, u32* win) {
+  const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x, b = t >> 1;
+  if (b * 256 >= n) return;
+  const bool upper = (t & 1) != 0;
+  const u64 k = upper ? (b * 256 + 256 < n ? b * 256 + 255 : n - 1) : b * 256;
+  win[t] = (u32)table_bound(st, 0, n, load_fe<Fr>(sa + 8 * k), upper);
+}
+
+// run starts of the sorted inputs take their value and mark their value's
+// first table copy used (the block's window narrows the search)
+__global__ void k_perm_lookup(const u32* sa, const u32* st, u64 n, const u32* win, u32* start,
+                              u32* avail, View res_table) {
   const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Fr a = load_fe<Fr>(sa + 8 * i);
   const bool s = i == 0 || cmp_num(a, load_fe<Fr>(sa + 8 * (i - 1))) != 0;
   start[i] = s ? 1u : 0u;
   if (!s) return;
-  u64 lo = 0, hi = n;
-  while (lo < hi) {
-    const u64 mid = lo + (hi - lo) / 2;
-    const int c = cmp_num(load_fe<Fr>(st + 8 * mid), a);
-    if (c < 0) {
-      lo = mid + 1;
-    } else if (c > 0) {
-      hi = mid;
-    } else {
-      avail[mid] = 0;
-      break;
-    }
-  }
+  const u64 wl = win[2 * (i >> 8)], wh = win[2 * (i >> 8) + 1];
+  const u64 p = table_bound(st, wl, wh, a, false);
+  if (p < wh && cmp_num(load_fe<Fr>(st + 8 * p), a) == 0) avail[p] = 0;
   vst(res_table, i, from_number<FrParams>(a));
 }
 
@@ -1212,8 +1228,13 @@ extern "C" hipError_t permute(void* temp_buffer, unsigned long* buffer_size, uns
   IMP_CHECK(sort256(at(pp.k1), at(pp.k2), n, T + pp.sort, stream));
   hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(256), 0, stream, at(pp.avail), n, 1u);
   IMP_CHECK(hipGetLastError());
+  // the table windows of the input blocks, in the pos area (2 words per 256
+  // inputs; pos is first written after the lookup)
+  hipLaunchKernelGGL(k_perm_window, dim3((u32)((2 * ((n + 255) / 256) + 255) / 256)), dim3(256),
+                     0, stream, at(pp.k0), at(pp.k1), n, at(pp.pos));
+  IMP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_perm_lookup, dim3(g), dim3(256), 0, stream, at(pp.k0), at(pp.k1), n,
-                     at(pp.start), at(pp.avail), rt);
+                     at(pp.pos), at(pp.start), at(pp.avail), rt);
   IMP_CHECK(hipGetLastError());
   // positions of the leftover table values, then the run starts before each row
   IMP_CHECK(hipMemcpyAsync(at(pp.pos), at(pp.avail), n * 4, hipMemcpyDeviceToDevice, stream));
