@@ -837,15 +837,10 @@ __device__ __forceinline__ void combine_body(const MsmJob& J, u32 level, u64 gi)
   u32* dst = (level & 1) ? J.partB : J.partA;
   const u64 base = (u64)pw * s.task_cap + to[b];
   const u32 G = 1u << s.g;
-  // the next partial is loaded before each addition
   G1Xyzz29 acc = load_xyzz29_g(src + kPt * (base + (u64)k * G));
   const u32 jn = n_prev - k * G < G ? n_prev - k * G : G;
-  G1Xyzz29 nxt = jn > 1 ? load_xyzz29_g(src + kPt * (base + (u64)k * G + 1)) : acc;
-  for (u32 j = 1; j < jn; j++) {
-    const G1Xyzz29 cur = nxt;
-    if (j + 1 < jn) nxt = load_xyzz29_g(src + kPt * (base + (u64)k * G + j + 1));
-    acc = xyzz29_add(acc, cur);
-  }
+  for (u32 j = 1; j < jn; j++)
+    acc = xyzz29_add(acc, load_xyzz29_g(src + kPt * (base + (u64)k * G + j)));
   store_xyzz29_g(dst + kPt * (base + k), acc);
 }
 
